@@ -1,0 +1,281 @@
+"""Python bindings for libcollatz_amd.so, the MI355X Collatz range verifier.
+
+The library is hand-written HIP for gfx950 behind the C ABI declared in include/collatz.h;
+this module only loads it with ctypes and converts results to numpy records.  There is no
+fallback: if the library is missing, or has no HIP device to run on, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcollatz_amd.so")
+MASK64 = (1 << 64) - 1
+
+OK, EINVAL, ENOMEM, EDEVICE, ECAPACITY, ENOTREADY = 0, -1, -2, -3, -4, -5
+SEM_CUDA, SEM_RUST, SEM_OPENCL = 0, 1, 2
+FLAG_OVERFLOW, FLAG_WIDE = 1, 2
+FORM_PLAIN, FORM_PERSISTENT = 0, 1
+
+# Entry points declared in include/collatz.h.
+EXPORTS = [
+    "collatz_sieve_build", "collatz_sieve_query", "collatz_sieve_copy", "collatz_sieve_free",
+    "collatz_verify_range", "collatz_verify_batch", "collatz_reference_run",
+    "collatz_server_create", "collatz_server_submit", "collatz_server_wait",
+    "collatz_server_destroy", "collatz_version", "collatz_last_error",
+]
+
+
+class SieveParams(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("mod9", C.c_uint32), ("device", C.c_int32),
+                ("reserved", C.c_uint32)]
+
+
+class SieveInfo(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("mod9", C.c_uint32), ("survivors", C.c_uint64),
+                ("bitarray_bits", C.c_uint64), ("device_bytes", C.c_uint64),
+                ("build_ms", C.c_double)]
+
+
+class Chunk(C.Structure):
+    _fields_ = [
+        ("chunk_index", C.c_uint64), ("n_checked", C.c_uint64), ("count", C.c_uint64),
+        ("stop_sum", C.c_uint64), ("stop_xor", C.c_uint64), ("max_exc", C.c_uint64 * 3),
+        ("max_start", C.c_uint64 * 2), ("overflow_count", C.c_uint64), ("hash", C.c_uint32),
+        ("flags", C.c_uint32),
+    ]
+
+
+CHUNK_DTYPE = np.dtype([
+    ("chunk_index", "<u8"), ("n_checked", "<u8"), ("count", "<u8"), ("stop_sum", "<u8"),
+    ("stop_xor", "<u8"), ("max_exc", "<u8", (3,)), ("max_start", "<u8", (2,)),
+    ("overflow_count", "<u8"), ("hash", "<u4"), ("flags", "<u4"),
+])
+assert CHUNK_DTYPE.itemsize == C.sizeof(Chunk)
+
+
+class RangeResult(C.Structure):
+    _fields_ = [("total", Chunk), ("chunks", C.POINTER(Chunk)), ("chunks_cap", C.c_uint64),
+                ("chunks_written", C.c_uint64), ("overflow_starts", C.POINTER(C.c_uint64)),
+                ("overflow_cap", C.c_uint64), ("overflow_written", C.c_uint64),
+                ("escalated", C.c_uint64), ("checks", C.c_uint64), ("device_ms", C.c_double)]
+
+
+class VerifyOpts(C.Structure):
+    _fields_ = [("chunk_log2", C.c_uint32), ("limbs", C.c_uint32), ("cutoff_log2", C.c_uint32),
+                ("form", C.c_uint32)]
+
+
+class Interval(C.Structure):
+    _fields_ = [("lo", C.c_uint64 * 2), ("hi", C.c_uint64 * 2)]
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("chunk_log2", C.c_uint32), ("limbs", C.c_uint32), ("cutoff_log2", C.c_uint32),
+                ("use_graph", C.c_uint32), ("tile_residues", C.c_uint32),
+                ("waves_per_cu", C.c_uint32), ("checks_per_refill", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class Node(C.Structure):
+    _fields_ = [("base", C.c_uint64 * 2), ("number", C.c_uint64 * 2), ("sigfig", C.c_uint32),
+                ("threepower", C.c_uint32), ("doubleeven", C.c_uint32),
+                ("numevens", C.c_uint32)]
+
+
+class RefParams(C.Structure):
+    _fields_ = [("N", C.c_uint32), ("A", C.c_uint32), ("B", C.c_uint32), ("num_bv", C.c_uint32),
+                ("semantics", C.c_uint32), ("cutoff_log2", C.c_uint32), ("device", C.c_int32),
+                ("reserved", C.c_uint32)]
+
+
+class RefResult(C.Structure):
+    _fields_ = [("complete_hash", C.c_uint32), ("pad", C.c_uint32), ("launches", C.c_uint64),
+                ("leaves", C.c_uint64), ("candidates", C.c_uint64),
+                ("bin_leaves", C.c_uint64 * 17), ("overflow_count", C.c_uint64),
+                ("overflow_starts", C.POINTER(C.c_uint64)), ("overflow_cap", C.c_uint64),
+                ("overflow_written", C.c_uint64), ("device_ms", C.c_double)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("chunk_log2", C.c_uint32), ("cutoff_log2", C.c_uint32),
+                ("max_device_bytes", C.c_uint64), ("slice_candidates", C.c_uint64),
+                ("max_inflight", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("total", Chunk), ("chunks", C.c_uint64), ("submit_ms", C.c_double),
+                ("admit_ms", C.c_double), ("done_ms", C.c_double)]
+
+
+class CollatzError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"collatz error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Loads the HIP library (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+    L = C.CDLL(path)
+    P = C.POINTER
+    L.collatz_version.restype = C.c_int
+    L.collatz_last_error.restype = C.c_char_p
+    L.collatz_sieve_build.argtypes = [P(SieveParams), P(C.c_void_p)]
+    L.collatz_sieve_query.argtypes = [C.c_void_p, P(SieveInfo)]
+    L.collatz_sieve_copy.argtypes = [C.c_void_p, P(C.c_uint64), P(C.c_uint64), P(C.c_uint8),
+                                     C.c_uint64]
+    L.collatz_sieve_free.argtypes = [C.c_void_p]
+    L.collatz_verify_range.argtypes = [C.c_void_p, P(C.c_uint64), P(C.c_uint64), P(VerifyOpts),
+                                       C.c_void_p, P(RangeResult)]
+    L.collatz_verify_batch.argtypes = [C.c_void_p, C.c_uint64, P(Interval), P(BatchOpts),
+                                       C.c_void_p, P(RangeResult)]
+    L.collatz_reference_run.argtypes = [P(RefParams), P(Node), C.c_void_p, P(RefResult)]
+    L.collatz_server_create.argtypes = [C.c_void_p, P(ServerOpts), P(C.c_void_p)]
+    L.collatz_server_submit.argtypes = [C.c_void_p, P(Interval)]
+    L.collatz_server_submit.restype = C.c_int64
+    L.collatz_server_wait.argtypes = [C.c_void_p, C.c_int64, C.c_double, P(RequestResult),
+                                      P(Chunk), C.c_uint64]
+    L.collatz_server_destroy.argtypes = [C.c_void_p]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != OK:
+        raise CollatzError(rc, load().collatz_last_error().decode())
+
+
+def u128_pair(x: int):
+    return (C.c_uint64 * 2)(x & MASK64, (x >> 64) & MASK64)
+
+
+def words_to_int(ws) -> int:
+    v = 0
+    for i, w in enumerate(ws):
+        v |= int(w) << (64 * i)
+    return v
+
+
+@dataclass
+class RangeOutput:
+    total: np.void
+    chunks: np.ndarray
+    overflow_starts: list
+    escalated: int
+    device_ms: float
+    checks: int = 0
+
+
+class Sieve:
+    """Resident survivor list of the 2^k residue sieve (collatz_sieve_build)."""
+
+    def __init__(self, k: int, mod9: bool = True, device: int = -1):
+        L = load()
+        h = C.c_void_p()
+        _check(L.collatz_sieve_build(C.byref(SieveParams(k, int(mod9), device, 0)), C.byref(h)))
+        self._h = h
+        self.k = k
+        self.mod9 = mod9
+
+    def close(self):
+        if self._h:
+            load().collatz_sieve_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> SieveInfo:
+        inf = SieveInfo()
+        _check(load().collatz_sieve_query(self._h, C.byref(inf)))
+        return inf
+
+    def survivors(self):
+        n = self.info().survivors
+        r = np.zeros(n, np.uint64)
+        num = np.zeros(n, np.uint64)
+        f = np.zeros(n, np.uint8)
+        P = C.POINTER
+        _check(load().collatz_sieve_copy(self._h, r.ctypes.data_as(P(C.c_uint64)),
+                                         num.ctypes.data_as(P(C.c_uint64)),
+                                         f.ctypes.data_as(P(C.c_uint8)), n))
+        return r, num, f
+
+    def verify_range(self, lo: int, hi: int, chunk_log2: int = 24, limbs: int = 2,
+                     cutoff_log2: int = 187, form: int = FORM_PLAIN, overflow_cap: int = 4096,
+                     stream=None) -> RangeOutput:
+        n = ((hi - 1) >> chunk_log2) - (lo >> chunk_log2) + 1
+        chunks = np.zeros(n, CHUNK_DTYPE)
+        ovf = np.zeros(2 * max(overflow_cap, 1), np.uint64)
+        res = RangeResult()
+        res.chunks = chunks.ctypes.data_as(C.POINTER(Chunk))
+        res.chunks_cap = n
+        res.overflow_starts = ovf.ctypes.data_as(C.POINTER(C.c_uint64))
+        res.overflow_cap = overflow_cap
+        opts = VerifyOpts(chunk_log2, limbs, cutoff_log2, form)
+        _check(load().collatz_verify_range(self._h, u128_pair(lo), u128_pair(hi), C.byref(opts),
+                                           stream, C.byref(res)))
+        total = np.frombuffer(bytearray(res.total), CHUNK_DTYPE)[0]
+        nov = min(res.overflow_written, overflow_cap)
+        starts = [int(ovf[2 * i]) | (int(ovf[2 * i + 1]) << 64) for i in range(nov)]
+        return RangeOutput(total, chunks, starts, res.escalated, res.device_ms, res.checks)
+
+    def verify_batch(self, intervals, chunk_log2: int = 24, limbs: int = 2,
+                     cutoff_log2: int = 187, use_graph: bool = False, tile_residues: int = 0,
+                     waves_per_cu: int = 0, checks_per_refill: int = 0, stream=None):
+        m = len(intervals)
+        ivs = (Interval * m)()
+        outs = (RangeResult * m)()
+        chunk_arrays = []
+        for i, (lo, hi) in enumerate(intervals):
+            ivs[i].lo[0], ivs[i].lo[1] = lo & MASK64, lo >> 64
+            ivs[i].hi[0], ivs[i].hi[1] = hi & MASK64, hi >> 64
+            n = ((hi - 1) >> chunk_log2) - (lo >> chunk_log2) + 1
+            arr = np.zeros(n, CHUNK_DTYPE)
+            chunk_arrays.append(arr)
+            outs[i].chunks = arr.ctypes.data_as(C.POINTER(Chunk))
+            outs[i].chunks_cap = n
+        opts = BatchOpts(chunk_log2, limbs, cutoff_log2, int(use_graph), tile_residues,
+                         waves_per_cu, checks_per_refill, 0)
+        _check(load().collatz_verify_batch(self._h, m, ivs, C.byref(opts), stream, outs))
+        result = []
+        for i in range(m):
+            total = np.frombuffer(bytearray(outs[i].total), CHUNK_DTYPE)[0]
+            result.append(RangeOutput(total, chunk_arrays[i], [], outs[i].escalated,
+                                      outs[i].device_ms, outs[i].checks))
+        return result
+
+
+def reference_run(t, N=45, A=10, B=24, num_bv=16, semantics=SEM_CUDA, cutoff_log2=100,
+                  overflow_cap=1024, device=-1, stream=None) -> RefResult:
+    """One case line (base, sigfig, threepower, number, doubleeven, numevens)."""
+    base, sigfig, tp, number, de, ne = t
+    node = Node((C.c_uint64 * 2)(base & MASK64, base >> 64),
+                (C.c_uint64 * 2)(number & MASK64, number >> 64), sigfig, tp, de, ne)
+    ovf = np.zeros(2 * max(overflow_cap, 1), np.uint64)
+    out = RefResult()
+    out.overflow_starts = ovf.ctypes.data_as(C.POINTER(C.c_uint64))
+    out.overflow_cap = overflow_cap
+    params = RefParams(N, A, B, num_bv, semantics, cutoff_log2, device, 0)
+    _check(load().collatz_reference_run(C.byref(params), C.byref(node), stream, C.byref(out)))
+    out._ovf = ovf
+    return out

@@ -1,0 +1,362 @@
+// C ABI: sieve handles and single-range verification (plain form).
+//
+// Replaces the reference's host driver around the kernel: main()'s table setup and
+// search1 (collatz_cudav7.cu:95-183, :497-563) and launchkernel (:565-649).  Unlike the
+// reference, which allocates, copies, launches and synchronises per batch of 2^16 leaves,
+// the survivor list is built and kept on the device, and one call launches the whole range.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "host_common.hpp"
+
+namespace cz {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+void chunk_from_rec(const DevRec& r, uint64_t chunk_index, uint64_t n_checked,
+                    collatz_chunk* out) {
+  std::memset(out, 0, sizeof(*out));
+  out->chunk_index = chunk_index;
+  out->n_checked = n_checked;
+  out->count = r.count;
+  out->stop_sum = r.stop_sum;
+  out->stop_xor = r.stop_xor;
+  out->overflow_count = r.overflow;
+  out->hash = r.hash;
+  if (r.count) {
+    // T-value = number - 1.
+    uint64_t m[3] = {r.maxnum[0], r.maxnum[1], r.maxnum[2]};
+    for (int i = 0; i < 3; ++i) {
+      if (m[i]-- != 0) break;
+    }
+    std::memcpy(out->max_exc, m, sizeof(m));
+    out->max_start[0] = r.max_start[0];
+    out->max_start[1] = r.max_start[1];
+    const bool wide = r.maxnum[2] != 0 || (r.maxnum[1] >> (kWideLog2 - 64)) != 0;
+    if (wide) out->flags |= COLLATZ_FLAG_WIDE;
+  }
+  if (r.overflow) out->flags |= COLLATZ_FLAG_OVERFLOW;
+}
+
+void chunk_merge(collatz_chunk* a, const collatz_chunk& b) {
+  if (b.count) {
+    const int c = cmp3h(b.max_exc, a->max_exc);
+    const bool smaller =
+        mk128(b.max_start[0], b.max_start[1]) < mk128(a->max_start[0], a->max_start[1]);
+    if (a->count == 0 || c > 0 || (c == 0 && smaller)) {
+      std::memcpy(a->max_exc, b.max_exc, sizeof(a->max_exc));
+      std::memcpy(a->max_start, b.max_start, sizeof(a->max_start));
+    }
+  }
+  a->n_checked += b.n_checked;
+  a->count += b.count;
+  a->stop_sum += b.stop_sum;
+  a->stop_xor ^= b.stop_xor;
+  a->overflow_count += b.overflow_count;
+  a->hash ^= b.hash;
+  a->flags |= b.flags;
+}
+
+Limits make_limits(int limbs, uint32_t cutoff) {
+  Limits l;
+  l.cutoff = cutoff;
+  if (limbs == 2) {
+    const uint32_t c = std::min<uint32_t>(cutoff, kWideLog2);
+    l.fast_word = 2;
+    l.fast_thr = 1ull << (c - 64);
+  } else if (cutoff >= 128) {
+    l.fast_word = 4;
+    l.fast_thr = 1ull << (cutoff - 128);
+  } else {
+    l.fast_word = 2;
+    l.fast_thr = 1ull << (cutoff - 64);
+  }
+  return l;
+}
+
+int plan_rows(uint32_t k, u128 lo, u128 hi, uint32_t C, uint32_t slot_base, RowPlan& plan,
+              uint64_t max_rows) {
+  const u128 first = lo >> C, last = (hi - 1) >> C;
+  const u128 nch = last - first + 1;
+  if (nch > (u128)0x7FFFFFFF) return fail(COLLATZ_EINVAL, "range has too many chunks");
+  plan.first_chunk = (uint64_t)first;
+  plan.nchunks = (uint64_t)nch;
+  const size_t base_rows = plan.rows.size();
+  plan.chunk_first_row.reserve(plan.chunk_first_row.size() + plan.nchunks + 1);
+  const u128 kspan = (u128)1 << k;
+  for (uint64_t cj = 0; cj < plan.nchunks; ++cj) {
+    const u128 j = first + cj;
+    const u128 a = j << C;
+    u128 c_lo = a < lo ? lo : a;
+    const u128 width = (u128)1 << C;
+    u128 c_hi = (hi - a > width) ? a + width : hi;
+    plan.n_checked.push_back((uint64_t)(c_hi - c_lo));
+    plan.chunk_first_row.push_back(plan.rows.size() - base_rows);
+    for (u128 q = c_lo >> k; q <= (c_hi - 1) >> k; ++q) {
+      if (plan.rows.size() - base_rows >= max_rows)
+        return fail(COLLATZ_EINVAL, "range decomposes into too many rows");
+      const u128 qb = q << k;
+      const u128 r_lo = c_lo > qb ? c_lo - qb : 0;
+      const u128 r_hi = (c_hi - qb) < kspan ? (c_hi - qb) : kspan;
+      RowDesc row;
+      std::memset(&row, 0, sizeof(row));
+      row.q_lo = lo64(q);
+      row.q_hi = hi64(q);
+      row.qb_lo = lo64(qb);
+      row.qb_hi = hi64(qb);
+      row.slot = slot_base + (uint32_t)cj;
+      row.qb_mod9 = (uint32_t)(qb % 9);
+      if (r_lo == 0) {
+        plan.key_lo.push_back(-1);
+      } else {
+        plan.key_lo.push_back((int64_t)plan.keys.size());
+        plan.keys.push_back((uint64_t)r_lo);
+      }
+      if (r_hi == kspan) {
+        plan.key_hi.push_back(-1);
+      } else {
+        plan.key_hi.push_back((int64_t)plan.keys.size());
+        plan.keys.push_back((uint64_t)r_hi);
+      }
+      plan.rows.push_back(row);
+    }
+  }
+  plan.chunk_first_row.push_back(plan.rows.size() - base_rows);
+  return COLLATZ_OK;
+}
+
+static bool root_pruned(const SieveRoot& r) {
+  if (r.rust && r.number <= r.base) return true;
+  if (kDenominator * r.threepower <= kNumerator * r.sigfig) return true;
+  if (r.numevens >= 2 && r.numevens % 2 == 0 &&
+      kNumerator * (r.sigfig - 1) >= kDenominator * (r.threepower - 1))
+    return true;
+  return false;
+}
+
+static uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
+
+// Leaves of the search1 tree below `root` at root.depth, sorted by residue.
+struct LeafSet {
+  DevBuf<uint64_t> res, num;
+  DevBuf<uint32_t> meta;
+  DevBuf<uint32_t> bits;   // dense bit array over the extensions h (padded)
+  uint64_t bits_bits = 0;
+  uint64_t count = 0;
+};
+
+int build_leaves(const SieveRoot& root, hipStream_t s, LeafSet& out) {
+  if (root.depth < root.sigfig || root.depth - root.sigfig > 36 || root.depth > 40)
+    return fail(COLLATZ_EINVAL, "sieve depth out of range");
+  out.count = 0;
+  const uint32_t e = root.depth - root.sigfig;
+  if (root_pruned(root) || e < 7) {
+    // Tiny trees (and pruned roots) are not worth a device pass; the bit array is empty
+    // and the leaves, if any, are produced by walking on the host.
+    std::vector<uint64_t> r, n;
+    std::vector<uint32_t> m;
+    if (!root_pruned(root)) {
+      for (uint64_t h = 0; h < (1ull << e); ++h) {
+        uint64_t base = root.base, number = root.number, p3 = 1;
+        for (uint32_t i = 0; i < root.threepower; ++i) p3 *= 3;
+        uint32_t f = root.threepower, de = root.doubleeven, ne = root.numevens;
+        bool alive = true;
+        for (uint32_t j = 0; j < e && alive; ++j) {
+          const uint32_t sd = root.sigfig + j;
+          const uint32_t bit = (h >> j) & 1;
+          const uint64_t nb = number + (bit ? p3 : 0);
+          if (bit) base |= 1ull << sd;
+          const uint32_t inplay = kNumerator * (sd + 1) >= kDenominator * f;
+          if ((nb & 1) == 0) {
+            number = nb + nb / 2; f += 1; p3 *= 3; de = (de | inplay) & ~2u; ne = 0;
+          } else if (de >= 2) {
+            alive = false;
+          } else {
+            number = nb / 2 + 1; de <<= 1; ne += 1;
+          }
+          if (!alive) break;
+          const uint32_t s1 = sd + 1;
+          if (root.rust && number <= base) alive = false;
+          else if (kDenominator * f <= kNumerator * s1) alive = false;
+          else if (ne >= 2 && ne % 2 == 0 && kNumerator * (s1 - 1) >= kDenominator * (f - 1))
+            alive = false;
+        }
+        if (alive) {
+          r.push_back(base);
+          n.push_back(number);
+          m.push_back(f | ((uint32_t)(base % 9) << 8) | (de << 12) | ((ne & 0xFF) << 16));
+        }
+      }
+    }
+    out.count = r.size();
+    CZ_HIP(out.res.alloc(std::max<size_t>(1, r.size())));
+    CZ_HIP(out.num.alloc(std::max<size_t>(1, r.size())));
+    CZ_HIP(out.meta.alloc(std::max<size_t>(1, r.size())));
+    if (!r.empty()) {
+      CZ_HIP(hipMemcpyAsync(out.res.p, r.data(), r.size() * 8, hipMemcpyHostToDevice, s));
+      CZ_HIP(hipMemcpyAsync(out.num.p, n.data(), n.size() * 8, hipMemcpyHostToDevice, s));
+      CZ_HIP(hipMemcpyAsync(out.meta.p, m.data(), m.size() * 4, hipMemcpyHostToDevice, s));
+      CZ_HIP(hipStreamSynchronize(s));
+    }
+    out.bits_bits = 0;
+    return COLLATZ_OK;
+  }
+  const uint32_t p = std::min<uint32_t>(e, 20);
+  const uint64_t np = 1ull << p;
+  DevBuf<uint64_t> pnum;
+  DevBuf<uint32_t> pmeta;
+  DevBuf<uint64_t> alive;
+  CZ_HIP(pnum.alloc(np));
+  CZ_HIP(pmeta.alloc(np));
+  const uint64_t alive_words64 = round_up(std::max<uint64_t>(1, np / 64), 512);
+  CZ_HIP(alive.alloc(alive_words64));
+  CZ_HIP(hipMemsetAsync(alive.p, 0, alive_words64 * 8, s));
+  CZ_HIP(launch_sieve_prefix(root, p, pnum.p, pmeta.p, alive.p, s));
+  const uint32_t* bits32;
+  uint64_t nwords32;
+  if (e > p) {
+    const uint64_t words = 1ull << (e - 5);
+    nwords32 = round_up(words, 1024);
+    CZ_HIP(out.bits.alloc(nwords32));
+    if (nwords32 > words)
+      CZ_HIP(hipMemsetAsync(out.bits.p + words, 0, (nwords32 - words) * 4, s));
+    CZ_HIP(launch_sieve_expand(root, p, pnum.p, pmeta.p, alive.p, out.bits.p, s));
+    bits32 = out.bits.p;
+  } else {
+    nwords32 = alive_words64 * 2;
+    CZ_HIP(out.bits.alloc(nwords32));
+    CZ_HIP(hipMemcpyAsync(out.bits.p, alive.p, nwords32 * 4, hipMemcpyDeviceToDevice, s));
+    bits32 = out.bits.p;
+  }
+  out.bits_bits = 1ull << e;
+  const uint64_t nblocks = nwords32 / 1024;
+  DevBuf<uint32_t> counts;
+  DevBuf<uint64_t> offs;
+  CZ_HIP(counts.alloc(nblocks));
+  CZ_HIP(offs.alloc(nblocks + 1));
+  CZ_HIP(launch_compact_count(bits32, nwords32, counts.p, nblocks, s));
+  CZ_HIP(launch_scan(counts.p, nblocks, offs.p, s));
+  uint64_t total = 0;
+  CZ_HIP(hipMemcpyAsync(&total, offs.p + nblocks, 8, hipMemcpyDeviceToHost, s));
+  CZ_HIP(hipStreamSynchronize(s));
+  out.count = total;
+  CZ_HIP(out.res.alloc(std::max<uint64_t>(1, total)));
+  CZ_HIP(out.num.alloc(std::max<uint64_t>(1, total)));
+  CZ_HIP(out.meta.alloc(std::max<uint64_t>(1, total)));
+  CZ_HIP(launch_compact_write(bits32, nwords32, offs.p, out.res.p, s));
+  CZ_HIP(launch_sieve_fill(root, p, pnum.p, pmeta.p, out.res.p, out.num.p, out.meta.p, total,
+                           s));
+  CZ_HIP(hipStreamSynchronize(s));
+  return COLLATZ_OK;
+}
+
+}  // namespace cz
+
+using namespace cz;
+
+extern "C" {
+
+int collatz_version(void) { return COLLATZ_ABI_VERSION; }
+
+const char* collatz_last_error(void) { return g_error.c_str(); }
+
+int collatz_sieve_build(const collatz_sieve_params* params, collatz_sieve** out) {
+  if (!params || !out) return fail(COLLATZ_EINVAL, "null argument");
+  *out = nullptr;
+  if (params->k < 8 || params->k > 38) return fail(COLLATZ_EINVAL, "k must be in [8, 38]");
+  const auto t0 = std::chrono::steady_clock::now();
+  HipDeviceGuard guard(params->device);
+  collatz_sieve* sv = new (std::nothrow) collatz_sieve();
+  if (!sv) return fail(COLLATZ_ENOMEM, "out of host memory");
+  if (hipGetDevice(&sv->device) != hipSuccess) {
+    delete sv;
+    return fail(COLLATZ_EDEVICE, "no HIP device");
+  }
+  sv->k = params->k;
+  sv->mod9 = params->mod9 ? 1 : 0;
+  // The canonical root "3 2 2 9 1 0" (collatz_cudav7.cu:6, :173).
+  SieveRoot root;
+  root.base = 3;
+  root.number = 9;
+  root.sigfig = 2;
+  root.threepower = 2;
+  root.doubleeven = 1;
+  root.numevens = 0;
+  root.depth = params->k;
+  root.rust = 0;
+  hipStream_t s = nullptr;
+  LeafSet leaves;
+  int rc = build_leaves(root, s, leaves);
+  if (rc != COLLATZ_OK) {
+    delete sv;
+    return rc;
+  }
+  sv->count = leaves.count;
+  sv->res = std::move(leaves.res);
+  sv->num = std::move(leaves.num);
+  sv->meta = std::move(leaves.meta);
+  sv->bits = std::move(leaves.bits);
+  sv->bits_bits = leaves.bits_bits;
+  uint64_t p3[kPow3Max] = {0};
+  uint64_t x = 1;
+  for (int i = 0; i <= 40; ++i) {
+    p3[i] = x;
+    x *= 3;
+  }
+  if (sv->pow3.alloc(kPow3Max) != hipSuccess ||
+      hipMemcpy(sv->pow3.p, p3, sizeof(p3), hipMemcpyHostToDevice) != hipSuccess) {
+    delete sv;
+    return fail(COLLATZ_EDEVICE, "pow3 table upload failed");
+  }
+  sv->build_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  *out = sv;
+  return COLLATZ_OK;
+}
+
+int collatz_sieve_query(const collatz_sieve* sv, collatz_sieve_info* info) {
+  if (!sv || !info) return fail(COLLATZ_EINVAL, "null argument");
+  info->k = sv->k;
+  info->mod9 = sv->mod9;
+  info->survivors = sv->count;
+  info->bitarray_bits = sv->bits_bits;
+  info->device_bytes = sv->res.bytes() + sv->num.bytes() + sv->meta.bytes() +
+                       sv->bits.bytes() + sv->pow3.bytes();
+  info->build_ms = sv->build_ms;
+  return COLLATZ_OK;
+}
+
+int collatz_sieve_copy(const collatz_sieve* sv, uint64_t* r, uint64_t* number, uint8_t* f,
+                       uint64_t cap) {
+  if (!sv) return fail(COLLATZ_EINVAL, "null sieve");
+  HipDeviceGuard guard(sv->device);
+  const uint64_t n = std::min(cap, sv->count);
+  if (n == 0) return COLLATZ_OK;
+  if (r) CZ_HIP(hipMemcpy(r, sv->res.p, n * 8, hipMemcpyDeviceToHost));
+  if (number) CZ_HIP(hipMemcpy(number, sv->num.p, n * 8, hipMemcpyDeviceToHost));
+  if (f) {
+    std::vector<uint32_t> m(n);
+    CZ_HIP(hipMemcpy(m.data(), sv->meta.p, n * 4, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < n; ++i) f[i] = (uint8_t)meta_f(m[i]);
+  }
+  return COLLATZ_OK;
+}
+
+int collatz_sieve_free(collatz_sieve* sv) {
+  if (!sv) return COLLATZ_OK;
+  HipDeviceGuard guard(sv->device);
+  delete sv;
+  return COLLATZ_OK;
+}
+
+}  // extern "C"

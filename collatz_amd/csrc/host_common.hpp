@@ -1,0 +1,146 @@
+// Host-side helpers shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/collatz.h"
+#include "internal.hpp"
+
+namespace cz {
+
+using u128 = unsigned __int128;
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+inline u128 mk128(uint64_t lo, uint64_t hi) { return ((u128)hi << 64) | lo; }
+inline uint64_t lo64(u128 x) { return (uint64_t)x; }
+inline uint64_t hi64(u128 x) { return (uint64_t)(x >> 64); }
+
+#define CZ_HIP(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e__ = (expr);                                                          \
+    if (e__ != hipSuccess)                                                            \
+      return ::cz::fail(COLLATZ_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e__)); \
+  } while (0)
+
+// Owning device buffer.
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { reset(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { reset(); }
+  hipError_t alloc(size_t count) {
+    reset();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) { p = nullptr; return e; }
+    n = count;
+    return hipSuccess;
+  }
+  // Grows (never shrinks) a scratch buffer.
+  hipError_t reserve(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    return alloc(count);
+  }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// Host-side merge with the device semantics (larger maxnum wins, ties to the smaller start).
+inline int cmp3h(const uint64_t* a, const uint64_t* b) {
+  for (int i = 2; i >= 0; --i)
+    if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+  return 0;
+}
+
+inline void rec_merge_host(DevRec& a, const DevRec& b) {
+  if (b.count) {
+    const int c = cmp3h(b.maxnum, a.maxnum);
+    const bool smaller =
+        mk128(b.max_start[0], b.max_start[1]) < mk128(a.max_start[0], a.max_start[1]);
+    if (a.count == 0 || c > 0 || (c == 0 && smaller)) {
+      std::memcpy(a.maxnum, b.maxnum, sizeof(a.maxnum));
+      std::memcpy(a.max_start, b.max_start, sizeof(a.max_start));
+    }
+  }
+  a.count += b.count;
+  a.stop_sum += b.stop_sum;
+  a.stop_xor ^= b.stop_xor;
+  a.overflow += b.overflow;
+  a.checks += b.checks;
+  a.hash ^= b.hash;
+}
+
+void chunk_from_rec(const DevRec& r, uint64_t chunk_index, uint64_t n_checked,
+                    collatz_chunk* out);
+void chunk_merge(collatz_chunk* into, const collatz_chunk& from);
+
+// Decomposition of [lo, hi) into rows (one q, one chunk, a residue window).
+struct RowPlan {
+  std::vector<RowDesc> rows;
+  std::vector<uint64_t> keys;        // residue bounds to look up
+  std::vector<int64_t> key_lo, key_hi;  // per row: index into keys, or -1 for 0 / S
+  std::vector<uint64_t> chunk_first_row;  // per chunk: first row; size nchunks + 1
+  std::vector<uint64_t> n_checked;  // per chunk
+  uint64_t first_chunk = 0;
+  uint64_t nchunks = 0;
+};
+
+int plan_rows(uint32_t k, u128 lo, u128 hi, uint32_t C, uint32_t slot_base, RowPlan& plan,
+              uint64_t max_rows);
+
+Limits make_limits(int limbs, uint32_t cutoff);
+
+struct HipDeviceGuard {
+  int prev = -1;
+  bool changed = false;
+  explicit HipDeviceGuard(int dev) {
+    if (dev >= 0 && hipGetDevice(&prev) == hipSuccess && prev != dev) {
+      changed = hipSetDevice(dev) == hipSuccess;
+    }
+  }
+  ~HipDeviceGuard() {
+    if (changed) (void)hipSetDevice(prev);
+  }
+};
+
+}  // namespace cz
+
+struct collatz_sieve {
+  int device = 0;
+  uint32_t k = 0, mod9 = 1;
+  uint64_t count = 0;
+  cz::DevBuf<uint64_t> res, num;
+  cz::DevBuf<uint32_t> meta;
+  cz::DevBuf<uint32_t> bits;
+  uint64_t bits_bits = 0;
+  cz::DevBuf<uint64_t> pow3;
+  double build_ms = 0;
+  cz::SieveArrays arrays() const {
+    cz::SieveArrays a;
+    a.res = res.p;
+    a.num = num.p;
+    a.meta = meta.p;
+    a.count = count;
+    a.k = k;
+    return a;
+  }
+};

@@ -1,0 +1,124 @@
+// Definitions shared by the host orchestration (host.cpp) and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace cz {
+
+constexpr uint32_t kNumerator = 306;    // 306/485 ~ log 2 / log 3 (collatz_cudav7.cu:50-52)
+constexpr uint32_t kDenominator = 485;
+constexpr uint32_t kWideLog2 = 123;     // values at or above 2^123 need a third u64 limb
+constexpr uint32_t kMod9Keep = 0xCBu;   // bits {0,1,3,6,7}: residues mod 9 that stay
+constexpr int kPow3Max = 64;            // 3^f tables cover f < 64
+
+// Root node of a sieve walk (a node of the reference's search1 tree,
+// collatz_cudav7.cu:497-505): n = base + h * 2^sigfig for the extension h.
+struct SieveRoot {
+  uint64_t base;        // n mod 2^sigfig
+  uint64_t number;      // T^sigfig(base) + 1
+  uint32_t sigfig;      // depth of the root
+  uint32_t threepower;  // f at the root
+  uint32_t doubleeven;  // odd-even-even state
+  uint32_t numevens;    // consecutive even steps
+  uint32_t depth;       // target depth D
+  uint32_t rust;        // main.rs:216 prune (number <= base) on
+};
+
+// Survivor / leaf meta word: f in bits 0-7, n mod 9 in bits 8-11, doubleeven in 12-13,
+// numevens in 16-23.
+__host__ __device__ inline uint32_t meta_f(uint32_t m) { return m & 0xFFu; }
+__host__ __device__ inline uint32_t meta_mod9(uint32_t m) { return (m >> 8) & 0xFu; }
+__host__ __device__ inline uint32_t meta_de(uint32_t m) { return (m >> 12) & 0x3u; }
+
+// One row of range work: residues with sieve indices [i_lo, i_hi) and one q, all inside
+// one output chunk.  n = (q << k) | r.
+struct RowDesc {
+  uint64_t q_lo, q_hi;        // q
+  uint64_t qb_lo, qb_hi;      // q << k (low k bits zero)
+  uint64_t i_lo, i_hi;        // sieve index range
+  uint64_t block_start;       // first plain-kernel block of this row
+  uint32_t slot;              // output record slot (chunk of an interval)
+  uint32_t qb_mod9;           // (q << k) mod 9
+};
+
+// Partial or final record of one chunk, as the kernels produce it.  maxnum is the max of
+// the shifted value number = T-value + 1, so a record with count == 0 has maxnum == 0.
+struct DevRec {
+  uint64_t count;
+  uint64_t stop_sum;
+  uint64_t stop_xor;
+  uint64_t maxnum[3];
+  uint64_t max_start[2];
+  uint64_t overflow;
+  uint64_t checks;   // checks executed (for the roofline accounting)
+  uint32_t hash;
+  uint32_t pad;
+};
+
+// A candidate handed from the two-limb kernels to the three-limb pass.
+struct EscEntry {
+  uint64_t n_lo, n_hi;
+  uint64_t res_index;
+  uint32_t slot;
+  uint32_t pad;
+};
+
+struct DeviceLists {
+  EscEntry* esc;              // escalation list
+  unsigned long long* esc_count;
+  uint64_t esc_cap;
+  uint64_t* ovf;              // overflow starts, 2 words each
+  unsigned long long* ovf_count;
+  uint64_t ovf_cap;
+  unsigned long long* stall_count;  // candidates abandoned after kMaxChecks checks
+};
+
+// Safety bound on checks per candidate.  Stopping times of all n < 2^128 that have been
+// computed are a few thousand T-steps; a candidate still running after this many checks
+// means a defect, and is counted instead of looping forever.
+constexpr uint32_t kMaxChecks = 1u << 16;
+
+// Limits of the schedule for one launch (cutoff = CUTOFFPOWER, collatz_cudav7.cu:44-46).
+struct Limits {
+  uint32_t cutoff;       // overflow at number >= 2^cutoff
+  uint32_t fast_word;    // index of the low word of the 64-bit pair checked on the fast path
+  uint64_t fast_thr;     // fast path: pair >= fast_thr, or any word above the pair nonzero
+};
+
+struct SieveArrays {
+  const uint64_t* res;    // residues r (sorted)
+  const uint64_t* num;    // T^k(r) + 1
+  const uint32_t* meta;
+  uint64_t count;
+  uint32_t k;
+};
+
+// ---- launchers (defined in the .hip files) ------------------------------------------
+hipError_t launch_sieve_prefix(const SieveRoot& root, uint32_t p, uint64_t* pnum,
+                               uint32_t* pmeta, uint64_t* alive_bits, hipStream_t s);
+hipError_t launch_sieve_expand(const SieveRoot& root, uint32_t p, const uint64_t* pnum,
+                               const uint32_t* pmeta, const uint64_t* alive_bits,
+                               uint32_t* bits, hipStream_t s);
+hipError_t launch_compact_count(const uint32_t* bits, uint64_t nwords, uint32_t* block_counts,
+                                uint64_t nblocks, hipStream_t s);
+hipError_t launch_scan(const uint32_t* in, uint64_t n, uint64_t* out_excl, hipStream_t s);
+hipError_t launch_compact_write(const uint32_t* bits, uint64_t nwords,
+                                const uint64_t* block_offsets, uint64_t* out_h, hipStream_t s);
+hipError_t launch_sieve_fill(const SieveRoot& root, uint32_t p, const uint64_t* pnum,
+                             const uint32_t* pmeta, uint64_t* res_inout, uint64_t* num,
+                             uint32_t* meta, uint64_t count, hipStream_t s);
+hipError_t launch_lower_bound(const uint64_t* sorted, uint64_t n, const uint64_t* keys,
+                              uint64_t nkeys, uint64_t* out, hipStream_t s);
+
+hipError_t launch_verify_plain(int limbs, const SieveArrays& sv, const RowDesc* rows,
+                               uint32_t nrows, uint64_t nblocks, uint32_t R, uint32_t mod9,
+                               const Limits& lim, const uint64_t* pow3, DevRec* block_recs,
+                               DeviceLists lists, hipStream_t s);
+hipError_t launch_reduce_chunks(const DevRec* parts, const uint64_t* part_start,
+                                uint32_t nchunks, DevRec* out, hipStream_t s);
+hipError_t launch_escalated(const SieveArrays& sv, const EscEntry* esc, uint64_t n,
+                            uint32_t mod9_unused, const Limits& lim, const uint64_t* pow3,
+                            DevRec* out, DeviceLists lists, hipStream_t s);
+
+}  // namespace cz

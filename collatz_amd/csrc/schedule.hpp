@@ -1,0 +1,255 @@
+// The iteration schedule of the reference on L 32-bit words (L = 4: two u64 limbs,
+// L = 6: three u64 limbs), plus the per-lane accumulators and wave reductions.
+//
+// The schedule follows compute_kernel (collatz_cudav7.cu:280-366):
+//   check = odd run, even run, odd run, even run; then hash ^= low word; stop if number < n
+//   odd run : a = ctz(number | 2^8), number = (number >> a) * 3^a
+//   even run: b = ctz((number - 1) | 2^31), number = ((number - 1) >> b) + 1
+// where number = T-value + 1.  The even run is computed here as
+//   number = (number + 2^b - 1) >> b,
+// which is the same integer because number - 1 is a multiple of 2^b, and which needs one
+// carry chain instead of a borrow and a conditional carry.
+#pragma once
+
+#include "internal.hpp"
+
+namespace cz {
+
+template <int L>
+struct Num {
+  uint32_t w[L];
+};
+
+__device__ __forceinline__ uint32_t alignr(uint32_t hi, uint32_t lo, uint32_t s) {
+  return __builtin_amdgcn_alignbit(hi, lo, s);  // low word of (hi:lo) >> (s & 31)
+}
+
+// 3^a for a in [0, 8]: 3^(a & 3) from packed bytes, times 81^(a >> 2).
+__device__ __forceinline__ uint32_t pow3_small(uint32_t a) {
+  const uint32_t lo = __builtin_amdgcn_ubfe(0x1B090301u, (a & 3u) << 3, 8);
+  const uint32_t hi = (a >= 8u) ? 6561u : ((a >= 4u) ? 81u : 1u);
+  return __umul24(lo, hi);
+}
+
+template <int L>
+__device__ __forceinline__ uint32_t odd_run(Num<L>& v) {
+  const uint32_t a = __builtin_ctz(v.w[0] | 0x100u);
+  const uint32_t m = pow3_small(a);
+  uint64_t t = (uint64_t)alignr(v.w[1], v.w[0], a) * m;
+  v.w[0] = (uint32_t)t;
+#pragma unroll
+  for (int i = 1; i < L - 1; ++i) {
+    t = (uint64_t)alignr(v.w[i + 1], v.w[i], a) * m + (t >> 32);
+    v.w[i] = (uint32_t)t;
+  }
+  v.w[L - 1] = (uint32_t)((uint64_t)(v.w[L - 1] >> a) * m + (t >> 32));
+  return a;
+}
+
+template <int L>
+__device__ __forceinline__ uint32_t even_run(Num<L>& v) {
+  const uint32_t b = __builtin_ctz((v.w[0] - 1u) | 0x80000000u);
+  const uint32_t add = (1u << b) - 1u;
+  uint64_t t = (uint64_t)v.w[0] + add;
+  uint32_t x[L];
+  x[0] = (uint32_t)t;
+#pragma unroll
+  for (int i = 1; i < L; ++i) {
+    t = (uint64_t)v.w[i] + (t >> 32);
+    x[i] = (uint32_t)t;
+  }
+#pragma unroll
+  for (int i = 0; i < L - 1; ++i) v.w[i] = alignr(x[i + 1], x[i], b);
+  v.w[L - 1] = x[L - 1] >> b;
+  return b;
+}
+
+// x >= y over L words.
+template <int L>
+__device__ __forceinline__ bool ge_words(const uint32_t* x, const uint32_t* y) {
+  uint64_t t = (uint64_t)x[0] - y[0];
+#pragma unroll
+  for (int i = 1; i < L; ++i) t = (uint64_t)x[i] - y[i] - (uint32_t)(t >> 63);
+  return (t >> 63) == 0;
+}
+
+// number < n where n has 4 words.
+template <int L>
+__device__ __forceinline__ bool lt_base(const Num<L>& v, const uint32_t* n) {
+  uint64_t t = (uint64_t)v.w[0] - n[0];
+#pragma unroll
+  for (int i = 1; i < 4; ++i) t = (uint64_t)v.w[i] - n[i] - (uint32_t)(t >> 63);
+  bool lt = (t >> 63) != 0;
+  if constexpr (L > 4) {
+    uint32_t hi = 0;
+#pragma unroll
+    for (int i = 4; i < L; ++i) hi |= v.w[i];
+    lt = lt && hi == 0;
+  }
+  return lt;
+}
+
+__device__ __forceinline__ uint64_t pair(uint32_t lo, uint32_t hi) {
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Fast test "number may be >= the limit": a 64-bit compare on the top word pair (plus, for
+// three limbs with a cutoff below 2^128, the top two words being nonzero).
+template <int L, bool LOWCUT>
+__device__ __forceinline__ bool over_fast(const Num<L>& v, uint64_t thr) {
+  if constexpr (L == 4) {
+    return pair(v.w[2], v.w[3]) >= thr;
+  } else if constexpr (!LOWCUT) {
+    return pair(v.w[4], v.w[5]) >= thr;
+  } else {
+    return (v.w[4] | v.w[5]) != 0 || pair(v.w[2], v.w[3]) >= thr;
+  }
+}
+
+template <int L>
+__device__ __forceinline__ bool ge_pow2(const Num<L>& v, uint32_t c) {
+  const uint32_t cw = c >> 5, cb = c & 31u;
+  bool ge = false;
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    if ((uint32_t)i > cw) ge |= v.w[i] != 0;
+    else if ((uint32_t)i == cw) ge |= (v.w[i] >> cb) != 0;
+  }
+  return ge;
+}
+
+// Per-lane accumulators over the candidates one lane finishes.
+template <int L>
+struct Acc {
+  uint32_t cnt, hash, ssum, sxor, ovf, chk;
+  uint32_t mx[L];      // max number seen (T-value + 1), 0 if none
+  uint32_t mstart[4];  // its start
+  __device__ void init() {
+    cnt = hash = ssum = sxor = ovf = chk = 0;
+#pragma unroll
+    for (int i = 0; i < L; ++i) mx[i] = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) mstart[i] = 0;
+  }
+};
+
+// Offer (value, start) to the lane max: take it if larger, or equal with a smaller start.
+template <int L>
+__device__ __forceinline__ void offer_max(Acc<L>& acc, const Num<L>& v, const uint32_t* n) {
+  if (ge_words<L>(v.w, acc.mx)) {
+    bool eq = true;
+#pragma unroll
+    for (int i = 0; i < L; ++i) eq = eq && v.w[i] == acc.mx[i];
+    const bool take = !eq || !ge_words<4>(n, acc.mstart);
+    if (take) {
+#pragma unroll
+      for (int i = 0; i < L; ++i) acc.mx[i] = v.w[i];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc.mstart[i] = n[i];
+    }
+  }
+}
+
+// ---- record merging ------------------------------------------------------------------
+__device__ __forceinline__ int cmp3(const uint64_t* a, const uint64_t* b) {
+  if (a[2] != b[2]) return a[2] < b[2] ? -1 : 1;
+  if (a[1] != b[1]) return a[1] < b[1] ? -1 : 1;
+  if (a[0] != b[0]) return a[0] < b[0] ? -1 : 1;
+  return 0;
+}
+
+__device__ __forceinline__ bool less128(uint64_t alo, uint64_t ahi, uint64_t blo, uint64_t bhi) {
+  return ahi < bhi || (ahi == bhi && alo < blo);
+}
+
+__device__ __forceinline__ void rec_zero(DevRec& r) {
+  r.count = r.stop_sum = r.stop_xor = r.overflow = r.checks = 0;
+  r.maxnum[0] = r.maxnum[1] = r.maxnum[2] = 0;
+  r.max_start[0] = r.max_start[1] = 0;
+  r.hash = 0;
+  r.pad = 0;
+}
+
+// Merge b into a.  The max goes to the larger maxnum, ties to the smaller start; records
+// without candidates carry maxnum 0, which every candidate exceeds (number >= 2).
+__device__ __forceinline__ void rec_merge(DevRec& a, const DevRec& b) {
+  const int c = cmp3(b.maxnum, a.maxnum);
+  if (c > 0 || (c == 0 && b.count && less128(b.max_start[0], b.max_start[1], a.max_start[0],
+                                             a.max_start[1]))) {
+    a.maxnum[0] = b.maxnum[0];
+    a.maxnum[1] = b.maxnum[1];
+    a.maxnum[2] = b.maxnum[2];
+    a.max_start[0] = b.max_start[0];
+    a.max_start[1] = b.max_start[1];
+  }
+  a.count += b.count;
+  a.stop_sum += b.stop_sum;
+  a.stop_xor ^= b.stop_xor;
+  a.overflow += b.overflow;
+  a.checks += b.checks;
+  a.hash ^= b.hash;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t x, int m) {
+  const uint32_t lo = __shfl_xor((uint32_t)x, m, 64);
+  const uint32_t hi = __shfl_xor((uint32_t)(x >> 32), m, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Butterfly merge across the 64 lanes of a wave; every lane ends with the wave's record.
+__device__ __forceinline__ void wave_merge(DevRec& r) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    DevRec o;
+    o.count = shfl_xor64(r.count, m);
+    o.stop_sum = shfl_xor64(r.stop_sum, m);
+    o.stop_xor = shfl_xor64(r.stop_xor, m);
+    o.overflow = shfl_xor64(r.overflow, m);
+    o.checks = shfl_xor64(r.checks, m);
+    o.maxnum[0] = shfl_xor64(r.maxnum[0], m);
+    o.maxnum[1] = shfl_xor64(r.maxnum[1], m);
+    o.maxnum[2] = shfl_xor64(r.maxnum[2], m);
+    o.max_start[0] = shfl_xor64(r.max_start[0], m);
+    o.max_start[1] = shfl_xor64(r.max_start[1], m);
+    o.hash = __shfl_xor(r.hash, m, 64);
+    rec_merge(r, o);
+  }
+}
+
+template <int L>
+__device__ __forceinline__ DevRec acc_to_rec(const Acc<L>& a) {
+  DevRec r;
+  r.count = a.cnt;
+  r.stop_sum = a.ssum;
+  r.stop_xor = a.sxor;
+  r.overflow = a.ovf;
+  r.checks = a.chk;
+  r.hash = a.hash;
+  r.pad = 0;
+  r.maxnum[0] = pair(a.mx[0], a.mx[1]);
+  r.maxnum[1] = pair(a.mx[2], a.mx[3]);
+  if constexpr (L > 4) r.maxnum[2] = pair(a.mx[4], a.mx[5]);
+  else r.maxnum[2] = 0;
+  r.max_start[0] = pair(a.mstart[0], a.mstart[1]);
+  r.max_start[1] = pair(a.mstart[2], a.mstart[3]);
+  if (a.cnt == 0) {
+    r.maxnum[0] = r.maxnum[1] = r.maxnum[2] = 0;
+    r.max_start[0] = r.max_start[1] = 0;
+  }
+  return r;
+}
+
+// Block-wide merge for blocks of `nwaves` waves; thread 0 returns the block's record.
+template <int NWAVES>
+__device__ __forceinline__ DevRec block_merge(DevRec r, DevRec* lds) {
+  wave_merge(r);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) lds[wv] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < NWAVES; ++i) rec_merge(r, lds[i]);
+  }
+  return r;
+}
+
+}  // namespace cz

@@ -1,0 +1,223 @@
+/*
+ * collatz.h -- C ABI of the MI355X Collatz range verifier (libcollatz_amd.so).
+ *
+ * Plain C types only: integers, pointers, sizes.  Streams are HIP streams passed as
+ * `void*` (NULL = the default stream).  Every entry point returns COLLATZ_OK (0) or a
+ * negative status; collatz_last_error() returns a message for the calling thread.
+ * No exceptions cross this boundary and no kernel prints.
+ *
+ * The reference (vigleik0/collatz) has no library interface; each entry point below
+ * replaces a piece of its host program, cited at the declaration.
+ */
+#ifndef COLLATZ_H
+#define COLLATZ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COLLATZ_ABI_VERSION 1
+
+enum collatz_status {
+  COLLATZ_OK = 0,
+  COLLATZ_EINVAL = -1,    /* bad argument */
+  COLLATZ_ENOMEM = -2,    /* device or host allocation failed */
+  COLLATZ_EDEVICE = -3,   /* a HIP call failed */
+  COLLATZ_ECAPACITY = -4, /* an output buffer was too small (the count is still reported) */
+  COLLATZ_ENOTREADY = -5  /* server: the request is not finished yet */
+};
+
+enum collatz_semantics {
+  COLLATZ_SEM_CUDA = 0,   /* collatz_cudav7.cu */
+  COLLATZ_SEM_RUST = 1,   /* main.rs: odd-even-even term in the bit vectors, extra filters */
+  COLLATZ_SEM_OPENCL = 2  /* collatz_openclv2.cpp: unsigned compares in precompute */
+};
+
+enum collatz_flags {
+  COLLATZ_FLAG_OVERFLOW = 1u, /* a trajectory reached 2^cutoff_log2 (reported, then reset) */
+  COLLATZ_FLAG_WIDE = 2u      /* a trajectory reached 2^123 and ran on three u64 limbs */
+};
+
+/* ---- sieve ---------------------------------------------------------------------------
+ * Replaces the reference's Step 1: search1() from the root "3 2 2 9 1 0" down to depth k
+ * (collatz_cudav7.cu:497-563, :173), evaluated on the device for every residue class,
+ * stored as a bit array and compacted into a sorted survivor list (r, T^k(r)+1, f) that
+ * stays resident in device memory. */
+typedef struct collatz_sieve collatz_sieve;
+
+typedef struct {
+  uint32_t k;       /* sieve depth: residues mod 2^k, 8 <= k <= 38 */
+  uint32_t mod9;    /* 1: apply the mod 9 preimage sieve (collatz_cudav7.cu:482-494) */
+  int32_t device;   /* HIP device ordinal, -1 = current device */
+  uint32_t reserved;
+} collatz_sieve_params;
+
+typedef struct {
+  uint32_t k, mod9;
+  uint64_t survivors;      /* |S_k| */
+  uint64_t bitarray_bits;  /* bits of the dense array (classes r = 3 mod 4) */
+  uint64_t device_bytes;   /* device memory held by the handle */
+  double build_ms;         /* wall time of collatz_sieve_build */
+} collatz_sieve_info;
+
+int collatz_sieve_build(const collatz_sieve_params* params, collatz_sieve** out);
+int collatz_sieve_query(const collatz_sieve* sieve, collatz_sieve_info* info);
+/* Copies up to cap survivors (residue, T^k(r)+1, f) to host memory; for tests and tools. */
+int collatz_sieve_copy(const collatz_sieve* sieve, uint64_t* r, uint64_t* number, uint8_t* f,
+                       uint64_t cap);
+int collatz_sieve_free(collatz_sieve* sieve);
+
+/* ---- range verification --------------------------------------------------------------
+ * One record per aligned chunk [j*2^C, (j+1)*2^C) intersected with the range.  Every
+ * n in the chunk whose residue mod 2^k survives the sieve and (with mod9) whose n mod 9
+ * is in {0,1,3,6,7} is iterated with the reference schedule (collatz_cudav7.cu:280-366)
+ * from T^k(n)+1 (Lemma 2.3) until the check number < n passes. */
+typedef struct {
+  uint64_t chunk_index;    /* j */
+  uint64_t n_checked;      /* integers of the range in this chunk */
+  uint64_t count;          /* candidates iterated */
+  uint64_t stop_sum;       /* sum over candidates of J = k + T-steps to the passing check */
+  uint64_t stop_xor;       /* xor of the same J */
+  uint64_t max_exc[3];     /* largest T-value seen (at the start or after an odd run) */
+  uint64_t max_start[2];   /* smallest start n that reached max_exc */
+  uint64_t overflow_count; /* times a value reached 2^cutoff_log2 */
+  uint32_t hash;           /* xor of the per-candidate reference hashes */
+  uint32_t flags;          /* collatz_flags */
+} collatz_chunk;
+
+typedef struct {
+  collatz_chunk total;        /* merge of all chunks; chunk_index = first chunk */
+  collatz_chunk* chunks;      /* caller-owned array, or NULL */
+  uint64_t chunks_cap;
+  uint64_t chunks_written;    /* number of chunks of the range (even if > chunks_cap) */
+  uint64_t* overflow_starts;  /* caller-owned, two words (lo, hi) per start, or NULL */
+  uint64_t overflow_cap;
+  uint64_t overflow_written;  /* starts reported (even if > overflow_cap) */
+  uint64_t escalated;         /* candidates moved to three limbs */
+  uint64_t checks;            /* checks (two odd and two even runs) executed */
+  double device_ms;           /* device time of the main verification kernel */
+} collatz_range_result;
+
+enum collatz_form {
+  COLLATZ_FORM_PLAIN = 0,     /* one lane per (residue, q): the drop-in kernel */
+  COLLATZ_FORM_PERSISTENT = 1 /* persistent waves with lane refill and work stealing */
+};
+
+typedef struct {
+  uint32_t chunk_log2;   /* C, 0 = 24 */
+  uint32_t limbs;        /* 2 (three on demand) or 3; 0 = 2 */
+  uint32_t cutoff_log2;  /* CUTOFFPOWER, 64..187; 0 = 187 */
+  uint32_t form;         /* collatz_form */
+} collatz_verify_opts;
+
+/* Replaces launchkernel() + compute_kernel (collatz_cudav7.cu:565-649, :185-411):
+ * [lo, hi) with 128-bit endpoints lo = lo[0] + 2^64 lo[1]. */
+int collatz_verify_range(const collatz_sieve* sieve, const uint64_t lo[2], const uint64_t hi[2],
+                         const collatz_verify_opts* opts, void* stream,
+                         collatz_range_result* out);
+
+/* ---- batched form --------------------------------------------------------------------
+ * Many intervals in one persistent launch (no counterpart in the reference, which runs one
+ * batch of 2^16 leaves per synchronous launch, collatz_cudav7.cu:532-533).  outs[i] gets
+ * interval i's records exactly as collatz_verify_range would produce them. */
+typedef struct {
+  uint64_t lo[2];
+  uint64_t hi[2];
+} collatz_interval;
+
+typedef struct {
+  uint32_t chunk_log2;       /* 0 = 24 */
+  uint32_t limbs;            /* 0 = 2 */
+  uint32_t cutoff_log2;      /* 0 = 187 */
+  uint32_t use_graph;        /* 1: capture the launch sequence in a hipGraph */
+  uint32_t tile_residues;    /* residues per work tile; 0 = autotuned */
+  uint32_t waves_per_cu;     /* persistent waves per CU; 0 = autotuned */
+  uint32_t checks_per_refill;/* checks between lane refills; 0 = autotuned */
+  uint32_t reserved;
+} collatz_batch_opts;
+
+int collatz_verify_batch(const collatz_sieve* sieve, uint64_t n,
+                         const collatz_interval* intervals, const collatz_batch_opts* opts,
+                         void* stream, collatz_range_result* outs);
+
+/* ---- reference mode ------------------------------------------------------------------
+ * The reference's own full program for one case line "base sigfig threepower number
+ * doubleeven numevens" (collatz_cudav7.cu:95-183): search1 to depth N - A, the BV_i
+ * look-ahead and mod 9 filter of the top A bits (Step 2, :219-235), the schedule, the
+ * complete hash and the launch count it would print. */
+typedef struct {
+  uint64_t base[2];
+  uint64_t number[2];
+  uint32_t sigfig, threepower, doubleeven, numevens;
+} collatz_node;
+
+typedef struct {
+  uint32_t N;            /* MAXBITS (N - A <= 38) */
+  uint32_t A;            /* LASTBITS, 1..12 */
+  uint32_t B;            /* STEPSIZEINITIAL, 24 */
+  uint32_t num_bv;       /* bit vectors, 16 (CUDA/OpenCL) or 8 (Rust) */
+  uint32_t semantics;    /* collatz_semantics */
+  uint32_t cutoff_log2;  /* CUTOFFPOWER, 64..187 */
+  int32_t device;
+  uint32_t reserved;
+} collatz_ref_params;
+
+typedef struct {
+  uint32_t complete_hash;     /* "Complete hash = %X" */
+  uint32_t pad;
+  uint64_t launches;          /* "Number of kernel launches = %ld" */
+  uint64_t leaves;            /* leaves at depth N - A */
+  uint64_t candidates;        /* candidates iterated */
+  uint64_t bin_leaves[17];    /* leaves per bigindex 0..15, and >= 16 */
+  uint64_t overflow_count;
+  uint64_t* overflow_starts;  /* caller-owned, two words per start, or NULL */
+  uint64_t overflow_cap;
+  uint64_t overflow_written;
+  double device_ms;
+} collatz_ref_result;
+
+int collatz_reference_run(const collatz_ref_params* params, const collatz_node* root,
+                          void* stream, collatz_ref_result* out);
+
+/* ---- served form ---------------------------------------------------------------------
+ * A request loop: submit ranges at any time; a host thread packs the pending work of all
+ * admitted requests into back-to-back persistent launches (continuous batching) while the
+ * sieve stays resident.  Requests are admitted while their device footprint fits. */
+typedef struct collatz_server collatz_server;
+
+typedef struct {
+  uint32_t chunk_log2;       /* 0 = 24 */
+  uint32_t cutoff_log2;      /* 0 = 187 */
+  uint64_t max_device_bytes; /* admission budget for per-request buffers; 0 = 1 GiB */
+  uint64_t slice_candidates; /* target work per launch; 0 = autotuned */
+  uint32_t max_inflight;     /* admitted requests at once; 0 = 1024 */
+  uint32_t reserved;
+} collatz_server_opts;
+
+typedef struct {
+  collatz_chunk total;
+  uint64_t chunks;           /* chunk records of the request */
+  double submit_ms;          /* host clock at submission (ms since server start) */
+  double admit_ms;           /* when it was admitted */
+  double done_ms;            /* when its last record was folded */
+} collatz_request_result;
+
+int collatz_server_create(const collatz_sieve* sieve, const collatz_server_opts* opts,
+                          collatz_server** out);
+/* Returns a ticket >= 0, or a negative status. */
+int64_t collatz_server_submit(collatz_server* srv, const collatz_interval* range);
+/* Waits up to timeout_ms (< 0: forever) for the ticket; COLLATZ_ENOTREADY on timeout. */
+int collatz_server_wait(collatz_server* srv, int64_t ticket, double timeout_ms,
+                        collatz_request_result* out, collatz_chunk* chunks, uint64_t cap);
+int collatz_server_destroy(collatz_server* srv);
+
+/* ---- misc ---------------------------------------------------------------------------- */
+int collatz_version(void);
+const char* collatz_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COLLATZ_H */

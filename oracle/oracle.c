@@ -1,0 +1,655 @@
+/*
+ * CPU oracle for the Collatz range verifier -- TEST INFRASTRUCTURE ONLY.
+ * See oracle.h for what each function restates and how it is pinned.
+ *
+ * Plain C11 with unsigned __int128.  Deliberately written as straightforward scalar code
+ * (u64 limbs, one T-run at a time) so that it shares no arithmetic with the HIP kernels
+ * (u32 words, funnel shifts, wave-wide reductions).
+ */
+#include "oracle.h"
+
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef unsigned __int128 u128;
+
+static inline u128 mk128(uint64_t lo, uint64_t hi) { return ((u128)hi << 64) | lo; }
+
+/* 3^f as u128, f <= 80 (THREEPOWERS, collatz_cudav7.cu:63,104-105). */
+static u128 pow3_128(uint32_t f) {
+  u128 x = 1;
+  while (f--) x *= 3;
+  return x;
+}
+
+void or_pow3(uint32_t f, uint64_t out[2]) {
+  u128 x = pow3_128(f);
+  out[0] = (uint64_t)x;
+  out[1] = (uint64_t)(x >> 64);
+}
+
+/* 3^{-e} mod 2^32: ONETHIRD = 2863311531 raised to e (collatz_cudav7.cu:66,122-128). */
+uint32_t or_magic(uint32_t e) {
+  uint32_t m = 1;
+  while (e--) m *= 2863311531u;
+  return m;
+}
+
+/* THREEPOWEROFFSET = floor(MBI * 306 / 485) + 1 (collatz_cudav7.cu:67). */
+uint32_t or_tpo(uint32_t mbi) { return mbi * OR_NUMERATOR / OR_DENOMINATOR + 1; }
+
+/* ------------------------------------------------------------------------------------
+ * search1: collatz_cudav7.cu:497-563 (C++), main.rs:206-278 (Rust).
+ * number = T^sigfig(base) + 1.  Shifted map: number even -> odd step number*3/2,
+ * number odd -> even step number/2 + 1 (collatz_cudav7.cu:545-561).
+ * ---------------------------------------------------------------------------------- */
+typedef void (*leaf_fn)(void* ctx, const or_node* leaf);
+
+typedef struct {
+  uint32_t depth, sem;
+  leaf_fn fn;
+  void* ctx;
+  int64_t count;
+  u128 p3[128];
+} s1_ctx;
+
+static void s1_rec(s1_ctx* c, u128 base, uint32_t s, uint32_t f, u128 number, uint32_t de,
+                   uint32_t ne) {
+  /* main.rs:216, Rust only (never removes a leaf at the depths used here). */
+  if (c->sem == OR_SEM_RUST && number <= base) return;
+  /* Induction sieve: 485 f <= 306 k (collatz_cudav7.cu:507). */
+  if (OR_DENOMINATOR * f <= OR_NUMERATOR * s) return;
+  /* Mod 3 preimage / path merging sieve (collatz_cudav7.cu:509-511). */
+  if (ne >= 2 && ne % 2 == 0) {
+    if (OR_NUMERATOR * (s - 1) >= OR_DENOMINATOR * (f - 1)) return;
+  }
+  if (s == c->depth) {
+    c->count++;
+    if (c->fn) {
+      or_node n;
+      n.base_lo = (uint64_t)base;
+      n.base_hi = (uint64_t)(base >> 64);
+      n.number_lo = (uint64_t)number;
+      n.number_hi = (uint64_t)(number >> 64);
+      n.sigfig = s;
+      n.threepower = f;
+      n.doubleeven = de;
+      n.numevens = ne;
+      c->fn(c->ctx, &n);
+    }
+    return;
+  }
+  /* Odd-even-even state (collatz_cudav7.cu:538-543). */
+  uint32_t inplay = (OR_NUMERATOR * (s + 1) >= OR_DENOMINATOR * f) ? 1u : 0u;
+  u128 nA = number;
+  u128 nB = number + c->p3[f];
+  u128 bit = (u128)1 << s;
+  if ((nA & 1) == 0) {
+    s1_rec(c, base, s + 1, f + 1, nA + nA / 2, (de | inplay) & ~2u, 0);
+    if (de < 2) s1_rec(c, base | bit, s + 1, f, nB / 2 + 1, de << 1, ne + 1);
+  } else {
+    if (de < 2) s1_rec(c, base, s + 1, f, nA / 2 + 1, de << 1, ne + 1);
+    s1_rec(c, base | bit, s + 1, f + 1, nB + nB / 2, (de | inplay) & ~2u, 0);
+  }
+}
+
+static int64_t s1_walk(const or_node* root, uint32_t depth, uint32_t sem, leaf_fn fn,
+                       void* ctx) {
+  if (!root || depth > 100 || root->sigfig > depth) return -1;
+  s1_ctx* c = (s1_ctx*)malloc(sizeof(s1_ctx));
+  if (!c) return -1;
+  c->depth = depth;
+  c->sem = sem;
+  c->fn = fn;
+  c->ctx = ctx;
+  c->count = 0;
+  u128 x = 1;
+  for (int i = 0; i < 128; ++i) {
+    c->p3[i] = x;
+    x *= 3;
+  }
+  s1_rec(c, mk128(root->base_lo, root->base_hi), root->sigfig, root->threepower,
+         mk128(root->number_lo, root->number_hi), root->doubleeven, root->numevens);
+  int64_t n = c->count;
+  free(c);
+  return n;
+}
+
+typedef struct {
+  or_node* out;
+  int64_t cap, n;
+} store_ctx;
+
+static void store_leaf(void* vctx, const or_node* leaf) {
+  store_ctx* s = (store_ctx*)vctx;
+  if (s->n < s->cap) s->out[s->n] = *leaf;
+  s->n++;
+}
+
+int64_t or_search1(const or_node* root, uint32_t depth, uint32_t semantics, or_node* leaves,
+                   int64_t cap) {
+  store_ctx s = {leaves, leaves ? cap : 0, 0};
+  return s1_walk(root, depth, semantics, leaves ? store_leaf : NULL, &s);
+}
+
+/* Leaves from the canonical root, which the reference starts from: "3 2 2 9 1 0"
+ * (collatz_cudav7.cu:6, :173), sorted by residue with an LSD radix sort. */
+typedef struct {
+  uint64_t* r;
+  uint64_t* num;
+  uint8_t* f;
+  int64_t cap, n;
+} sorted_ctx;
+
+static void sorted_leaf(void* vctx, const or_node* leaf) {
+  sorted_ctx* s = (sorted_ctx*)vctx;
+  if (s->n < s->cap) {
+    s->r[s->n] = leaf->base_lo;
+    s->num[s->n] = leaf->number_lo;
+    s->f[s->n] = (uint8_t)leaf->threepower;
+  }
+  s->n++;
+}
+
+int64_t or_sieve_sorted(uint32_t k, uint64_t* r, uint64_t* number, uint8_t* f, int64_t cap) {
+  if (k < 2 || k > 40) return -1;
+  or_node root = {3, 0, 9, 0, 2, 2, 1, 0};
+  if (!r || !number || !f) return s1_walk(&root, k, OR_SEM_CXX, NULL, NULL);
+  sorted_ctx s = {r, number, f, cap, 0};
+  int64_t n = s1_walk(&root, k, OR_SEM_CXX, sorted_leaf, &s);
+  if (n < 0 || n > cap) return n;
+  /* LSD radix sort by r, 16-bit digits, carrying number and f along. */
+  uint64_t* r2 = (uint64_t*)malloc((size_t)n * 8 + 8);
+  uint64_t* n2 = (uint64_t*)malloc((size_t)n * 8 + 8);
+  uint8_t* f2 = (uint8_t*)malloc((size_t)n + 1);
+  int64_t* cnt = (int64_t*)malloc(sizeof(int64_t) * 65537);
+  if (!r2 || !n2 || !f2 || !cnt) {
+    free(r2); free(n2); free(f2); free(cnt);
+    return -1;
+  }
+  for (uint32_t shift = 0; shift < k; shift += 16) {
+    memset(cnt, 0, sizeof(int64_t) * 65537);
+    for (int64_t i = 0; i < n; ++i) cnt[((r[i] >> shift) & 0xFFFF) + 1]++;
+    for (int i = 0; i < 65536; ++i) cnt[i + 1] += cnt[i];
+    for (int64_t i = 0; i < n; ++i) {
+      int64_t d = cnt[(r[i] >> shift) & 0xFFFF]++;
+      r2[d] = r[i];
+      n2[d] = number[i];
+      f2[d] = f[i];
+    }
+    memcpy(r, r2, (size_t)n * 8);
+    memcpy(number, n2, (size_t)n * 8);
+    memcpy(f, f2, (size_t)n);
+  }
+  free(r2); free(n2); free(f2); free(cnt);
+  return n;
+}
+
+/* ------------------------------------------------------------------------------------
+ * precompute: collatz_cudav7.cu:413-494 (C++), main.rs:101-107,125-203 (Rust).
+ * ---------------------------------------------------------------------------------- */
+
+/* collatz_cudav7.cu:415-421.  In C++, `(x) & 0b11 == 0` parses as `x & (0b11 == 0)`,
+ * which is `x & 0`: the test is never true.  main.rs:104 binds `&` first and tests the
+ * two bits.  Both versions are kept, selected by semantics. */
+int or_doubleeven(uint32_t upordown, uint32_t i, uint32_t B, uint32_t semantics) {
+  uint32_t a = (uint32_t)__builtin_ctz((upordown >> i) + 1u);
+  if (i + a + 2 >= B) return 0;
+  if (semantics != OR_SEM_RUST) return 0;
+  return ((upordown >> (i + a)) & 3u) == 0;
+}
+
+/* `x > y` as the precompute evaluates it: signed in collatz_cudav7.cu:449,461 and main.rs,
+ * unsigned (int + u32 promotes to u32) in collatz_openclv2.cpp:319,331. */
+static inline int gt_sem(int32_t x, int32_t y, int uns) {
+  return uns ? ((uint32_t)x > (uint32_t)y) : (x > y);
+}
+
+int or_precompute_bv(const or_knobs* kn, uint64_t* bv) {
+  if (!kn || !bv || kn->B < 6 || kn->B > 30 || kn->A > kn->N) return -1;
+  const uint32_t B = kn->B, mbi = kn->N - kn->A, tpo = or_tpo(mbi);
+  const int32_t first = (int32_t)(int16_t)(OR_DENOMINATOR * tpo - OR_NUMERATOR * mbi);
+  const uint64_t words = 1ull << (B - 6);
+  const uint32_t mask = (B >= 32) ? 0xFFFFFFFFu : ((1u << B) - 1u);
+  const int uns = kn->semantics == OR_SEM_OPENCL;
+  uint32_t magic[64];
+  for (uint32_t a = 0; a < kn->num_bv && a < 64; ++a) magic[a] = or_magic(tpo + a);
+  memset(bv, 0, (size_t)kn->num_bv * words * 8);
+  for (uint64_t n0 = 0; n0 < (1ull << B); ++n0) {
+    uint32_t upordown = 0;
+    uint64_t n = n0;
+    for (uint32_t i = 0; i < B; ++i) {
+      if (n % 2 == 0) {
+        n = n + n / 2;
+        upordown |= 1u << i;
+      } else {
+        n = n / 2 + 1;
+      }
+    }
+    int32_t cur = 0, mx = 0;
+    uint32_t ne = 0;
+    for (uint32_t i = 0; i < B; ++i) {
+      if (upordown & (1u << i)) {
+        if (gt_sem(cur + (int32_t)OR_NUMERATOR, mx, uns) &&
+            or_doubleeven(upordown, i, B, kn->semantics))
+          mx = cur + (int32_t)OR_NUMERATOR;
+        cur -= (int32_t)(OR_DENOMINATOR - OR_NUMERATOR);
+        ne = 0;
+      } else {
+        cur += (int32_t)OR_NUMERATOR;
+        ne += 1;
+        if (ne % 2 == 0 && gt_sem(cur + (int32_t)(OR_DENOMINATOR - OR_NUMERATOR), mx, uns))
+          mx = cur + (int32_t)(OR_DENOMINATOR - OR_NUMERATOR);
+      }
+      if (cur > mx) mx = cur;
+    }
+    for (uint32_t a = 0; a < kn->num_bv; ++a) {
+      if (mx <= first + (int32_t)(a * OR_DENOMINATOR)) {
+        uint32_t idx = ((uint32_t)n0 * magic[a]) & mask;
+        bv[a * words + idx / 64] |= 1ull << (idx % 64);
+      }
+    }
+  }
+  return 0;
+}
+
+static int mod9_keep(uint32_t r) { return r == 0 || r == 1 || r == 3 || r == 6 || r == 7; }
+
+/* collatz_cudav7.cu:482-494: row r, bit i set iff (r + i * 2^MBI) mod 9 in {0,1,3,6,7}. */
+int or_mod9_rows(uint32_t mbi, uint32_t A, uint64_t* rows) {
+  if (!rows || A > 20 || mbi > 120) return -1;
+  const uint64_t words = (A >= 6) ? (1ull << (A - 6)) : 1;
+  const uint32_t step = (uint32_t)(((u128)1 << mbi) % 9);
+  memset(rows, 0, 9 * words * 8);
+  for (uint32_t a = 0; a < 9; ++a) {
+    uint32_t red = a;
+    for (uint64_t i = 0; i < (1ull << A); ++i) {
+      if (mod9_keep(red)) rows[a * words + i / 64] |= 1ull << (i % 64);
+      red += step;
+      if (red >= 9) red -= 9;
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------------------
+ * The iteration schedule (collatz_cudav7.cu:280-366 for the kernel, :684-715 for
+ * search3).  One check = odd run, even run, odd run, even run, then hash ^= low 32 bits
+ * and the test number < base.
+ *   odd run : a = ctz(number | 2^8) <= 8,  number = (number >> a) * 3^a
+ *   even run: number -= 1; b = ctz(number | 2^31) <= 31; number = (number >> b) + 1
+ * If number reaches 2^cutoff after an odd run it is reported and replaced by 2.
+ * Values are kept exactly in three u64 limbs (the cutoff is at most 187, so an odd run,
+ * which multiplies by at most (3/2)^8 < 2^5, cannot leave 192 bits).
+ * ---------------------------------------------------------------------------------- */
+typedef struct {
+  uint64_t w[3];
+} big3;
+
+static inline int big_cmp(const big3* x, const big3* y) {
+  for (int i = 2; i >= 0; --i) {
+    if (x->w[i] != y->w[i]) return x->w[i] < y->w[i] ? -1 : 1;
+  }
+  return 0;
+}
+
+static inline void big_shr(big3* x, uint32_t s) {
+  if (s == 0) return;
+  x->w[0] = (x->w[0] >> s) | (x->w[1] << (64 - s));
+  x->w[1] = (x->w[1] >> s) | (x->w[2] << (64 - s));
+  x->w[2] >>= s;
+}
+
+static inline void big_mul_small(big3* x, uint64_t m) {
+  u128 carry = 0;
+  for (int i = 0; i < 3; ++i) {
+    u128 t = (u128)x->w[i] * m + carry;
+    x->w[i] = (uint64_t)t;
+    carry = t >> 64;
+  }
+}
+
+static inline void big_sub1(big3* x) {
+  for (int i = 0; i < 3; ++i) {
+    if (x->w[i]-- != 0) break;
+  }
+}
+
+static inline void big_add1(big3* x) {
+  for (int i = 0; i < 3; ++i) {
+    if (++x->w[i] != 0) break;
+  }
+}
+
+static inline int big_ge_pow2(const big3* x, uint32_t c) {
+  uint32_t wi = c / 64, bi = c % 64;
+  for (uint32_t i = 2; i > wi; --i)
+    if (x->w[i]) return 1;
+  return (x->w[wi] >> bi) != 0;
+}
+
+static const uint64_t POW3_SMALL[9] = {1, 3, 9, 27, 81, 243, 729, 2187, 6561};
+
+void or_traj(const uint64_t base[2], const uint64_t num0[3], uint32_t cutoff, int second_gt,
+             or_traj_out* out) {
+  big3 v = {{num0[0], num0[1], num0[2]}};
+  big3 mx = v;
+  big3 b = {{base[0], base[1], 0}};
+  uint32_t hash = 0, ovf = 0;
+  uint64_t steps = 0;
+  if (cutoff > 187) cutoff = 187;
+  for (;;) {
+    for (int half = 0; half < 2; ++half) {
+      uint32_t a = (uint32_t)__builtin_ctzll(v.w[0] | 0x100u);
+      big_shr(&v, a);
+      big_mul_small(&v, POW3_SMALL[a]);
+      steps += a;
+      if (big_cmp(&v, &mx) > 0) mx = v;
+      int over = big_ge_pow2(&v, cutoff);
+      if (over && second_gt && half == 1) {
+        /* collatz_cudav7.cu:702 uses '>' here: exactly 2^cutoff does not count. */
+        big3 p = {{0, 0, 0}};
+        p.w[cutoff / 64] = 1ull << (cutoff % 64);
+        over = big_cmp(&v, &p) > 0;
+      }
+      if (over) {
+        ovf++;
+        v.w[0] = 2;
+        v.w[1] = 0;
+        v.w[2] = 0;
+      }
+      big_sub1(&v);
+      uint32_t bb = (uint32_t)__builtin_ctzll(v.w[0] | 0x80000000u);
+      big_shr(&v, bb);
+      big_add1(&v);
+      steps += bb;
+    }
+    hash ^= (uint32_t)v.w[0];
+    if (big_cmp(&v, &b) < 0) break;
+  }
+  out->hash = hash;
+  out->overflow = ovf;
+  out->steps = steps;
+  out->maxnum[0] = mx.w[0];
+  out->maxnum[1] = mx.w[1];
+  out->maxnum[2] = mx.w[2];
+}
+
+/* ------------------------------------------------------------------------------------
+ * Reference mode: search2 (collatz_cudav7.cu:654-682, main.rs:280-319) and whole case
+ * lines (collatz_cudav7.cu:95-183 with launchkernel :565-649).
+ * ---------------------------------------------------------------------------------- */
+static inline int bit_at(const uint64_t* words, uint64_t i) { return (words[i / 64] >> (i % 64)) & 1; }
+
+typedef struct {
+  const or_knobs* kn;
+  const uint64_t* bv;
+  const uint64_t* mod9;
+  uint32_t hash;
+  uint64_t cand;
+  uint64_t ovf;
+  int count_only;
+} leaf_ctx;
+
+static void leaf_candidates(leaf_ctx* c, const or_node* leaf) {
+  const or_knobs* kn = c->kn;
+  const uint32_t mbi = kn->N - kn->A, tpo = or_tpo(mbi), f = leaf->threepower;
+  const uint32_t bidx = f - tpo;
+  const uint32_t mask = (1u << kn->B) - 1u;
+  const uint64_t bvwords = 1ull << (kn->B - 6);
+  const uint64_t m9words = (kn->A >= 6) ? (1ull << (kn->A - 6)) : 1;
+  const u128 base = mk128(leaf->base_lo, leaf->base_hi);
+  const u128 number = mk128(leaf->number_lo, leaf->number_hi);
+  const uint32_t red = (uint32_t)(base % 9);
+  const int use_bv = bidx < kn->num_bv;
+  const uint32_t idx = use_bv ? ((or_magic(f) * (uint32_t)number) & mask) : 0;
+  const u128 p3 = pow3_128(f);
+  for (uint64_t a = 0; a < (1ull << kn->A); ++a) {
+    if (!bit_at(c->mod9 + red * m9words, a)) continue;
+    if (use_bv && !bit_at(c->bv + bidx * bvwords, (idx + a) & mask)) continue;
+    if (kn->semantics == OR_SEM_RUST) {
+      /* main.rs:309-313 */
+      uint64_t nl = (uint64_t)number;
+      if (leaf->doubleeven == 2 && (nl + a) % 2 == 1) continue;
+      if (leaf->doubleeven == 1) {
+        if (f % 2 == 0 && (nl + a) % 4 == 1) continue;
+        if (f % 2 == 1 && (nl + 3 * a) % 4 == 1) continue;
+      }
+    }
+    c->cand++;
+    if (c->count_only) continue;
+    u128 n = base + ((u128)a << mbi);
+    u128 v = number + (u128)a * p3;
+    uint64_t b2[2] = {(uint64_t)n, (uint64_t)(n >> 64)};
+    uint64_t v3[3] = {(uint64_t)v, (uint64_t)(v >> 64), 0};
+    or_traj_out o;
+    /* Leaves without a bit vector run on the CPU in the reference (search2/search3,
+     * collatz_cudav7.cu:517-519), whose second overflow test is '>'. */
+    or_traj(b2, v3, kn->cutoff, !use_bv, &o);
+    c->hash ^= o.hash;
+    c->ovf += o.overflow;
+  }
+}
+
+uint32_t or_leaf_hash(const or_knobs* kn, const uint64_t* bv, const uint64_t* mod9,
+                      const or_node* leaf, uint64_t* ncand) {
+  leaf_ctx c = {kn, bv, mod9, 0, 0, 0, 0};
+  leaf_candidates(&c, leaf);
+  if (ncand) *ncand = c.cand;
+  return c.hash;
+}
+
+typedef struct {
+  leaf_ctx lc;
+  or_case_out* out;
+  uint32_t tpo;
+} case_ctx;
+
+static void case_leaf(void* vctx, const or_node* leaf) {
+  case_ctx* c = (case_ctx*)vctx;
+  uint32_t b = leaf->threepower - c->tpo;
+  c->out->bin_leaves[b < 16 ? b : 16]++;
+  leaf_candidates(&c->lc, leaf);
+}
+
+int or_case_run(const or_knobs* kn, const uint64_t* bv, const uint64_t* mod9,
+                const or_node* root, or_case_out* out) {
+  if (!kn || !bv || !mod9 || !root || !out || kn->A > kn->N) return -1;
+  memset(out, 0, sizeof(*out));
+  case_ctx c = {{kn, bv, mod9, 0, 0, 0, 0}, out, or_tpo(kn->N - kn->A)};
+  int64_t n = s1_walk(root, kn->N - kn->A, kn->semantics, case_leaf, &c);
+  if (n < 0) return -1;
+  out->leaves = (uint64_t)n;
+  out->complete_hash = c.lc.hash;
+  out->candidates = c.lc.cand;
+  out->overflow_count = c.lc.ovf;
+  /* One launch per full batch of 2^16 leaves plus one flush per non-empty bin
+   * (collatz_cudav7.cu:532-533, :177-179). */
+  for (int b = 0; b < 16; ++b) out->launches += (out->bin_leaves[b] + 65535) / 65536;
+  return 0;
+}
+
+int64_t or_rust_count(const or_knobs* kn, const uint64_t* bv, const uint64_t* mod9,
+                      const or_node* root) {
+  case_ctx c;
+  or_case_out scratch;
+  memset(&scratch, 0, sizeof(scratch));
+  leaf_ctx lc = {kn, bv, mod9, 0, 0, 0, 1};
+  c.lc = lc;
+  c.out = &scratch;
+  c.tpo = or_tpo(kn->N - kn->A);
+  int64_t n = s1_walk(root, kn->N - kn->A, kn->semantics, case_leaf, &c);
+  if (n < 0) return -1;
+  return (int64_t)c.lc.cand;
+}
+
+/* ------------------------------------------------------------------------------------
+ * Range mode.  n = q*2^k + r with r a sieve survivor; start value
+ * number = T^k(r) + 1 + q*3^f (Lemma 2.3, paper p.4; collatz_cudav7.cu:262-278),
+ * mod 9 filter on n (collatz_cudav7.cu:482-494), then the schedule above.
+ * ---------------------------------------------------------------------------------- */
+void or_chunk_merge(or_chunk* into, const or_chunk* from) {
+  if (from->count > 0) {
+    int take = into->count == 0;
+    if (!take) {
+      big3 a = {{from->max_exc[0], from->max_exc[1], from->max_exc[2]}};
+      big3 b = {{into->max_exc[0], into->max_exc[1], into->max_exc[2]}};
+      int c = big_cmp(&a, &b);
+      if (c > 0) take = 1;
+      if (c == 0 && mk128(from->max_start[0], from->max_start[1]) <
+                        mk128(into->max_start[0], into->max_start[1]))
+        take = 1;
+    }
+    if (take) {
+      memcpy(into->max_exc, from->max_exc, sizeof(into->max_exc));
+      memcpy(into->max_start, from->max_start, sizeof(into->max_start));
+    }
+  }
+  into->n_checked += from->n_checked;
+  into->count += from->count;
+  into->stop_sum += from->stop_sum;
+  into->stop_xor ^= from->stop_xor;
+  into->overflow_count += from->overflow_count;
+  into->hash ^= from->hash;
+  into->flags |= from->flags;
+}
+
+static int64_t lower_bound_u64(const uint64_t* a, int64_t n, uint64_t key) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = lo + (hi - lo) / 2;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+typedef struct {
+  uint32_t k, C, cutoff;
+  int mod9;
+  int64_t nres;
+  const uint64_t* r;
+  const uint64_t* number;
+  const uint8_t* f;
+  u128 lo, hi;
+  uint64_t nchunks;
+  or_chunk* chunks;
+  atomic_ullong next;
+  u128 p3[64];
+} range_ctx;
+
+static void range_chunk(range_ctx* c, uint64_t ci, or_chunk* out) {
+  memset(out, 0, sizeof(*out));
+  const u128 j = (c->lo >> c->C) + ci;
+  u128 c_lo = j << c->C;
+  u128 c_hi = c_lo + ((u128)1 << c->C);
+  if (c_lo < c->lo) c_lo = c->lo;
+  if (c_hi > c->hi) c_hi = c->hi;
+  out->chunk_index = (uint64_t)j;
+  out->n_checked = (uint64_t)(c_hi - c_lo);
+  const u128 kspan = (u128)1 << c->k;
+  big3 best = {{0, 0, 0}};
+  u128 best_start = 0;
+  for (u128 q = c_lo >> c->k; q <= (c_hi - 1) >> c->k; ++q) {
+    const u128 qbase = q << c->k;
+    const uint64_t rlo = c_lo > qbase ? (uint64_t)(c_lo - qbase) : 0;
+    const u128 rhi128 = c_hi - qbase;
+    const uint64_t rhi = rhi128 > kspan ? (uint64_t)kspan : (uint64_t)rhi128;
+    const int64_t i0 = lower_bound_u64(c->r, c->nres, rlo);
+    const int64_t i1 = lower_bound_u64(c->r, c->nres, rhi);
+    const uint64_t qlo = (uint64_t)q, qhi = (uint64_t)(q >> 64);
+    for (int64_t i = i0; i < i1; ++i) {
+      const u128 n = qbase + c->r[i];
+      if (c->mod9 && !mod9_keep((uint32_t)(n % 9))) continue;
+      /* number = number_r + q * 3^f, up to 192 bits */
+      const u128 p = c->p3[c->f[i]];
+      const uint64_t plo = (uint64_t)p, phi = (uint64_t)(p >> 64);
+      big3 v = {{0, 0, 0}};
+      u128 t0 = (u128)qlo * plo;
+      u128 t1 = (u128)qlo * phi + (t0 >> 64);
+      u128 t2 = (u128)qhi * plo + (uint64_t)t1;
+      v.w[0] = (uint64_t)t0;
+      v.w[1] = (uint64_t)t2;
+      v.w[2] = (uint64_t)(t1 >> 64) + (uint64_t)(t2 >> 64) + qhi * phi;
+      u128 s = (u128)v.w[0] + c->number[i];
+      v.w[0] = (uint64_t)s;
+      s = (u128)v.w[1] + (uint64_t)(s >> 64);
+      v.w[1] = (uint64_t)s;
+      v.w[2] += (uint64_t)(s >> 64);
+      const uint64_t b2[2] = {(uint64_t)n, (uint64_t)(n >> 64)};
+      or_traj_out o;
+      or_traj(b2, v.w, c->cutoff, 0, &o);
+      const uint64_t J = c->k + o.steps;
+      out->count++;
+      out->hash ^= o.hash;
+      out->stop_sum += J;
+      out->stop_xor ^= J;
+      out->overflow_count += o.overflow;
+      big3 m = {{o.maxnum[0], o.maxnum[1], o.maxnum[2]}};
+      int cmp = big_cmp(&m, &best);
+      if (out->count == 1 || cmp > 0 || (cmp == 0 && n < best_start)) {
+        best = m;
+        best_start = n;
+      }
+    }
+  }
+  if (out->count) {
+    big_sub1(&best); /* T-value = number - 1 */
+    memcpy(out->max_exc, best.w, sizeof(best.w));
+    out->max_start[0] = (uint64_t)best_start;
+    out->max_start[1] = (uint64_t)(best_start >> 64);
+    if (out->overflow_count) out->flags |= 1u;
+    big_add1(&best);
+    if (big_ge_pow2(&best, OR_WIDE_LOG2)) out->flags |= 2u;
+  }
+}
+
+static void* range_worker(void* vctx) {
+  range_ctx* c = (range_ctx*)vctx;
+  for (;;) {
+    uint64_t ci = atomic_fetch_add(&c->next, 1);
+    if (ci >= c->nchunks) break;
+    range_chunk(c, ci, &c->chunks[ci]);
+  }
+  return NULL;
+}
+
+int or_verify_range(uint32_t k, int64_t nres, const uint64_t* r, const uint64_t* number,
+                    const uint8_t* f, int mod9, const uint64_t lo[2], const uint64_t hi[2],
+                    uint32_t C, uint32_t cutoff, int threads, or_chunk* chunks,
+                    or_chunk* total) {
+  if (k < 2 || k > 40 || C > 100 || !chunks || !total || nres < 0) return -1;
+  const u128 L = mk128(lo[0], lo[1]), H = mk128(hi[0], hi[1]);
+  if (L >= H) return -1;
+  range_ctx* c = (range_ctx*)calloc(1, sizeof(range_ctx));
+  if (!c) return -1;
+  c->k = k;
+  c->C = C;
+  c->cutoff = cutoff;
+  c->mod9 = mod9;
+  c->nres = nres;
+  c->r = r;
+  c->number = number;
+  c->f = f;
+  c->lo = L;
+  c->hi = H;
+  c->nchunks = (uint64_t)(((H - 1) >> C) - (L >> C) + 1);
+  c->chunks = chunks;
+  atomic_init(&c->next, 0);
+  for (int i = 0; i < 64; ++i) c->p3[i] = pow3_128((uint32_t)i);
+  if (threads <= 1) {
+    range_worker(c);
+  } else {
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)threads);
+    for (int t = 0; t < threads; ++t) pthread_create(&th[t], NULL, range_worker, c);
+    for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+    free(th);
+  }
+  memset(total, 0, sizeof(*total));
+  for (uint64_t i = 0; i < c->nchunks; ++i) or_chunk_merge(total, &chunks[i]);
+  total->chunk_index = (uint64_t)(L >> C);
+  free(c);
+  return 0;
+}
