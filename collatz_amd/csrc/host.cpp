@@ -84,8 +84,8 @@ Limits make_limits(int limbs, uint32_t cutoff) {
   return l;
 }
 
-int plan_rows(uint32_t k, u128 lo, u128 hi, uint32_t C, uint32_t slot_base, RowPlan& plan,
-              uint64_t max_rows) {
+int plan_rows(uint32_t k, u128 lo, u128 hi, uint32_t C, uint32_t slot_base, bool mod9,
+              RowPlan& plan, uint64_t max_rows) {
   const u128 first = lo >> C, last = (hi - 1) >> C;
   const u128 nch = last - first + 1;
   if (nch > (u128)0x7FFFFFFF) return fail(COLLATZ_EINVAL, "range has too many chunks");
@@ -103,32 +103,40 @@ int plan_rows(uint32_t k, u128 lo, u128 hi, uint32_t C, uint32_t slot_base, RowP
     plan.n_checked.push_back((uint64_t)(c_hi - c_lo));
     plan.chunk_first_row.push_back(plan.rows.size() - base_rows);
     for (u128 q = c_lo >> k; q <= (c_hi - 1) >> k; ++q) {
-      if (plan.rows.size() - base_rows >= max_rows)
-        return fail(COLLATZ_EINVAL, "range decomposes into too many rows");
       const u128 qb = q << k;
       const u128 r_lo = c_lo > qb ? c_lo - qb : 0;
       const u128 r_hi = (c_hi - qb) < kspan ? (c_hi - qb) : kspan;
-      RowDesc row;
-      std::memset(&row, 0, sizeof(row));
-      row.q_lo = lo64(q);
-      row.q_hi = hi64(q);
-      row.qb_lo = lo64(qb);
-      row.qb_hi = hi64(qb);
-      row.slot = slot_base + (uint32_t)cj;
-      row.qb_mod9 = (uint32_t)(qb % 9);
-      if (r_lo == 0) {
-        plan.key_lo.push_back(-1);
-      } else {
-        plan.key_lo.push_back((int64_t)plan.keys.size());
-        plan.keys.push_back((uint64_t)r_lo);
+      const uint32_t qb9 = (uint32_t)(qb % 9);
+      // n mod 9 = (qb + r) mod 9: only classes that the mod 9 sieve keeps get a row
+      // (collatz_cudav7.cu:482-494).
+      for (uint32_t c = 0; c < 9; ++c) {
+        if (mod9 && !((kMod9Keep >> ((qb9 + c) % 9)) & 1u)) continue;
+        if (plan.rows.size() - base_rows >= max_rows)
+          return fail(COLLATZ_EINVAL, "range decomposes into too many rows");
+        RowDesc row;
+        std::memset(&row, 0, sizeof(row));
+        row.q_lo = lo64(q);
+        row.q_hi = hi64(q);
+        row.qb_lo = lo64(qb);
+        row.qb_hi = hi64(qb);
+        row.slot = slot_base + (uint32_t)cj;
+        row.cls = c;
+        if (r_lo == 0) {
+          plan.key_lo.push_back(-1);
+        } else {
+          plan.key_lo.push_back((int64_t)plan.keys.size());
+          plan.keys.push_back((uint64_t)r_lo);
+          plan.key_cls.push_back(c);
+        }
+        if (r_hi == kspan) {
+          plan.key_hi.push_back(-1);
+        } else {
+          plan.key_hi.push_back((int64_t)plan.keys.size());
+          plan.keys.push_back((uint64_t)r_hi);
+          plan.key_cls.push_back(c);
+        }
+        plan.rows.push_back(row);
       }
-      if (r_hi == kspan) {
-        plan.key_hi.push_back(-1);
-      } else {
-        plan.key_hi.push_back((int64_t)plan.keys.size());
-        plan.keys.push_back((uint64_t)r_hi);
-      }
-      plan.rows.push_back(row);
     }
   }
   plan.chunk_first_row.push_back(plan.rows.size() - base_rows);
@@ -302,11 +310,35 @@ int collatz_sieve_build(const collatz_sieve_params* params, collatz_sieve** out)
     return rc;
   }
   sv->count = leaves.count;
-  sv->res = std::move(leaves.res);
-  sv->num = std::move(leaves.num);
-  sv->meta = std::move(leaves.meta);
   sv->bits = std::move(leaves.bits);
   sv->bits_bits = leaves.bits_bits;
+  {
+    // Class-major layout by r mod 9, each class sorted by r.
+    const uint64_t n = leaves.count;
+    const uint64_t nb = (n + 2047) / 2048;
+    DevBuf<uint32_t> counts;
+    DevBuf<uint64_t> offs;
+    const bool ok =
+        sv->res.alloc(std::max<uint64_t>(1, n)) == hipSuccess &&
+        sv->num.alloc(std::max<uint64_t>(1, n)) == hipSuccess &&
+        sv->meta.alloc(std::max<uint64_t>(1, n)) == hipSuccess &&
+        counts.alloc(std::max<uint64_t>(1, nb)) == hipSuccess &&
+        offs.alloc(nb + 1) == hipSuccess && sv->class_start_dev.alloc(10) == hipSuccess;
+    if (!ok) {
+      delete sv;
+      return fail(COLLATZ_ENOMEM, "device allocation failed (sieve)");
+    }
+    hipError_t e = launch_class_partition(leaves.res.p, leaves.num.p, leaves.meta.p, n,
+                                          sv->res.p, sv->num.p, sv->meta.p, counts.p, offs.p,
+                                          sv->class_start, s);
+    if (e == hipSuccess)
+      e = hipMemcpy(sv->class_start_dev.p, sv->class_start, sizeof(sv->class_start),
+                    hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      delete sv;
+      return fail(COLLATZ_EDEVICE, std::string("class partition: ") + hipGetErrorString(e));
+    }
+  }
   uint64_t p3[kPow3Max] = {0};
   uint64_t x = 1;
   for (int i = 0; i <= 40; ++i) {
@@ -331,7 +363,7 @@ int collatz_sieve_query(const collatz_sieve* sv, collatz_sieve_info* info) {
   info->survivors = sv->count;
   info->bitarray_bits = sv->bits_bits;
   info->device_bytes = sv->res.bytes() + sv->num.bytes() + sv->meta.bytes() +
-                       sv->bits.bytes() + sv->pow3.bytes();
+                       sv->bits.bytes() + sv->pow3.bytes() + sv->class_start_dev.bytes();
   info->build_ms = sv->build_ms;
   return COLLATZ_OK;
 }
@@ -340,14 +372,23 @@ int collatz_sieve_copy(const collatz_sieve* sv, uint64_t* r, uint64_t* number, u
                        uint64_t cap) {
   if (!sv) return fail(COLLATZ_EINVAL, "null sieve");
   HipDeviceGuard guard(sv->device);
-  const uint64_t n = std::min(cap, sv->count);
-  if (n == 0) return COLLATZ_OK;
-  if (r) CZ_HIP(hipMemcpy(r, sv->res.p, n * 8, hipMemcpyDeviceToHost));
-  if (number) CZ_HIP(hipMemcpy(number, sv->num.p, n * 8, hipMemcpyDeviceToHost));
-  if (f) {
-    std::vector<uint32_t> m(n);
-    CZ_HIP(hipMemcpy(m.data(), sv->meta.p, n * 4, hipMemcpyDeviceToHost));
-    for (uint64_t i = 0; i < n; ++i) f[i] = (uint8_t)meta_f(m[i]);
+  // Survivors are stored class-major; return them sorted by residue.
+  const uint64_t all = sv->count;
+  std::vector<uint64_t> hr(all), hn(all);
+  std::vector<uint32_t> hm(all);
+  if (all) {
+    CZ_HIP(hipMemcpy(hr.data(), sv->res.p, all * 8, hipMemcpyDeviceToHost));
+    CZ_HIP(hipMemcpy(hn.data(), sv->num.p, all * 8, hipMemcpyDeviceToHost));
+    CZ_HIP(hipMemcpy(hm.data(), sv->meta.p, all * 4, hipMemcpyDeviceToHost));
+  }
+  std::vector<uint64_t> order(all);
+  for (uint64_t i = 0; i < all; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return hr[a] < hr[b]; });
+  const uint64_t n = std::min(cap, all);
+  for (uint64_t i = 0; i < n; ++i) {
+    if (r) r[i] = hr[order[i]];
+    if (number) number[i] = hn[order[i]];
+    if (f) f[i] = (uint8_t)meta_f(hm[order[i]]);
   }
   return COLLATZ_OK;
 }

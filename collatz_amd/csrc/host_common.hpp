@@ -97,6 +97,7 @@ void chunk_merge(collatz_chunk* into, const collatz_chunk& from);
 struct RowPlan {
   std::vector<RowDesc> rows;
   std::vector<uint64_t> keys;        // residue bounds to look up
+  std::vector<uint32_t> key_cls;     // class of each key
   std::vector<int64_t> key_lo, key_hi;  // per row: index into keys, or -1 for 0 / S
   std::vector<uint64_t> chunk_first_row;  // per chunk: first row; size nchunks + 1
   std::vector<uint64_t> n_checked;  // per chunk
@@ -104,8 +105,9 @@ struct RowPlan {
   uint64_t nchunks = 0;
 };
 
-int plan_rows(uint32_t k, u128 lo, u128 hi, uint32_t C, uint32_t slot_base, RowPlan& plan,
-              uint64_t max_rows);
+int plan_rows(uint32_t k, u128 lo, u128 hi, uint32_t C, uint32_t slot_base, bool mod9,
+              RowPlan& plan, uint64_t max_rows);
+int resolve_rows(const struct ::collatz_sieve* sv, RowPlan& plan, hipStream_t s);
 
 Limits make_limits(int limbs, uint32_t cutoff);
 
@@ -133,6 +135,8 @@ struct collatz_sieve {
   cz::DevBuf<uint32_t> bits;
   uint64_t bits_bits = 0;
   cz::DevBuf<uint64_t> pow3;
+  uint64_t class_start[10] = {0};       // survivors are stored class-major by r mod 9
+  cz::DevBuf<uint64_t> class_start_dev;
   double build_ms = 0;
   cz::SieveArrays arrays() const {
     cz::SieveArrays a;

@@ -39,7 +39,14 @@ struct RowDesc {
   uint64_t i_lo, i_hi;        // sieve index range
   uint64_t block_start;       // first plain-kernel block of this row
   uint32_t slot;              // output record slot (chunk of an interval)
-  uint32_t qb_mod9;           // (q << k) mod 9
+  uint32_t cls;               // residue class r mod 9 of this row's survivors
+};
+
+// A unit of persistent work: sieve indices [i_lo, i_hi) of one row.
+struct TileDesc {
+  uint64_t i_lo, i_hi;
+  uint32_t row;
+  uint32_t pad;
 };
 
 // Partial or final record of one chunk, as the kernels produce it.  maxnum is the max of
@@ -68,7 +75,7 @@ struct DeviceLists {
   EscEntry* esc;              // escalation list
   unsigned long long* esc_count;
   uint64_t esc_cap;
-  uint64_t* ovf;              // overflow starts, 2 words each
+  uint64_t* ovf;              // overflow reports, 4 words each (n_lo, n_hi, slot, 0)
   unsigned long long* ovf_count;
   uint64_t ovf_cap;
   unsigned long long* stall_count;  // candidates abandoned after kMaxChecks checks
@@ -108,13 +115,26 @@ hipError_t launch_compact_write(const uint32_t* bits, uint64_t nwords,
 hipError_t launch_sieve_fill(const SieveRoot& root, uint32_t p, const uint64_t* pnum,
                              const uint32_t* pmeta, uint64_t* res_inout, uint64_t* num,
                              uint32_t* meta, uint64_t count, hipStream_t s);
-hipError_t launch_lower_bound(const uint64_t* sorted, uint64_t n, const uint64_t* keys,
-                              uint64_t nkeys, uint64_t* out, hipStream_t s);
+hipError_t launch_lower_bound(const uint64_t* sorted, const uint64_t* class_start,
+                              const uint64_t* keys, const uint32_t* kcls, uint64_t nkeys,
+                              uint64_t* out, hipStream_t s);
+hipError_t launch_class_partition(const uint64_t* res, const uint64_t* num, const uint32_t* meta,
+                                  uint64_t n, uint64_t* ores, uint64_t* onum, uint32_t* ometa,
+                                  uint32_t* counts, uint64_t* offs, uint64_t* class_start_host,
+                                  hipStream_t s);
 
 hipError_t launch_verify_plain(int limbs, const SieveArrays& sv, const RowDesc* rows,
                                uint32_t nrows, uint64_t nblocks, uint32_t R, uint32_t mod9,
                                const Limits& lim, const uint64_t* pow3, DevRec* block_recs,
                                DeviceLists lists, hipStream_t s);
+hipError_t launch_verify_persistent(int limbs, const SieveArrays& sv, const RowDesc* rows,
+                                    const TileDesc* tiles, const uint32_t* ntiles,
+                                    uint32_t* tile_counter, uint32_t nblocks, uint32_t U,
+                                    const Limits& lim, const uint64_t* pow3, DevRec* tile_recs,
+                                    DeviceLists lists, hipStream_t s);
+// Clears counters with kernels (no memset nodes, so the sequence can be graph-captured).
+hipError_t launch_zero(uint32_t* p32, uint32_t n32, unsigned long long* p64, uint32_t n64,
+                       hipStream_t s);
 hipError_t launch_reduce_chunks(const DevRec* parts, const uint64_t* part_start,
                                 uint32_t nchunks, DevRec* out, hipStream_t s);
 hipError_t launch_escalated(const SieveArrays& sv, const EscEntry* esc, uint64_t n,

@@ -27,17 +27,23 @@ int resolve_rows(const collatz_sieve* sv, RowPlan& plan, hipStream_t s) {
   std::vector<uint64_t> idx(plan.keys.size());
   if (!plan.keys.empty()) {
     DevBuf<uint64_t> dkeys, dout;
+    DevBuf<uint32_t> dcls;
     CZ_HIP(dkeys.alloc(plan.keys.size()));
+    CZ_HIP(dcls.alloc(plan.keys.size()));
     CZ_HIP(dout.alloc(plan.keys.size()));
     CZ_HIP(hipMemcpyAsync(dkeys.p, plan.keys.data(), plan.keys.size() * 8,
                           hipMemcpyHostToDevice, s));
-    CZ_HIP(launch_lower_bound(sv->res.p, sv->count, dkeys.p, plan.keys.size(), dout.p, s));
+    CZ_HIP(hipMemcpyAsync(dcls.p, plan.key_cls.data(), plan.key_cls.size() * 4,
+                          hipMemcpyHostToDevice, s));
+    CZ_HIP(launch_lower_bound(sv->res.p, sv->class_start_dev.p, dkeys.p, dcls.p,
+                              plan.keys.size(), dout.p, s));
     CZ_HIP(hipMemcpyAsync(idx.data(), dout.p, idx.size() * 8, hipMemcpyDeviceToHost, s));
     CZ_HIP(hipStreamSynchronize(s));
   }
   for (size_t i = 0; i < plan.rows.size(); ++i) {
-    plan.rows[i].i_lo = plan.key_lo[i] < 0 ? 0 : idx[plan.key_lo[i]];
-    plan.rows[i].i_hi = plan.key_hi[i] < 0 ? sv->count : idx[plan.key_hi[i]];
+    const uint32_t c = plan.rows[i].cls;
+    plan.rows[i].i_lo = plan.key_lo[i] < 0 ? sv->class_start[c] : idx[plan.key_lo[i]];
+    plan.rows[i].i_hi = plan.key_hi[i] < 0 ? sv->class_start[c + 1] : idx[plan.key_hi[i]];
     if (plan.rows[i].i_hi < plan.rows[i].i_lo) plan.rows[i].i_hi = plan.rows[i].i_lo;
   }
   return COLLATZ_OK;
@@ -48,6 +54,10 @@ int run_escalated(const collatz_sieve* sv, const DevBuf<EscEntry>& esc, uint64_t
                   uint32_t cutoff, DeviceLists lists, std::vector<DevRec>& recs,
                   uint32_t slot_base, hipStream_t s) {
   if (n == 0) return COLLATZ_OK;
+  // The list holds at most esc_cap entries; more escalations than that were dropped by the
+  // kernels, so the range must be rerun on three limbs instead.
+  if (n > esc.n || n > lists.esc_cap)
+    return fail(COLLATZ_ECAPACITY, "escalation list overflowed; rerun with limbs = 3");
   DevBuf<DevRec> out;
   CZ_HIP(out.alloc(n));
   const Limits lim3 = make_limits(3, cutoff);
@@ -68,7 +78,7 @@ int run_escalated(const collatz_sieve* sv, const DevBuf<EscEntry>& esc, uint64_t
 static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs& a,
                         hipStream_t s, collatz_range_result* out) {
   RowPlan plan;
-  int rc = plan_rows(sv->k, lo, hi, a.C, 0, plan, 1ull << 26);
+  int rc = plan_rows(sv->k, lo, hi, a.C, 0, sv->mod9 != 0, plan, 1ull << 26);
   if (rc) return rc;
   rc = resolve_rows(sv, plan, s);
   if (rc) return rc;
@@ -107,7 +117,7 @@ static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeAr
   DevBuf<uint64_t> dovf;
   DevBuf<unsigned long long> dcnt;
   CZ_HIP(desc.alloc(a.limbs == 2 ? esc_cap : 1));
-  CZ_HIP(dovf.alloc(2 * ovf_cap));
+  CZ_HIP(dovf.alloc(4 * ovf_cap));
   CZ_HIP(dcnt.alloc(3));
   CZ_HIP(hipMemsetAsync(dcnt.p, 0, 24, s));
   DeviceLists lists;
@@ -170,7 +180,12 @@ static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeAr
   out->overflow_written = ovf_n;
   if (out->overflow_starts && out->overflow_cap) {
     const uint64_t n = std::min<uint64_t>(std::min(ovf_n, ovf_cap), out->overflow_cap);
-    if (n) CZ_HIP(hipMemcpy(out->overflow_starts, dovf.p, n * 16, hipMemcpyDeviceToHost));
+    std::vector<uint64_t> h(4 * n);
+    if (n) CZ_HIP(hipMemcpy(h.data(), dovf.p, n * 32, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < n; ++i) {
+      out->overflow_starts[2 * i] = h[4 * i];
+      out->overflow_starts[2 * i + 1] = h[4 * i + 1];
+    }
   }
   if ((out->chunks && plan.nchunks > out->chunks_cap) || ovf_n > out->overflow_cap)
     return fail(COLLATZ_ECAPACITY, "output buffer smaller than the result");

@@ -246,13 +246,68 @@ __global__ __launch_bounds__(256) void k_sieve_fill(SieveRoot root, uint32_t p,
   meta[t] = pack_meta(x.f, (uint32_t)(r % 9), x.de, x.ne, false);
 }
 
-__global__ __launch_bounds__(256) void k_lower_bound(const uint64_t* a, uint64_t n,
-                                                     const uint64_t* keys, uint64_t nkeys,
-                                                     uint64_t* out) {
+
+// ---- stable partition of the survivors by n mod 9 ---------------------------------------
+// The range kernels take the survivors class by class: for a row with (q << k) mod 9 = c,
+// only the residue classes r mod 9 with (c + r) mod 9 in {0,1,3,6,7} are candidates, so
+// every lane of a row holds a real candidate (no lane is lost to the mod 9 sieve).
+constexpr uint32_t kPartItems = 2048;  // elements per block (256 threads x 8)
+
+__global__ __launch_bounds__(256) void k_class_count(const uint32_t* meta, uint64_t n,
+                                                     uint32_t cls, uint32_t* block_counts) {
+  __shared__ uint32_t lds4[4];
+  const uint64_t base = (uint64_t)blockIdx.x * kPartItems + threadIdx.x * 8;
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint64_t i = base + j;
+    if (i < n && meta_mod9(meta[i]) == cls) ++c;
+  }
+  uint32_t total;
+  block_excl_scan256(c, lds4, total);
+  if (threadIdx.x == 0) block_counts[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_class_write(const uint64_t* res, const uint64_t* num,
+                                                     const uint32_t* meta, uint64_t n,
+                                                     uint32_t cls, const uint64_t* offs,
+                                                     uint64_t out_base, uint64_t* ores,
+                                                     uint64_t* onum, uint32_t* ometa) {
+  __shared__ uint32_t lds4[4];
+  const uint64_t base = (uint64_t)blockIdx.x * kPartItems + threadIdx.x * 8;
+  uint32_t m[8];
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint64_t i = base + j;
+    m[j] = (i < n) ? meta[i] : 0xFFFFFFFFu;
+    if (i < n && meta_mod9(m[j]) == cls) ++c;
+  }
+  uint32_t total;
+  uint64_t o = out_base + offs[blockIdx.x] + block_excl_scan256(c, lds4, total);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint64_t i = base + j;
+    if (i < n && meta_mod9(m[j]) == cls) {
+      ores[o] = res[i];
+      onum[o] = num[i];
+      ometa[o] = m[j];
+      ++o;
+    }
+  }
+}
+
+// keys[t] = residue bound, kcls[t] = class: searches class kcls[t] of a class-major array
+// whose classes start at class_start[0..9].
+__global__ __launch_bounds__(256) void k_lower_bound(const uint64_t* a,
+                                                     const uint64_t* class_start,
+                                                     const uint64_t* keys, const uint32_t* kcls,
+                                                     uint64_t nkeys, uint64_t* out) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nkeys) return;
   const uint64_t key = keys[t];
-  uint64_t lo = 0, hi = n;
+  const uint32_t c = kcls[t];
+  uint64_t lo = class_start[c], hi = class_start[c + 1];
   while (lo < hi) {
     const uint64_t mid = lo + ((hi - lo) >> 1);
     if (a[mid] < key) lo = mid + 1;
@@ -311,11 +366,36 @@ hipError_t launch_sieve_fill(const SieveRoot& root, uint32_t p, const uint64_t* 
   return hipGetLastError();
 }
 
-hipError_t launch_lower_bound(const uint64_t* sorted, uint64_t n, const uint64_t* keys,
-                              uint64_t nkeys, uint64_t* out, hipStream_t s) {
+hipError_t launch_lower_bound(const uint64_t* sorted, const uint64_t* class_start,
+                              const uint64_t* keys, const uint32_t* kcls, uint64_t nkeys,
+                              uint64_t* out, hipStream_t s) {
   if (nkeys == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_lower_bound, dim3(blocks_for(nkeys, 256)), dim3(256), 0, s, sorted, n,
-                     keys, nkeys, out);
+  hipLaunchKernelGGL(k_lower_bound, dim3(blocks_for(nkeys, 256)), dim3(256), 0, s, sorted,
+                     class_start, keys, kcls, nkeys, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_class_partition(const uint64_t* res, const uint64_t* num, const uint32_t* meta,
+                                  uint64_t n, uint64_t* ores, uint64_t* onum, uint32_t* ometa,
+                                  uint32_t* counts, uint64_t* offs, uint64_t* class_start_host,
+                                  hipStream_t s) {
+  const uint64_t nb = (n + kPartItems - 1) / kPartItems;
+  uint64_t base = 0;
+  for (uint32_t c = 0; c < 9; ++c) {
+    class_start_host[c] = base;
+    if (nb == 0) continue;
+    hipLaunchKernelGGL(k_class_count, dim3((uint32_t)nb), dim3(256), 0, s, meta, n, c, counts);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nb, offs);
+    hipLaunchKernelGGL(k_class_write, dim3((uint32_t)nb), dim3(256), 0, s, res, num, meta, n, c,
+                       offs, base, ores, onum, ometa);
+    uint64_t cnt = 0;
+    hipError_t e = hipMemcpyAsync(&cnt, offs + nb, 8, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    base += cnt;
+  }
+  class_start_host[9] = base;
   return hipGetLastError();
 }
 

@@ -4,10 +4,6 @@
 using namespace cz;
 
 extern "C" {
-int collatz_verify_batch(const collatz_sieve*, uint64_t, const collatz_interval*,
-                         const collatz_batch_opts*, void*, collatz_range_result*) {
-  return fail(COLLATZ_EINVAL, "collatz_verify_batch: not available in this build");
-}
 int collatz_reference_run(const collatz_ref_params*, const collatz_node*, void*,
                           collatz_ref_result*) {
   return fail(COLLATZ_EINVAL, "collatz_reference_run: not available in this build");
