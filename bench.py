@@ -54,7 +54,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--form", choices=["plain", "persistent"], default="plain")
+    ap.add_argument("--form", choices=["plain", "persistent", "graph"], default="plain")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-chunks", type=int, default=48)
     args = ap.parse_args()
@@ -66,13 +66,19 @@ def main():
     form = cz.FORM_PLAIN if args.form == "plain" else cz.FORM_PERSISTENT
     sv = cz.Sieve(K)
     info = sv.info()
+
+    def step():
+        if args.form == "graph":
+            return sv.verify_batch([(LO, LO + WIDTH)], chunk_log2=CHUNK, use_graph=True)[0]
+        return sv.verify_range(LO, LO + WIDTH, CHUNK, form=form)
+
     for _ in range(args.warmup):
-        sv.verify_range(LO, LO + WIDTH, CHUNK, form=form)
+        step()
     dev_ms = []
     t0 = time.perf_counter()
     out = None
     for _ in range(args.steps):
-        out = sv.verify_range(LO, LO + WIDTH, CHUNK, form=form)
+        out = step()
         dev_ms.append(out.device_ms)
     dt = time.perf_counter() - t0
     ms_step = dt / args.steps * 1e3

@@ -7,14 +7,35 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
+#include "engine.hpp"
 #include "host_common.hpp"
 
 namespace cz {
 
 static thread_local std::string g_error;
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+bool trace_on() {
+  static const int on = [] {
+    const char* e = std::getenv("COLLATZ_TRACE");
+    return (e && e[0] == '1') ? 1 : 0;
+  }();
+  return on != 0;
+}
+
+Span::Span(const char* n) : name(n), t0(trace_on() ? now_ms() : 0) {}
+Span::~Span() {
+  if (trace_on()) std::fprintf(stderr, "[collatz] %-22s %8.3f ms\n", name, now_ms() - t0);
+}
 
 void set_error(const std::string& msg) { g_error = msg; }
 
@@ -396,6 +417,8 @@ int collatz_sieve_copy(const collatz_sieve* sv, uint64_t* r, uint64_t* number, u
 int collatz_sieve_free(collatz_sieve* sv) {
   if (!sv) return COLLATZ_OK;
   HipDeviceGuard guard(sv->device);
+  delete sv->engine;
+  sv->engine = nullptr;
   delete sv;
   return COLLATZ_OK;
 }

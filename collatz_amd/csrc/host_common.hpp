@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -16,6 +17,16 @@ namespace cz {
 using u128 = unsigned __int128;
 
 void set_error(const std::string& msg);
+
+// Host-side span timing, printed to stderr when COLLATZ_TRACE=1.
+bool trace_on();
+struct Span {
+  const char* name;
+  double t0;
+  explicit Span(const char* n);
+  ~Span();
+};
+double now_ms();
 int fail(int code, const std::string& msg);
 
 inline u128 mk128(uint64_t lo, uint64_t hi) { return ((u128)hi << 64) | lo; }
@@ -124,6 +135,7 @@ struct HipDeviceGuard {
   }
 };
 
+class Engine;
 }  // namespace cz
 
 struct collatz_sieve {
@@ -138,6 +150,10 @@ struct collatz_sieve {
   uint64_t class_start[10] = {0};       // survivors are stored class-major by r mod 9
   cz::DevBuf<uint64_t> class_start_dev;
   double build_ms = 0;
+  // Buffers and captured graph of the persistent form, created on first use; calls that
+  // share a sieve handle are serialised on engine_mu.
+  mutable cz::Engine* engine = nullptr;
+  mutable std::mutex engine_mu;
   cz::SieveArrays arrays() const {
     cz::SieveArrays a;
     a.res = res.p;

@@ -42,11 +42,26 @@ struct RowDesc {
   uint32_t cls;               // residue class r mod 9 of this row's survivors
 };
 
-// A unit of persistent work: sieve indices [i_lo, i_hi) of one row.
+// A unit of persistent work: piece `piece` of `pieces` equal parts of one row's survivors
+// (the row's index range is resolved on the device, so the host only fixes the split).
 struct TileDesc {
-  uint64_t i_lo, i_hi;
   uint32_t row;
+  uint32_t piece;
+  uint32_t pieces;
   uint32_t pad;
+};
+
+// Row bounds still to be looked up: i_lo / i_hi hold kResolveFlag | key index.
+constexpr uint64_t kResolveFlag = 1ull << 63;
+
+// Sizes of one persistent launch, read by the kernels from device memory so that one
+// captured graph serves launches of any size.
+struct Ctl {
+  uint64_t nrows;
+  uint64_t nslots;
+  uint32_t ntiles;
+  uint32_t tile_counter;
+  uint64_t pad;
 };
 
 // Partial or final record of one chunk, as the kernels produce it.  maxnum is the max of
@@ -118,6 +133,14 @@ hipError_t launch_sieve_fill(const SieveRoot& root, uint32_t p, const uint64_t* 
 hipError_t launch_lower_bound(const uint64_t* sorted, const uint64_t* class_start,
                               const uint64_t* keys, const uint32_t* kcls, uint64_t nkeys,
                               uint64_t* out, hipStream_t s);
+// Replaces flagged row bounds by their lower_bound in the row's class.
+hipError_t launch_rows_resolve(RowDesc* rows, uint64_t nrows, const uint64_t* sorted,
+                               const uint64_t* class_start, const uint64_t* keys,
+                               hipStream_t s);
+// Same, for ctl->nrows rows, with a fixed grid (graph-capturable).
+hipError_t launch_rows_resolve_ctl(RowDesc* rows, const Ctl* ctl, const uint64_t* sorted,
+                                   const uint64_t* class_start, const uint64_t* keys,
+                                   hipStream_t s);
 hipError_t launch_class_partition(const uint64_t* res, const uint64_t* num, const uint32_t* meta,
                                   uint64_t n, uint64_t* ores, uint64_t* onum, uint32_t* ometa,
                                   uint32_t* counts, uint64_t* offs, uint64_t* class_start_host,
@@ -128,13 +151,18 @@ hipError_t launch_verify_plain(int limbs, const SieveArrays& sv, const RowDesc* 
                                const Limits& lim, const uint64_t* pow3, DevRec* block_recs,
                                DeviceLists lists, hipStream_t s);
 hipError_t launch_verify_persistent(int limbs, const SieveArrays& sv, const RowDesc* rows,
-                                    const TileDesc* tiles, const uint32_t* ntiles,
-                                    uint32_t* tile_counter, uint32_t nblocks, uint32_t U,
-                                    const Limits& lim, const uint64_t* pow3, DevRec* tile_recs,
-                                    DeviceLists lists, hipStream_t s);
-// Clears counters with kernels (no memset nodes, so the sequence can be graph-captured).
-hipError_t launch_zero(uint32_t* p32, uint32_t n32, unsigned long long* p64, uint32_t n64,
-                       hipStream_t s);
+                                    const TileDesc* tiles, Ctl* ctl, uint32_t nblocks,
+                                    uint32_t U, const Limits& lim, const uint64_t* pow3,
+                                    DevRec* tile_recs, DeviceLists lists, hipStream_t s);
+// Clears the tile counter and the list counters with a kernel (no memset nodes, so the
+// sequence can be graph-captured).
+hipError_t launch_engine_reset(Ctl* ctl, unsigned long long* cnt, uint32_t ncnt, hipStream_t s);
+// Expands rows into tiles: row r owns tiles [rows[r].block_start, rows[r+1].block_start)
+// (the last row up to ctl->ntiles).
+hipError_t launch_make_tiles(const RowDesc* rows, const Ctl* ctl, TileDesc* tiles, hipStream_t s);
+// Per-slot reduction for ctl->nslots slots, fixed grid.
+hipError_t launch_reduce_slots(const DevRec* parts, const uint64_t* part_start, const Ctl* ctl,
+                               DevRec* out, hipStream_t s);
 hipError_t launch_reduce_chunks(const DevRec* parts, const uint64_t* part_start,
                                 uint32_t nchunks, DevRec* out, hipStream_t s);
 hipError_t launch_escalated(const SieveArrays& sv, const EscEntry* esc, uint64_t n,

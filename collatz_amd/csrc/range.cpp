@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "engine.hpp"
 #include "host_common.hpp"
 
 namespace cz {

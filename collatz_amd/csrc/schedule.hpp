@@ -24,25 +24,37 @@ __device__ __forceinline__ uint32_t alignr(uint32_t hi, uint32_t lo, uint32_t s)
   return __builtin_amdgcn_alignbit(hi, lo, s);  // low word of (hi:lo) >> (s & 31)
 }
 
-// 3^a for a in [0, 8]: 3^(a & 3) from packed bytes, times 81^(a >> 2).
-__device__ __forceinline__ uint32_t pow3_small(uint32_t a) {
-  const uint32_t lo = __builtin_amdgcn_ubfe(0x1B090301u, (a & 3u) << 3, 8);
-  const uint32_t hi = (a >= 8u) ? 6561u : ((a >= 4u) ? 81u : 1u);
-  return __umul24(lo, hi);
+// 3^a for a in [0, 8], from a 16-entry table in LDS (every kernel that iterates stages it
+// once per workgroup with stage_pow3()).
+struct Pow3Lds {
+  uint32_t v[16];
+};
+
+__device__ __forceinline__ void stage_pow3(Pow3Lds& t) {
+  if (threadIdx.x < 16) {
+    uint32_t x = 1;
+    for (uint32_t i = 0; i < threadIdx.x && i < 8; ++i) x *= 3;
+    t.v[threadIdx.x] = x;
+  }
+}
+
+__device__ __forceinline__ uint32_t addc(uint32_t a, uint32_t b, uint32_t cin, uint32_t& cout) {
+  return __builtin_addc(a, b, cin, &cout);
 }
 
 template <int L>
-__device__ __forceinline__ uint32_t odd_run(Num<L>& v) {
+__device__ __forceinline__ uint32_t odd_run(Num<L>& v, const Pow3Lds& p3) {
   const uint32_t a = __builtin_ctz(v.w[0] | 0x100u);
-  const uint32_t m = pow3_small(a);
-  uint64_t t = (uint64_t)alignr(v.w[1], v.w[0], a) * m;
-  v.w[0] = (uint32_t)t;
+  const uint32_t m = p3.v[a];
+  // Products of the shifted words by m (< 2^13), then one carry chain.
+  uint64_t p[L];
 #pragma unroll
-  for (int i = 1; i < L - 1; ++i) {
-    t = (uint64_t)alignr(v.w[i + 1], v.w[i], a) * m + (t >> 32);
-    v.w[i] = (uint32_t)t;
-  }
-  v.w[L - 1] = (uint32_t)((uint64_t)(v.w[L - 1] >> a) * m + (t >> 32));
+  for (int i = 0; i < L - 1; ++i) p[i] = (uint64_t)alignr(v.w[i + 1], v.w[i], a) * m;
+  p[L - 1] = (uint64_t)(v.w[L - 1] >> a) * m;
+  uint32_t c = 0;
+  v.w[0] = (uint32_t)p[0];
+#pragma unroll
+  for (int i = 1; i < L; ++i) v.w[i] = addc((uint32_t)p[i], (uint32_t)(p[i - 1] >> 32), c, c);
   return a;
 }
 
@@ -50,48 +62,43 @@ template <int L>
 __device__ __forceinline__ uint32_t even_run(Num<L>& v) {
   const uint32_t b = __builtin_ctz((v.w[0] - 1u) | 0x80000000u);
   const uint32_t add = (1u << b) - 1u;
-  uint64_t t = (uint64_t)v.w[0] + add;
   uint32_t x[L];
-  x[0] = (uint32_t)t;
+  uint32_t c = 0;
+  x[0] = addc(v.w[0], add, 0u, c);
 #pragma unroll
-  for (int i = 1; i < L; ++i) {
-    t = (uint64_t)v.w[i] + (t >> 32);
-    x[i] = (uint32_t)t;
-  }
+  for (int i = 1; i < L; ++i) x[i] = addc(v.w[i], 0u, c, c);
 #pragma unroll
   for (int i = 0; i < L - 1; ++i) v.w[i] = alignr(x[i + 1], x[i], b);
   v.w[L - 1] = x[L - 1] >> b;
   return b;
 }
 
-// x >= y over L words.
+__device__ __forceinline__ uint64_t pair(uint32_t lo, uint32_t hi) {
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// x >= y over L words, as 64-bit pair compares.
 template <int L>
 __device__ __forceinline__ bool ge_words(const uint32_t* x, const uint32_t* y) {
-  uint64_t t = (uint64_t)x[0] - y[0];
-#pragma unroll
-  for (int i = 1; i < L; ++i) t = (uint64_t)x[i] - y[i] - (uint32_t)(t >> 63);
-  return (t >> 63) == 0;
+  if constexpr (L == 4) {
+    const uint64_t xh = pair(x[2], x[3]), yh = pair(y[2], y[3]);
+    return xh > yh || (xh == yh && pair(x[0], x[1]) >= pair(y[0], y[1]));
+  } else {
+    const uint64_t xt = pair(x[4], x[5]), yt = pair(y[4], y[5]);
+    const uint64_t xh = pair(x[2], x[3]), yh = pair(y[2], y[3]);
+    return xt > yt || (xt == yt && (xh > yh || (xh == yh && pair(x[0], x[1]) >= pair(y[0], y[1]))));
+  }
 }
 
 // number < n where n has 4 words.
 template <int L>
 __device__ __forceinline__ bool lt_base(const Num<L>& v, const uint32_t* n) {
-  uint64_t t = (uint64_t)v.w[0] - n[0];
-#pragma unroll
-  for (int i = 1; i < 4; ++i) t = (uint64_t)v.w[i] - n[i] - (uint32_t)(t >> 63);
-  bool lt = (t >> 63) != 0;
-  if constexpr (L > 4) {
-    uint32_t hi = 0;
-#pragma unroll
-    for (int i = 4; i < L; ++i) hi |= v.w[i];
-    lt = lt && hi == 0;
-  }
+  const uint64_t vh = pair(v.w[2], v.w[3]), nh = pair(n[2], n[3]);
+  bool lt = vh < nh || (vh == nh && pair(v.w[0], v.w[1]) < pair(n[0], n[1]));
+  if constexpr (L > 4) lt = lt && (v.w[4] | v.w[5]) == 0;
   return lt;
 }
 
-__device__ __forceinline__ uint64_t pair(uint32_t lo, uint32_t hi) {
-  return ((uint64_t)hi << 32) | lo;
-}
 
 // Fast test "number may be >= the limit": a 64-bit compare on the top word pair (plus, for
 // three limbs with a cutoff below 2^128, the top two words being nonzero).

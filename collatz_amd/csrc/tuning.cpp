@@ -26,7 +26,7 @@ static uint32_t bitlen(u128 x) {
   return b;
 }
 
-static const Tuning kDefault = {2048, 8, 1};
+static const Tuning kDefault = {16384, 24, 2};
 
 Tuning choose_tuning(const collatz_sieve* sv, int limbs, u128 max_hi, uint64_t candidates,
                      uint32_t tile, uint32_t waves, uint32_t U) {
@@ -42,18 +42,19 @@ Tuning choose_tuning(const collatz_sieve* sv, int limbs, u128 max_hi, uint64_t c
     if (it != g_cache.end()) {
       t = it->second;
     } else {
-      // A range of the bucket's magnitude with about 2^23 candidates.
+      // A range of the bucket's magnitude with about 2^26 candidates (enough tiles for the
+      // largest tile size to show its tail).
       const uint32_t bits = std::max<uint32_t>(bitlen(max_hi), sv->k + 1);
       const double density = (double)sv->count / (double)((u128)1 << sv->k) * 5.0 / 9.0;
-      double w = (double)(1u << 23) / (density > 0 ? density : 1e-3);
+      double w = (double)(1u << 26) / (density > 0 ? density : 1e-3);
       u128 width = (u128)w;
       if (width < ((u128)1 << 20)) width = (u128)1 << 20;
       const u128 lo = (u128)1 << (bits - 1);
       const u128 hi = lo + width;
       double best = 1e30;
-      static const uint32_t tiles[] = {1024, 4096, 16384};
-      static const uint32_t wavess[] = {4, 8, 16};
-      static const uint32_t Us[] = {1, 2, 4};
+      static const uint32_t tiles[] = {4096, 16384, 65536};
+      static const uint32_t wavess[] = {16, 24, 28};
+      static const uint32_t Us[] = {1, 2, 3};
       for (uint32_t a : tiles)
         for (uint32_t b : wavess)
           for (uint32_t c : Us) {

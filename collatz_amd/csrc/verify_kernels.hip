@@ -54,9 +54,11 @@ struct SinkTwo {
   uint32_t* cm;
   __device__ __forceinline__ void offer(const Num<L>& v, const uint32_t* n) {
     if (ge_words<L>(v.w, cm)) {
+      // Value selects (a pointer select between the two sets would put them in scratch).
+      const uint32_t m1 = 0u - cset, m0 = ~m1;
       uint32_t ms[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) ms[i] = cset ? a1.mstart[i] : a0.mstart[i];
+      for (int i = 0; i < 4; ++i) ms[i] = (a1.mstart[i] & m1) | (a0.mstart[i] & m0);
       bool eq = true;
 #pragma unroll
       for (int i = 0; i < L; ++i) eq = eq && v.w[i] == cm[i];
@@ -162,11 +164,11 @@ __device__ __forceinline__ bool after_odd(Cand<L>& c, Sink& sink, const Limits& 
 // it has to be escalated.
 template <int L, bool LOWCUT, class Sink>
 __device__ __forceinline__ uint32_t one_check(Cand<L>& c, Sink& sink, const Limits& lim,
-                                              const DeviceLists& lists) {
-  uint32_t a1 = odd_run<L>(c.v);
+                                              const DeviceLists& lists, const Pow3Lds& p3) {
+  uint32_t a1 = odd_run<L>(c.v, p3);
   if (after_odd<L, LOWCUT>(c, sink, lim, lists)) return 2;
   uint32_t b1 = even_run<L>(c.v);
-  uint32_t a2 = odd_run<L>(c.v);
+  uint32_t a2 = odd_run<L>(c.v, p3);
   if (after_odd<L, LOWCUT>(c, sink, lim, lists)) return 2;
   uint32_t b2 = even_run<L>(c.v);
   c.steps += a1 + b1 + a2 + b2;
@@ -198,11 +200,12 @@ __device__ __forceinline__ bool cand_enter(const Cand<L>& c, Sink& sink) {
 // Runs candidate c to completion.  Returns false if it was escalated.
 template <int L, bool LOWCUT>
 __device__ __forceinline__ bool run_candidate(Cand<L>& c, Acc<L>& acc, const Limits& lim,
-                                              const DeviceLists& lists, uint32_t k) {
+                                              const DeviceLists& lists, uint32_t k,
+                                              const Pow3Lds& p3) {
   SinkOne<L> sink{acc};
   if (!cand_enter<L>(c, sink)) return false;
   for (uint32_t guard = 0; guard < kMaxChecks; ++guard) {
-    const uint32_t s = one_check<L, LOWCUT>(c, sink, lim, lists);
+    const uint32_t s = one_check<L, LOWCUT>(c, sink, lim, lists, p3);
     if (s == 1) {
       finish<L>(c, acc, k);
       return true;
@@ -221,6 +224,8 @@ __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowD
                                                       DevRec* recs, DeviceLists lists) {
   __shared__ uint32_t q3[41][6];
   __shared__ DevRec lrec[4];
+  __shared__ Pow3Lds p3s;
+  stage_pow3(p3s);
   const uint64_t blk = blockIdx.x;
   uint32_t lo = 0, hi = nrows;
   while (hi - lo > 1) {
@@ -246,7 +251,7 @@ __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowD
     c.slot = row.slot;
     const bool big = L == 4 ? (q3[f][4] | q3[f][5]) != 0 : false;
     if (!cand_start<L>(c, M, q3[f], big, row.qb_lo, row.qb_hi, r) ||
-        !run_candidate<L, LOWCUT>(c, acc, lim, lists, sv.k)) {
+        !run_candidate<L, LOWCUT>(c, acc, lim, lists, sv.k, p3s)) {
       push_escalate(lists, c.n, i, row.slot);
     }
   }
@@ -259,6 +264,9 @@ __global__ __launch_bounds__(256) void k_escalated(SieveArrays sv, const EscEntr
                                                    uint64_t n, Limits lim,
                                                    const uint64_t* pow3, DevRec* out,
                                                    DeviceLists lists) {
+  __shared__ Pow3Lds p3s;
+  stage_pow3(p3s);
+  __syncthreads();
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const EscEntry e = esc[t];
@@ -277,8 +285,8 @@ __global__ __launch_bounds__(256) void k_escalated(SieveArrays sv, const EscEntr
   Cand<6> c;
   c.slot = e.slot;
   cand_start<6>(c, M, q3, false, qb_lo, qb_hi, r);
-  if (lim.cutoff >= 128) run_candidate<6, false>(c, acc, lim, lists, k);
-  else run_candidate<6, true>(c, acc, lim, lists, k);
+  if (lim.cutoff >= 128) run_candidate<6, false>(c, acc, lim, lists, k, p3s);
+  else run_candidate<6, true>(c, acc, lim, lists, k, p3s);
   DevRec rec = acc_to_rec<6>(acc);
   rec.pad = e.slot;
   out[t] = rec;
@@ -286,179 +294,121 @@ __global__ __launch_bounds__(256) void k_escalated(SieveArrays sv, const EscEntr
 
 
 // ---- persistent kernel: waves with lane refill and work stealing ------------------------
-// Every wave is an independent worker.  It takes tiles (sieve index ranges of one row) from
-// a global counter, and keeps its 64 lanes busy: a lane whose candidate has descended takes
-// the next residue of the tile at the next refill point, so the wave never waits for its
-// slowest lane except at the very end.  Two tiles can be in flight per wave (two
-// accumulator sets); a tile's record is written when its last candidate is done.  Candidate
-// data is prefetched one refill ahead so the loads overlap the checks in between.
+// Every wave is an independent worker.  It takes tiles (a piece of one row's survivors)
+// from a global counter and keeps its 64 lanes busy: a lane whose candidate has descended
+// starts the next residue of the tile at the next check, so inside a tile the wave waits
+// for no lane; only the tile's last candidates run with some lanes idle.  Candidate data
+// is prefetched one check ahead so the loads overlap the arithmetic.  Row data (q << k,
+// the q * 3^f table) is the same for the whole tile and stays in SGPRs and LDS.
 constexpr int kWavesPerBlock = 4;
 
 template <int L>
 struct WaveTables {
-  uint32_t q3[2][41][L];   // q * 3^f per set
-  uint32_t big[2][41];     // q * 3^f does not fit L words
-  uint64_t qb[2][2];       // q << k per set
+  uint32_t q3[41][L];   // q * 3^f
+  uint32_t big[41];     // q * 3^f does not fit L words
 };
 
 template <int L, bool LOWCUT>
 __global__ __launch_bounds__(256) void k_verify_persistent(
-    SieveArrays sv, const RowDesc* rows, const TileDesc* tiles, const uint32_t* ntiles_ptr,
-    uint32_t* tile_counter, Limits lim, const uint64_t* pow3, DevRec* tile_recs,
-    DeviceLists lists, uint32_t U) {
+    SieveArrays sv, const RowDesc* rows, const TileDesc* tiles, Ctl* ctl, Limits lim,
+    const uint64_t* pow3, DevRec* tile_recs, DeviceLists lists, uint32_t U) {
   __shared__ WaveTables<L> tabs[kWavesPerBlock];
+  __shared__ Pow3Lds p3s;
+  stage_pow3(p3s);
+  __syncthreads();
   const uint32_t wv = threadIdx.x >> 6;
   const uint32_t lane = threadIdx.x & 63;
   WaveTables<L>& T = tabs[wv];
-  const uint32_t ntiles = *ntiles_ptr;
-
-  // Wave-uniform state.
-  int32_t set_tile0 = -1, set_tile1 = -1;
-  uint32_t set_slot0 = 0, set_slot1 = 0;
-  int32_t fill = -1;
-  uint64_t cur = 0, end = 0;
-  bool exhausted = false;
-
-  // Lane state.
-  Acc<L> acc0, acc1;
-  acc0.init();
-  acc1.init();
-  Cand<L> c;
-  bool act = false;
-  uint32_t cset = 0, cslot = 0;
-  uint64_t cidx = 0;
-  uint32_t cm[L];
-  bool pf = false;
-  uint64_t pf_r = 0, pf_M = 0, pf_idx = 0;
-  uint32_t pf_meta = 0, pf_set = 0, pf_slot = 0;
+  const uint32_t ntiles = ctl->ntiles;
+  uint32_t* tile_counter = &ctl->tile_counter;
 
   for (;;) {
-    // (1) Lanes without a candidate start their prefetched one.
-    if (!act && pf) {
-      pf = false;
-      const uint32_t f = meta_f(pf_meta);
-      const bool fits = cand_start<L>(c, pf_M, T.q3[pf_set][f], T.big[pf_set][f] != 0,
-                                      T.qb[pf_set][0], T.qb[pf_set][1], pf_r);
-      cset = pf_set;
-      cslot = pf_slot;
-      cidx = pf_idx;
-      c.slot = pf_slot;
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(tile_counter, 1u);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= ntiles) break;
+    const TileDesc td = tiles[t];
+    const RowDesc rw = rows[td.row];
+    const uint64_t len = rw.i_hi - rw.i_lo;
+    uint64_t cur = rw.i_lo + len * td.piece / td.pieces;
+    const uint64_t end = rw.i_lo + len * (td.piece + 1) / td.pieces;
+    if (lane < 41) {
+      uint32_t w6[6];
+      q_times_pow3(rw.q_lo, rw.q_hi, pow3[lane], w6);
 #pragma unroll
-      for (int i = 0; i < L; ++i) cm[i] = cset ? acc1.mx[i] : acc0.mx[i];
-      SinkTwo<L> sink{acc0, acc1, cset, cm};
-      if (fits && cand_enter<L>(c, sink)) {
-        act = true;
-      } else {
-        push_escalate(lists, c.n, cidx, cslot);
-      }
+      for (int i = 0; i < L; ++i) T.q3[lane][i] = w6[i];
+      uint32_t big = 0;
+#pragma unroll
+      for (int i = L; i < 6; ++i) big |= w6[i];
+      T.big[lane] = big != 0;
     }
-    // (2) Assign new prefetches from the tile cursor; open tiles as needed.
-    uint64_t need = __ballot(!pf);
-    while (need && !exhausted) {
-      if (cur >= end) {
-        // The fill tile is used up: open the next tile in the other set, if it is free.
-        const int32_t other = fill < 0 ? 0 : (fill ^ 1);
-        const int32_t other_tile = other ? set_tile1 : set_tile0;
-        if (other_tile >= 0) break;
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(tile_counter, 1u);
-        t = __builtin_amdgcn_readfirstlane(t);
-        if (t >= ntiles) {
-          exhausted = true;
-          fill = -1;
-          break;
-        }
-        const TileDesc td = tiles[t];
-        const RowDesc rw = rows[td.row];
-        if (lane < 41) {
-          uint32_t w6[6];
-          q_times_pow3(rw.q_lo, rw.q_hi, pow3[lane], w6);
-#pragma unroll
-          for (int i = 0; i < L; ++i) T.q3[other][lane][i] = w6[i];
-          uint32_t big = 0;
-#pragma unroll
-          for (int i = L; i < 6; ++i) big |= w6[i];
-          T.big[other][lane] = big != 0;
-        }
-        if (lane == 0) {
-          T.qb[other][0] = rw.qb_lo;
-          T.qb[other][1] = rw.qb_hi;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (other) {
-          set_tile1 = (int32_t)t;
-          set_slot1 = rw.slot;
-        } else {
-          set_tile0 = (int32_t)t;
-          set_slot0 = rw.slot;
-        }
-        fill = other;
-        cur = td.i_lo;
-        end = td.i_hi;
-        continue;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint64_t qb_lo = rw.qb_lo, qb_hi = rw.qb_hi;
+    const uint32_t slot = rw.slot;
+
+    Acc<L> acc;
+    acc.init();
+    SinkOne<L> sink{acc};
+    Cand<L> c;
+    bool act = false;
+    uint64_t cidx = 0;
+    // Prefetch slot.
+    bool pf = false;
+    uint64_t pf_r = 0, pf_M = 0, pf_idx = 0;
+    uint32_t pf_meta = 0;
+    for (;;) {
+      // (1) Lanes without a candidate start their prefetched one.
+      if (!act && pf) {
+        pf = false;
+        const uint32_t f = meta_f(pf_meta);
+        const bool fits = cand_start<L>(c, pf_M, T.q3[f], T.big[f] != 0, qb_lo, qb_hi, pf_r);
+        c.slot = slot;
+        cidx = pf_idx;
+        if (fits && cand_enter<L>(c, sink)) act = true;
+        else push_escalate(lists, c.n, cidx, slot);
       }
-      const uint32_t want = __popcll(need);
-      const uint64_t avail = end - cur;
-      const uint32_t take = avail < want ? (uint32_t)avail : want;
-      const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32),
-                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
-      if (!pf && rank < take) {
-        const uint64_t i = cur + rank;
-        pf_r = sv.res[i];
-        pf_M = sv.num[i];
-        pf_meta = sv.meta[i];
-        pf_idx = i;
-        pf_set = (uint32_t)fill;
-        pf_slot = fill ? set_slot1 : set_slot0;
-        pf = true;
-      }
-      cur += take;
-      need = __ballot(!pf);
-    }
-    // (3) Write the record of every set that takes no more candidates and has none left.
-#pragma unroll
-    for (uint32_t sset = 0; sset < 2; ++sset) {
-      const int32_t tile = sset ? set_tile1 : set_tile0;
-      if (tile < 0 || (int32_t)sset == fill) continue;
-      const bool busy = (act && cset == sset) || (pf && pf_set == sset);
-      if (__ballot(busy) != 0) continue;
-      DevRec r = acc_to_rec<L>(sset ? acc1 : acc0);
-      wave_merge(r);
-      if (lane == 0) tile_recs[tile] = r;
-      if (sset) {
-        acc1.init();
-        set_tile1 = -1;
-      } else {
-        acc0.init();
-        set_tile0 = -1;
-      }
-    }
-    if (exhausted && set_tile0 < 0 && set_tile1 < 0) break;
-    // (4) Up to U checks.
-    if (act) {
-      SinkTwo<L> sink{acc0, acc1, cset, cm};
-      for (uint32_t u = 0; u < U; ++u) {
-        const uint32_t st = one_check<L, LOWCUT>(c, sink, lim, lists);
-        if (st == 1) {
-          if (cset) finish<L>(c, acc1, sv.k);
-          else finish<L>(c, acc0, sv.k);
-          act = false;
-          break;
+      // (2) Lanes without a prefetch take the next survivors of the tile.
+      const uint64_t need = __ballot(!pf);
+      if (need && cur < end) {
+        const uint32_t want = __popcll(need);
+        const uint64_t avail = end - cur;
+        const uint32_t take = avail < want ? (uint32_t)avail : want;
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+        if (!pf && rank < take) {
+          const uint64_t i = cur + rank;
+          pf_r = sv.res[i];
+          pf_M = sv.num[i];
+          pf_meta = sv.meta[i];
+          pf_idx = i;
+          pf = true;
         }
-        if (st == 2) {
-          push_escalate(lists, c.n, cidx, cslot);
-          act = false;
-          break;
-        }
-        if (c.checks >= kMaxChecks) {
-          atomicAdd(lists.stall_count, 1ull);
-          act = false;
-          break;
+        cur += take;
+      }
+      if (__ballot(act) == 0 && __ballot(pf) == 0) break;
+      // (3) Up to U checks.
+      if (act) {
+        for (uint32_t u = 0; u < U; ++u) {
+          const uint32_t st = one_check<L, LOWCUT>(c, sink, lim, lists, p3s);
+          if (st == 1) {
+            finish<L>(c, acc, sv.k);
+            act = false;
+            break;
+          }
+          if (st == 2 || c.checks >= kMaxChecks) {
+            if (st == 2) push_escalate(lists, c.n, cidx, slot);
+            else atomicAdd(lists.stall_count, 1ull);
+            act = false;
+            break;
+          }
         }
       }
     }
+    DevRec r = acc_to_rec<L>(acc);
+    wave_merge(r);
+    if (lane == 0) tile_recs[t] = r;
   }
 }
 
@@ -498,37 +448,75 @@ hipError_t launch_verify_plain(int limbs, const SieveArrays& sv, const RowDesc* 
 
 
 hipError_t launch_verify_persistent(int limbs, const SieveArrays& sv, const RowDesc* rows,
-                                    const TileDesc* tiles, const uint32_t* ntiles,
-                                    uint32_t* tile_counter, uint32_t nblocks, uint32_t U,
-                                    const Limits& lim, const uint64_t* pow3, DevRec* tile_recs,
-                                    DeviceLists lists, hipStream_t s) {
+                                    const TileDesc* tiles, Ctl* ctl, uint32_t nblocks,
+                                    uint32_t U, const Limits& lim, const uint64_t* pow3,
+                                    DevRec* tile_recs, DeviceLists lists, hipStream_t s) {
   const dim3 grid(nblocks), block(256);
   if (limbs == 2)
-    hipLaunchKernelGGL((k_verify_persistent<4, false>), grid, block, 0, s, sv, rows, tiles,
-                       ntiles, tile_counter, lim, pow3, tile_recs, lists, U);
+    hipLaunchKernelGGL((k_verify_persistent<4, false>), grid, block, 0, s, sv, rows, tiles, ctl,
+                       lim, pow3, tile_recs, lists, U);
   else if (lim.cutoff >= 128)
-    hipLaunchKernelGGL((k_verify_persistent<6, false>), grid, block, 0, s, sv, rows, tiles,
-                       ntiles, tile_counter, lim, pow3, tile_recs, lists, U);
+    hipLaunchKernelGGL((k_verify_persistent<6, false>), grid, block, 0, s, sv, rows, tiles, ctl,
+                       lim, pow3, tile_recs, lists, U);
   else
-    hipLaunchKernelGGL((k_verify_persistent<6, true>), grid, block, 0, s, sv, rows, tiles,
-                       ntiles, tile_counter, lim, pow3, tile_recs, lists, U);
+    hipLaunchKernelGGL((k_verify_persistent<6, true>), grid, block, 0, s, sv, rows, tiles, ctl,
+                       lim, pow3, tile_recs, lists, U);
   return hipGetLastError();
 }
 
-__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+__global__ void k_engine_reset(Ctl* ctl, unsigned long long* cnt, uint32_t ncnt) {
+  if (threadIdx.x == 0) ctl->tile_counter = 0;
+  if (threadIdx.x < ncnt) cnt[threadIdx.x] = 0;
 }
 
-__global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+hipError_t launch_engine_reset(Ctl* ctl, unsigned long long* cnt, uint32_t ncnt, hipStream_t s) {
+  hipLaunchKernelGGL(k_engine_reset, dim3(1), dim3(64), 0, s, ctl, cnt, ncnt);
+  return hipGetLastError();
 }
 
-hipError_t launch_zero(uint32_t* p32, uint32_t n32, unsigned long long* p64, uint32_t n64,
-                       hipStream_t s) {
-  if (n32) hipLaunchKernelGGL(k_zero_u32, dim3((n32 + 255) / 256), dim3(256), 0, s, p32, n32);
-  if (n64) hipLaunchKernelGGL(k_zero_u64, dim3((n64 + 255) / 256), dim3(256), 0, s, p64, n64);
+__global__ __launch_bounds__(256) void k_make_tiles(const RowDesc* rows, const Ctl* ctl,
+                                                    TileDesc* tiles) {
+  const uint64_t n = ctl->nrows;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t b = rows[r].block_start;
+    const uint64_t e = (r + 1 < n) ? rows[r + 1].block_start : ctl->ntiles;
+    const uint32_t pieces = (uint32_t)(e - b);
+    for (uint32_t j = 0; j < pieces; ++j) {
+      TileDesc t;
+      t.row = (uint32_t)r;
+      t.piece = j;
+      t.pieces = pieces;
+      t.pad = 0;
+      tiles[b + j] = t;
+    }
+  }
+}
+
+hipError_t launch_make_tiles(const RowDesc* rows, const Ctl* ctl, TileDesc* tiles, hipStream_t s) {
+  hipLaunchKernelGGL(k_make_tiles, dim3(256), dim3(256), 0, s, rows, ctl, tiles);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_reduce_slots(const DevRec* parts,
+                                                      const uint64_t* part_start, const Ctl* ctl,
+                                                      DevRec* out) {
+  __shared__ DevRec lrec[4];
+  const uint64_t n = ctl->nslots;
+  for (uint64_t c = blockIdx.x; c < n; c += gridDim.x) {
+    DevRec r;
+    rec_zero(r);
+    for (uint64_t j = part_start[c] + threadIdx.x; j < part_start[c + 1]; j += 256)
+      rec_merge(r, parts[j]);
+    r = block_merge<4>(r, lrec);
+    if (threadIdx.x == 0) out[c] = r;
+    __syncthreads();
+  }
+}
+
+hipError_t launch_reduce_slots(const DevRec* parts, const uint64_t* part_start, const Ctl* ctl,
+                               DevRec* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_slots, dim3(1024), dim3(256), 0, s, parts, part_start, ctl, out);
   return hipGetLastError();
 }
 
