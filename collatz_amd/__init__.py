@@ -241,11 +241,12 @@ class Sieve:
 
     def verify_batch(self, intervals, chunk_log2: int = 24, limbs: int = 2,
                      cutoff_log2: int = 187, use_graph: bool = False, tile_residues: int = 0,
-                     waves_per_cu: int = 0, checks_per_refill: int = 0, stream=None):
+                     waves_per_cu: int = 0, checks_per_refill: int = 0, overflow_cap: int = 0,
+                     stream=None):
         m = len(intervals)
         ivs = (Interval * m)()
         outs = (RangeResult * m)()
-        chunk_arrays = []
+        chunk_arrays, ovf_arrays = [], []
         for i, (lo, hi) in enumerate(intervals):
             ivs[i].lo[0], ivs[i].lo[1] = lo & MASK64, lo >> 64
             ivs[i].hi[0], ivs[i].hi[1] = hi & MASK64, hi >> 64
@@ -254,13 +255,23 @@ class Sieve:
             chunk_arrays.append(arr)
             outs[i].chunks = arr.ctypes.data_as(C.POINTER(Chunk))
             outs[i].chunks_cap = n
+            if overflow_cap:
+                ov = np.zeros(2 * overflow_cap, np.uint64)
+                ovf_arrays.append(ov)
+                outs[i].overflow_starts = ov.ctypes.data_as(C.POINTER(C.c_uint64))
+                outs[i].overflow_cap = overflow_cap
         opts = BatchOpts(chunk_log2, limbs, cutoff_log2, int(use_graph), tile_residues,
                          waves_per_cu, checks_per_refill, 0)
         _check(load().collatz_verify_batch(self._h, m, ivs, C.byref(opts), stream, outs))
         result = []
         for i in range(m):
             total = np.frombuffer(bytearray(outs[i].total), CHUNK_DTYPE)[0]
-            result.append(RangeOutput(total, chunk_arrays[i], [], outs[i].escalated,
+            starts = []
+            if overflow_cap:
+                ov = ovf_arrays[i]
+                nov = min(outs[i].overflow_written, overflow_cap)
+                starts = [int(ov[2 * j]) | (int(ov[2 * j + 1]) << 64) for j in range(nov)]
+            result.append(RangeOutput(total, chunk_arrays[i], starts, outs[i].escalated,
                                       outs[i].device_ms, outs[i].checks))
         return result
 

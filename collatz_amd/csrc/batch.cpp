@@ -70,7 +70,9 @@ int verify_batch_impl(const collatz_sieve* sv, uint64_t n, const collatz_interva
   }
   if (rc) return rc;
 
-  bool short_buf = res.ovf_total > ovf_cap;
+  bool short_buf = false;
+  for (uint64_t i = 0; i < n; ++i)
+    if (outs[i].overflow_starts && res.ovf_total > ovf_cap) short_buf = true;
   for (uint64_t i = 0; i < n; ++i) {
     collatz_range_result* out = &outs[i];
     const uint64_t nc = slot_base[i + 1] - slot_base[i];
@@ -98,7 +100,8 @@ int verify_batch_impl(const collatz_sieve* sv, uint64_t n, const collatz_interva
       }
       out->overflow_written++;
     }
-    if ((out->chunks && nc > out->chunks_cap) || out->overflow_written > out->overflow_cap)
+    if ((out->chunks && nc > out->chunks_cap) ||
+        (out->overflow_starts && out->overflow_written > out->overflow_cap))
       short_buf = true;
   }
   for (uint32_t sl : res.esc_slots) {

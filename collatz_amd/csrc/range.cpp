@@ -188,7 +188,8 @@ static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeAr
       out->overflow_starts[2 * i + 1] = h[4 * i + 1];
     }
   }
-  if ((out->chunks && plan.nchunks > out->chunks_cap) || ovf_n > out->overflow_cap)
+  if ((out->chunks && plan.nchunks > out->chunks_cap) ||
+      (out->overflow_starts && ovf_n > out->overflow_cap))
     return fail(COLLATZ_ECAPACITY, "output buffer smaller than the result");
   return COLLATZ_OK;
 }

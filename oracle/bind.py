@@ -52,7 +52,7 @@ class Knobs(C.Structure):
 
 class TrajOut(C.Structure):
     _fields_ = [("hash", C.c_uint32), ("overflow", C.c_uint32), ("steps", C.c_uint64),
-                ("maxnum", C.c_uint64 * 3)]
+                ("maxnum", C.c_uint64 * 3), ("final", C.c_uint64 * 3)]
 
 
 class CaseOut(C.Structure):
@@ -182,12 +182,15 @@ def mod9_rows(mbi, A) -> np.ndarray:
     return rows
 
 
-def traj(base: int, num0: int, cutoff: int = 187, second_gt: bool = False):
+def traj(base: int, num0: int, cutoff: int = 187, second_gt: bool = False, final=False):
     b = (C.c_uint64 * 2)(base & MASK64, base >> 64)
     v = (C.c_uint64 * 3)(num0 & MASK64, (num0 >> 64) & MASK64, num0 >> 128)
     o = TrajOut()
     lib().or_traj(b, v, cutoff, int(second_gt), C.byref(o))
     mx = o.maxnum[0] | (o.maxnum[1] << 64) | (o.maxnum[2] << 128)
+    if final:
+        fin = o.final[0] | (o.final[1] << 64) | (o.final[2] << 128)
+        return o.hash, o.overflow, o.steps, mx, fin
     return o.hash, o.overflow, o.steps, mx
 
 

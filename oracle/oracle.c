@@ -375,6 +375,9 @@ void or_traj(const uint64_t base[2], const uint64_t num0[3], uint32_t cutoff, in
   out->maxnum[0] = mx.w[0];
   out->maxnum[1] = mx.w[1];
   out->maxnum[2] = mx.w[2];
+  out->final[0] = v.w[0];
+  out->final[1] = v.w[1];
+  out->final[2] = v.w[2];
 }
 
 /* ------------------------------------------------------------------------------------

@@ -70,6 +70,7 @@ typedef struct {
   uint32_t overflow;    /* number of times the value reached 2^cutoff */
   uint64_t steps;       /* T-steps taken after the start value, up to the passing check */
   uint64_t maxnum[3];   /* max of number (= T-value + 1) over start and odd-run ends */
+  uint64_t final[3];    /* number at the passing check */
 } or_traj_out;
 
 /* Range-mode record of one aligned chunk [j*2^C, (j+1)*2^C) intersected with the range. */

@@ -1,0 +1,53 @@
+"""The C-ABI library loads and exports every entry point include/collatz.h declares.
+No compute calls (this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    text = open(os.path.join(ROOT, "include", "collatz.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(collatz_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared()
+    for must in ("collatz_sieve_build", "collatz_verify_range", "collatz_verify_batch"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    import collatz_amd as cz
+    lib = cz.load()
+    names = declared()
+    assert names
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(cz.EXPORTS)
+
+
+def test_version_and_error_string():
+    import collatz_amd as cz
+    lib = cz.load()
+    assert lib.collatz_version() == 1
+    assert isinstance(lib.collatz_last_error(), bytes)
+
+
+def test_null_arguments_are_rejected():
+    import collatz_amd as cz
+    lib = cz.load()
+    assert lib.collatz_sieve_build(None, None) == cz.EINVAL
+    assert lib.collatz_verify_range(None, None, None, None, None, None) == cz.EINVAL
+    assert lib.collatz_verify_batch(None, 0, None, None, None, None) == cz.EINVAL
+    assert lib.collatz_sieve_query(None, None) == cz.EINVAL
+    assert lib.collatz_sieve_free(None) == cz.OK
+
+
+def test_struct_layouts_match_header():
+    import collatz_amd as cz
+    assert ctypes.sizeof(cz.Chunk) == 11 * 8 + 8
+    assert ctypes.sizeof(cz.SieveParams) == 16
+    assert ctypes.sizeof(cz.Interval) == 32

@@ -1,0 +1,72 @@
+"""collatz_verify_batch: many intervals in one persistent launch; each interval's records
+equal the oracle's (and so the plain form's)."""
+import random
+
+import pytest
+
+from gpu_util import assert_chunks_equal, assert_total_equal
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def s20():
+    import collatz_amd as cz
+    return cz.Sieve(20)
+
+
+@pytest.fixture(scope="module")
+def o20(oracle):
+    return oracle.sieve(20)
+
+
+def intervals(seed, n):
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n):
+        lo = rng.randrange(1, 1 << rng.choice([30, 40, 62, 64, 70]))
+        out.append((lo, lo + rng.randrange(1, 1 << rng.choice([10, 20, 24]))))
+    return out
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_equals_oracle(s20, o20, oracle, graph):
+    ivs = intervals(7, 80)
+    res = s20.verify_batch(ivs, chunk_log2=22, use_graph=graph)
+    for (lo, hi), r in zip(ivs, res):
+        chunks, total = oracle.verify_range(o20, lo, hi, 22)
+        assert_chunks_equal(r.chunks, chunks, f"[{lo:#x},{hi:#x})")
+        assert_total_equal(r.total, total)
+
+
+def test_graph_replay_repeats(s20):
+    """Replays of the captured launch sequence give identical records each time, with
+    other work between the replays."""
+    ivs = intervals(8, 30)
+    first = s20.verify_batch(ivs, chunk_log2=20, use_graph=True)
+    for i in range(3):
+        s20.verify_range(1, 1 << 26)
+        again = s20.verify_batch(ivs, chunk_log2=20, use_graph=True)
+        for a, b in zip(first, again):
+            assert_chunks_equal(a.chunks, b.chunks, f"replay {i}")
+
+
+@pytest.mark.parametrize("tile,waves,U", [(64, 2, 8), (256, 4, 1), (4096, 16, 2), (65536, 28, 3)])
+def test_launch_shapes(s20, o20, oracle, tile, waves, U):
+    lo, hi = 1 << 34, (1 << 34) + (1 << 27)
+    chunks, _ = oracle.verify_range(o20, lo, hi, 24)
+    r = s20.verify_batch([(lo, hi)], chunk_log2=24, tile_residues=tile, waves_per_cu=waves,
+                         checks_per_refill=U)[0]
+    assert_chunks_equal(r.chunks, chunks, f"shape {tile},{waves},{U}")
+
+
+def test_overlapping_and_duplicate_intervals(s20, o20, oracle):
+    ivs = [(1000, 5000000), (1000, 5000000), (4000000, 9000000), ((1 << 62) + 7, (1 << 62) + 99999)]
+    res = s20.verify_batch(ivs, chunk_log2=21, cutoff_log2=70, overflow_cap=4096)
+    for (lo, hi), r in zip(ivs, res):
+        chunks, total = oracle.verify_range(o20, lo, hi, 21, 70)
+        assert_chunks_equal(r.chunks, chunks)
+        assert len(r.overflow_starts) == int(total["overflow_count"])
+        assert all(lo <= n < hi for n in r.overflow_starts)
+        single = s20.verify_range(lo, hi, 21, cutoff_log2=70)
+        assert sorted(single.overflow_starts) == sorted(r.overflow_starts)

@@ -1,0 +1,139 @@
+"""collatz_verify_range (plain and persistent forms) against the oracle, bit for bit."""
+import random
+
+import numpy as np
+import pytest
+
+from gpu_util import assert_chunks_equal, assert_total_equal
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def s20():
+    import collatz_amd as cz
+    return cz.Sieve(20)
+
+
+@pytest.fixture(scope="module")
+def o20(oracle):
+    return oracle.sieve(20)
+
+
+@pytest.fixture(scope="module")
+def s32():
+    import collatz_amd as cz
+    return cz.Sieve(32)
+
+
+@pytest.fixture(scope="module")
+def o32(oracle):
+    return oracle.sieve(32)
+
+
+def check(sv, osv, oracle, lo, hi, C=24, limbs=2, cutoff=187, forms=(0, 1)):
+    chunks, total = oracle.verify_range(osv, lo, hi, C, cutoff, threads=16)
+    outs = []
+    for form in forms:
+        g = sv.verify_range(lo, hi, C, limbs, cutoff, form=form, overflow_cap=1 << 18)
+        assert_chunks_equal(g.chunks, chunks, f"[{lo:#x},{hi:#x}) form {form} limbs {limbs}")
+        assert_total_equal(g.total, total, "total")
+        outs.append(g)
+    return outs, total
+
+
+def test_config0_full(s20, o20, oracle):
+    """Every n in [1, 2^32), k = 20, 2^24-wide chunks (256 records)."""
+    outs, total = check(s20, o20, oracle, 1, 1 << 32)
+    assert len(outs[0].chunks) == 256
+    assert int(total["n_checked"]) == (1 << 32) - 1
+    assert int(total["overflow_count"]) == 0
+
+
+def test_config1_slice(s32, o32, oracle):
+    """A 2^30-wide slice of [2^40, 2^40 + 2^36) with the k = 32 sieve (64 chunk records)."""
+    lo = (1 << 40) + (1 << 35)
+    check(s32, o32, oracle, lo, lo + (1 << 30))
+
+
+def test_config1_full_properties(s32):
+    """The whole [2^40, 2^40 + 2^36): forms agree, records add up."""
+    lo, hi = 1 << 40, (1 << 40) + (1 << 36)
+    a = s32.verify_range(lo, hi, 24, form=0)
+    b = s32.verify_range(lo, hi, 24, form=1)
+    assert_chunks_equal(a.chunks, b.chunks, "plain vs persistent")
+    assert len(a.chunks) == 4096
+    assert int(a.chunks["n_checked"].sum()) == 1 << 36
+    assert int(a.chunks["count"].sum()) == int(a.total["count"])
+    h = 0
+    for x in a.chunks["hash"]:
+        h ^= int(x)
+    assert h == int(a.total["hash"])
+    assert a.escalated == 0 and int(a.total["overflow_count"]) == 0
+
+
+@pytest.mark.parametrize("lo,hi,C", [
+    (1, 2, 24), (3, 4, 24), (27, 28, 10), (0, 1 << 21, 20), (3, (1 << 26) + 12345, 20),
+    ((1 << 40) + 777, (1 << 40) + (1 << 25), 22), ((1 << 64) - (1 << 21), (1 << 64) + (1 << 21), 20),
+    ((1 << 100) + 3, (1 << 100) + (1 << 21), 19), ((1 << 127) - (1 << 20), (1 << 127), 16),
+    ((1 << 50) + 1, (1 << 50) + (1 << 23) + 5, 32), ((1 << 33) + 9, (1 << 33) + (1 << 22), 63),
+])
+def test_edge_ranges(s20, o20, oracle, lo, hi, C):
+    check(s20, o20, oracle, lo, hi, C)
+
+
+@pytest.mark.parametrize("cutoff", [64, 70, 100, 123, 127, 128, 150, 187])
+def test_cutoffs_and_overflow_lists(s20, o20, oracle, cutoff):
+    lo = (1 << 62) + 5
+    hi = lo + (1 << 21)
+    outs, total = check(s20, o20, oracle, lo, hi, 20, cutoff=cutoff)
+    a, b = outs
+    assert sorted(a.overflow_starts) == sorted(b.overflow_starts)
+    assert len(a.overflow_starts) == int(total["overflow_count"])
+    if cutoff == 70:
+        assert len(a.overflow_starts) > 0
+
+
+def test_three_limbs_and_escalation(s20, o20, oracle):
+    """Starts near 2^110 reach 2^123: two limbs escalate to three; three limbs from the
+    start give the same records."""
+    lo = (1 << 110) + 12345
+    hi = lo + (1 << 20)
+    outs2, total = check(s20, o20, oracle, lo, hi, 18, limbs=2)
+    outs3, _ = check(s20, o20, oracle, lo, hi, 18, limbs=3)
+    assert outs2[0].escalated > 0
+    assert outs3[0].escalated == 0
+    assert int(total["flags"]) & 2
+
+
+def test_limbs3_low_cutoff(s20, o20, oracle):
+    check(s20, o20, oracle, (1 << 62) + 5, (1 << 62) + (1 << 20), 20, limbs=3, cutoff=100)
+    check(s20, o20, oracle, (1 << 62) + 5, (1 << 62) + (1 << 20), 20, limbs=3, cutoff=125)
+
+
+def test_mod9_off(oracle):
+    import collatz_amd as cz
+    sv = cz.Sieve(20, mod9=False)
+    o = oracle.sieve(20)
+    lo, hi = (1 << 36) + 1, (1 << 36) + (1 << 24)
+    chunks, total = oracle.verify_range(o, lo, hi, 22, mod9=False)
+    g = sv.verify_range(lo, hi, 22)
+    assert_chunks_equal(g.chunks, chunks, "mod9 off")
+
+
+def test_random_ranges(s20, o20, oracle):
+    rng = random.Random(11)
+    for _ in range(25):
+        lo = rng.randrange(1, 1 << rng.choice([30, 45, 61, 64, 80]))
+        hi = lo + rng.randrange(1, 1 << rng.choice([8, 16, 22]))
+        check(s20, o20, oracle, lo, hi, rng.choice([12, 16, 20, 24]))
+
+
+def test_bad_arguments(s20):
+    import collatz_amd as cz
+    with pytest.raises(cz.CollatzError):
+        s20.verify_range(5, 5)
+    with pytest.raises(cz.CollatzError):
+        s20.verify_range(1, 100, cutoff_log2=60)
+    with pytest.raises(cz.CollatzError):
+        s20.verify_range(1, 100, limbs=4)
