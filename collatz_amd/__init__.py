@@ -25,6 +25,7 @@ FORM_PLAIN, FORM_PERSISTENT = 0, 1
 EXPORTS = [
     "collatz_sieve_build", "collatz_sieve_query", "collatz_sieve_copy", "collatz_sieve_free",
     "collatz_verify_range", "collatz_verify_batch", "collatz_reference_run",
+    "collatz_reference_tables",
     "collatz_server_create", "collatz_server_submit", "collatz_server_wait",
     "collatz_server_destroy", "collatz_version", "collatz_last_error",
 ]
@@ -142,6 +143,7 @@ def load(path: str = LIB_PATH):
     L.collatz_verify_batch.argtypes = [C.c_void_p, C.c_uint64, P(Interval), P(BatchOpts),
                                        C.c_void_p, P(RangeResult)]
     L.collatz_reference_run.argtypes = [P(RefParams), P(Node), C.c_void_p, P(RefResult)]
+    L.collatz_reference_tables.argtypes = [P(RefParams), P(C.c_uint32), P(C.c_uint32)]
     L.collatz_server_create.argtypes = [C.c_void_p, P(ServerOpts), P(C.c_void_p)]
     L.collatz_server_submit.argtypes = [C.c_void_p, P(Interval)]
     L.collatz_server_submit.restype = C.c_int64
@@ -274,6 +276,16 @@ class Sieve:
             result.append(RangeOutput(total, chunk_arrays[i], starts, outs[i].escalated,
                                       outs[i].device_ms, outs[i].checks))
         return result
+
+
+def reference_tables(N=45, A=10, B=24, num_bv=16, semantics=SEM_CUDA, device=-1):
+    """BV_i vectors (num_bv x 2^(B-5) u32 words) and mod 9 rows (9 x ceil(2^A/32))."""
+    bv = np.zeros((num_bv, 1 << (B - 5)), np.uint32)
+    m9 = np.zeros((9, max(1, (1 << A) // 32)), np.uint32)
+    params = RefParams(N, A, B, num_bv, semantics, 100, device, 0)
+    _check(load().collatz_reference_tables(C.byref(params), bv.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                           m9.ctypes.data_as(C.POINTER(C.c_uint32))))
+    return bv, m9
 
 
 def reference_run(t, N=45, A=10, B=24, num_bv=16, semantics=SEM_CUDA, cutoff_log2=100,

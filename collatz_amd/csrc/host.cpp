@@ -175,15 +175,6 @@ static bool root_pruned(const SieveRoot& r) {
 
 static uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
 
-// Leaves of the search1 tree below `root` at root.depth, sorted by residue.
-struct LeafSet {
-  DevBuf<uint64_t> res, num;
-  DevBuf<uint32_t> meta;
-  DevBuf<uint32_t> bits;   // dense bit array over the extensions h (padded)
-  uint64_t bits_bits = 0;
-  uint64_t count = 0;
-};
-
 int build_leaves(const SieveRoot& root, hipStream_t s, LeafSet& out) {
   if (root.depth < root.sigfig || root.depth - root.sigfig > 36 || root.depth > 40)
     return fail(COLLATZ_EINVAL, "sieve depth out of range");

@@ -122,6 +122,17 @@ int resolve_rows(const struct ::collatz_sieve* sv, RowPlan& plan, hipStream_t s)
 
 Limits make_limits(int limbs, uint32_t cutoff);
 
+// Leaves of the search1 tree below a root at root.depth, sorted by residue: the device
+// sieve (sieve_kernels.hip) behind collatz_sieve_build and reference mode.
+struct LeafSet {
+  DevBuf<uint64_t> res, num;
+  DevBuf<uint32_t> meta;
+  DevBuf<uint32_t> bits;   // dense bit array over the extensions h (padded)
+  uint64_t bits_bits = 0;
+  uint64_t count = 0;
+};
+int build_leaves(const SieveRoot& root, hipStream_t s, LeafSet& out);
+
 struct HipDeviceGuard {
   int prev = -1;
   bool changed = false;

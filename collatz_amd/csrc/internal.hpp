@@ -116,6 +116,20 @@ struct SieveArrays {
   uint32_t k;
 };
 
+// Look-ahead tables of reference mode on the device.
+struct RefTablesDev {
+  const uint32_t* bv;      // num_bv vectors of words_per_bv words (2^B bits + mirrored tail)
+  const uint32_t* mod9;    // 9 rows of nw words
+  const uint32_t* magic;   // 3^{-(tpo+i)} mod 2^32
+  uint64_t words_per_bv;
+  uint32_t nw;             // words per window (ceil(2^A / 32))
+  uint32_t num_bv;
+  uint32_t tpo;
+  uint32_t B;
+  uint32_t A;
+  uint32_t rust;
+};
+
 // ---- launchers (defined in the .hip files) ------------------------------------------
 hipError_t launch_sieve_prefix(const SieveRoot& root, uint32_t p, uint64_t* pnum,
                                uint32_t* pmeta, uint64_t* alive_bits, hipStream_t s);
@@ -163,6 +177,21 @@ hipError_t launch_make_tiles(const RowDesc* rows, const Ctl* ctl, TileDesc* tile
 // Per-slot reduction for ctl->nslots slots, fixed grid.
 hipError_t launch_reduce_slots(const DevRec* parts, const uint64_t* part_start, const Ctl* ctl,
                                DevRec* out, hipStream_t s);
+hipError_t launch_bv_build(uint32_t B, uint32_t num_bv, int32_t first, uint32_t sem,
+                           const uint32_t* magic, uint32_t* bv, uint64_t words_per_bv,
+                           uint32_t mirror_words, hipStream_t s);
+hipError_t launch_leaf_count(const RefTablesDev& T, const uint64_t* base, const uint64_t* num,
+                             const uint32_t* meta, uint64_t n, uint32_t* counts, hipStream_t s);
+hipError_t launch_leaf_write(const RefTablesDev& T, const uint64_t* base, const uint64_t* num,
+                             const uint32_t* meta, uint64_t n, const uint64_t* offs,
+                             uint64_t* cands, hipStream_t s);
+hipError_t launch_leaf_bins(const uint32_t* meta, uint64_t n, uint32_t tpo,
+                            unsigned long long* bins, hipStream_t s);
+hipError_t launch_ref_iterate(int limbs, const uint64_t* cands, uint64_t ncand,
+                              const uint64_t* base, const uint64_t* num, const uint32_t* meta,
+                              uint32_t mbi, const Limits& lim, const uint64_t* pow3, uint32_t tile,
+                              uint32_t nblocks, void* tot, uint64_t* ovf, uint64_t ovf_cap,
+                              hipStream_t s);
 hipError_t launch_reduce_chunks(const DevRec* parts, const uint64_t* part_start,
                                 uint32_t nchunks, DevRec* out, hipStream_t s);
 hipError_t launch_escalated(const SieveArrays& sv, const EscEntry* esc, uint64_t n,

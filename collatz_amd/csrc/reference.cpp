@@ -1,0 +1,295 @@
+// C ABI: reference mode -- the reference program itself, per case line, on the device.
+//
+// Replaces main() / search1 / launchkernel / compute_kernel of collatz_cudav7.cu:95-411 and
+// its OpenCL twin (collatz_openclv2.cpp:91-593): the leaves of the case line's subtree at
+// depth N - A come from the device sieve (search1 evaluated per residue), the candidates of
+// each leaf from its BV_i window and mod 9 row (Step 2), and the complete hash from the
+// schedule (Step 3).  The launch count it reports is the one the reference would make:
+// one per full batch of 2^16 leaves of a bin plus one flush per non-empty bin
+// (collatz_cudav7.cu:532-533, :177-179).
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "engine.hpp"
+#include "host_common.hpp"
+
+namespace cz {
+
+
+
+
+struct RefTotalsHost {
+  unsigned long long count;
+  unsigned long long overflow;
+  unsigned long long stall;
+  unsigned int hash;
+  unsigned int tile_counter;
+};
+
+struct RefTables {
+  DevBuf<uint32_t> bv, mod9, magic;
+  DevBuf<uint64_t> pow3;
+  RefTablesDev dev;
+};
+
+static std::mutex g_ref_mu;
+static std::map<std::tuple<int, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>,
+                std::unique_ptr<RefTables>>
+    g_ref_cache;
+
+// Tables alone can be built for deeper runs (N - A up to 120, e.g. main.rs's N = 72); a run
+// needs N - A <= 38 (leaves with 64-bit base and number).
+static int check_params(const collatz_ref_params* p, uint32_t max_mbi = 38) {
+  if (!p) return fail(COLLATZ_EINVAL, "null params");
+  if (p->A < 1 || p->A > 12) return fail(COLLATZ_EINVAL, "A (LASTBITS) must be in [1, 12]");
+  if (p->N < p->A + 4 || p->N - p->A > max_mbi)
+    return fail(COLLATZ_EINVAL, "N - A out of range (4..38 for a run, 4..120 for tables)");
+  if (p->B < 8 || p->B > 28) return fail(COLLATZ_EINVAL, "B must be in [8, 28]");
+  if (p->num_bv < 1 || p->num_bv > 32) return fail(COLLATZ_EINVAL, "num_bv must be in [1, 32]");
+  if (p->semantics > 2) return fail(COLLATZ_EINVAL, "unknown semantics");
+  if (p->cutoff_log2 < 64 || p->cutoff_log2 > 187)
+    return fail(COLLATZ_EINVAL, "cutoff_log2 must be in [64, 187]");
+  return COLLATZ_OK;
+}
+
+static uint32_t tpo_of(uint32_t mbi) { return mbi * kNumerator / kDenominator + 1; }
+
+static int get_tables(const collatz_ref_params* p, int device, RefTables*& out) {
+  std::lock_guard<std::mutex> lk(g_ref_mu);
+  const auto key = std::make_tuple(device, p->N, p->A, p->B, p->num_bv, p->semantics);
+  auto it = g_ref_cache.find(key);
+  if (it != g_ref_cache.end()) {
+    out = it->second.get();
+    return COLLATZ_OK;
+  }
+  auto t = std::make_unique<RefTables>();
+  const uint32_t mbi = p->N - p->A, tpo = tpo_of(mbi);
+  const uint32_t nw = std::max<uint32_t>(1, (1u << p->A) / 32);
+  const uint64_t wpb = (1ull << (p->B - 5)) + nw;
+  // 3^{-(tpo + i)} mod 2^32 (collatz_cudav7.cu:122-128).
+  std::vector<uint32_t> magic(p->num_bv);
+  for (uint32_t i = 0; i < p->num_bv; ++i) {
+    uint32_t m = 1;
+    for (uint32_t e = 0; e < tpo + i; ++e) m *= 2863311531u;
+    magic[i] = m;
+  }
+  // cutoff_0 = 485 tpo - 306 mbi (collatz_cudav7.cu:424, held in an i16 there).
+  const int32_t first = (int32_t)(int16_t)(kDenominator * tpo - kNumerator * mbi);
+  // Mod 9 rows: bit i of row r set iff (r + i 2^mbi) mod 9 in {0,1,3,6,7} (:482-494).
+  std::vector<uint32_t> mod9(9 * nw, 0);
+  const uint32_t step = (uint32_t)(((u128)1 << mbi) % 9);
+  for (uint32_t r = 0; r < 9; ++r) {
+    uint32_t red = r;
+    for (uint32_t i = 0; i < (1u << p->A); ++i) {
+      if ((kMod9Keep >> red) & 1u) mod9[r * nw + i / 32] |= 1u << (i % 32);
+      red += step;
+      if (red >= 9) red -= 9;
+    }
+  }
+  uint64_t p3[kPow3Max] = {0};
+  uint64_t x = 1;
+  for (int i = 0; i <= 40; ++i) {
+    p3[i] = x;
+    x *= 3;
+  }
+  CZ_HIP(t->bv.alloc(p->num_bv * wpb));
+  CZ_HIP(t->mod9.alloc(mod9.size()));
+  CZ_HIP(t->magic.alloc(p->num_bv));
+  CZ_HIP(t->pow3.alloc(kPow3Max));
+  CZ_HIP(hipMemset(t->bv.p, 0, t->bv.bytes()));
+  CZ_HIP(hipMemcpy(t->mod9.p, mod9.data(), mod9.size() * 4, hipMemcpyHostToDevice));
+  CZ_HIP(hipMemcpy(t->magic.p, magic.data(), magic.size() * 4, hipMemcpyHostToDevice));
+  CZ_HIP(hipMemcpy(t->pow3.p, p3, sizeof(p3), hipMemcpyHostToDevice));
+  CZ_HIP(launch_bv_build(p->B, p->num_bv, first, p->semantics, t->magic.p, t->bv.p, wpb, nw, nullptr));
+  CZ_HIP(hipDeviceSynchronize());
+  t->dev.bv = t->bv.p;
+  t->dev.mod9 = t->mod9.p;
+  t->dev.magic = t->magic.p;
+  t->dev.words_per_bv = wpb;
+  t->dev.nw = nw;
+  t->dev.num_bv = p->num_bv;
+  t->dev.tpo = tpo;
+  t->dev.B = p->B;
+  t->dev.A = p->A;
+  t->dev.rust = p->semantics == COLLATZ_SEM_RUST ? 1u : 0u;
+  out = t.get();
+  g_ref_cache[key] = std::move(t);
+  return COLLATZ_OK;
+}
+
+// Frontier of the subtree below `root` at depth root.sigfig + d, found by walking on the
+// host (search1's prunes, collatz_cudav7.cu:506-561).
+static void frontier(const SieveRoot& root, uint32_t d, std::vector<SieveRoot>& out) {
+  struct St {
+    uint64_t base, number, p3;
+    uint32_t s, f, de, ne;
+  };
+  std::vector<St> stack;
+  uint64_t p3 = 1;
+  for (uint32_t i = 0; i < root.threepower; ++i) p3 *= 3;
+  stack.push_back({root.base, root.number, p3, root.sigfig, root.threepower, root.doubleeven,
+                   root.numevens});
+  const uint32_t target = root.sigfig + d;
+  while (!stack.empty()) {
+    St x = stack.back();
+    stack.pop_back();
+    if (x.s == target) {
+      SieveRoot r = root;
+      r.base = x.base;
+      r.number = x.number;
+      r.sigfig = x.s;
+      r.threepower = x.f;
+      r.doubleeven = x.de;
+      r.numevens = x.ne;
+      out.push_back(r);
+      continue;
+    }
+    for (uint32_t bit = 0; bit < 2; ++bit) {
+      St y = x;
+      const uint64_t nb = x.number + (bit ? x.p3 : 0);
+      if (bit) y.base |= 1ull << x.s;
+      const uint32_t inplay = kNumerator * (x.s + 1) >= kDenominator * x.f;
+      if ((nb & 1) == 0) {
+        y.number = nb + nb / 2;
+        y.f += 1;
+        y.p3 *= 3;
+        y.de = (x.de | inplay) & ~2u;
+        y.ne = 0;
+      } else {
+        if (x.de >= 2) continue;
+        y.number = nb / 2 + 1;
+        y.de = x.de << 1;
+        y.ne = x.ne + 1;
+      }
+      y.s = x.s + 1;
+      if (root.rust && y.number <= y.base) continue;
+      if (kDenominator * y.f <= kNumerator * y.s) continue;
+      if (y.ne >= 2 && y.ne % 2 == 0 && kNumerator * (y.s - 1) >= kDenominator * (y.f - 1)) continue;
+      stack.push_back(y);
+    }
+  }
+}
+
+}  // namespace cz
+
+using namespace cz;
+
+extern "C" int collatz_reference_tables(const collatz_ref_params* params, uint32_t* bv,
+                                        uint32_t* mod9) {
+  int rc = check_params(params, 120);
+  if (rc) return rc;
+  HipDeviceGuard guard(params->device);
+  int dev = 0;
+  CZ_HIP(hipGetDevice(&dev));
+  RefTables* t = nullptr;
+  rc = get_tables(params, dev, t);
+  if (rc) return rc;
+  const uint64_t words = 1ull << (params->B - 5);
+  if (bv)
+    for (uint32_t i = 0; i < params->num_bv; ++i)
+      CZ_HIP(hipMemcpy(bv + i * words, t->bv.p + i * t->dev.words_per_bv, words * 4,
+                       hipMemcpyDeviceToHost));
+  if (mod9) CZ_HIP(hipMemcpy(mod9, t->mod9.p, t->mod9.bytes(), hipMemcpyDeviceToHost));
+  return COLLATZ_OK;
+}
+
+extern "C" int collatz_reference_run(const collatz_ref_params* params, const collatz_node* root,
+                                     void* stream, collatz_ref_result* out) {
+  int rc = check_params(params);
+  if (rc) return rc;
+  if (!root || !out) return fail(COLLATZ_EINVAL, "null argument");
+  const uint32_t mbi = params->N - params->A;
+  if (root->base[1] || root->number[1] || root->sigfig > mbi || root->threepower > 40)
+    return fail(COLLATZ_EINVAL, "case line out of range (base and number must fit 64 bits)");
+  HipDeviceGuard guard(params->device);
+  hipStream_t s = (hipStream_t)stream;
+  int dev = 0;
+  CZ_HIP(hipGetDevice(&dev));
+  RefTables* T = nullptr;
+  rc = get_tables(params, dev, T);
+  if (rc) return rc;
+  const double t0 = now_ms();
+
+  SieveRoot r;
+  r.base = root->base[0];
+  r.number = root->number[0];
+  r.sigfig = root->sigfig;
+  r.threepower = root->threepower;
+  r.doubleeven = root->doubleeven;
+  r.numevens = root->numevens;
+  r.depth = mbi;
+  r.rust = params->semantics == COLLATZ_SEM_RUST ? 1u : 0u;
+  std::vector<SieveRoot> roots;
+  const uint32_t e = mbi - r.sigfig;
+  if (e > 30) frontier(r, e - 30, roots);
+  else roots.push_back(r);
+
+  const int limbs = params->cutoff_log2 <= kWideLog2 ? 2 : 3;
+  const Limits lim = make_limits(limbs, params->cutoff_log2);
+  DevBuf<unsigned long long> dbins;
+  DevBuf<RefTotalsHost> dtot;
+  DevBuf<uint64_t> dovf;
+  const uint64_t ovf_cap = out->overflow_starts ? std::max<uint64_t>(out->overflow_cap, 1) : 0;
+  CZ_HIP(dbins.alloc(17));
+  CZ_HIP(dtot.alloc(1));
+  CZ_HIP(dovf.alloc(std::max<uint64_t>(1, 2 * ovf_cap)));
+  CZ_HIP(hipMemsetAsync(dbins.p, 0, 17 * 8, s));
+  CZ_HIP(hipMemsetAsync(dtot.p, 0, sizeof(RefTotalsHost), s));
+  DevBuf<uint32_t> counts;
+  DevBuf<uint64_t> offs, cands;
+  uint64_t leaves = 0;
+  const uint64_t batch = 1ull << 22;
+  const uint32_t nblocks = (uint32_t)device_cu_count(dev) * 28 / 4;
+  for (const SieveRoot& sr : roots) {
+    LeafSet ls;
+    rc = build_leaves(sr, s, ls);
+    if (rc) return rc;
+    leaves += ls.count;
+    CZ_HIP(launch_leaf_bins(ls.meta.p, ls.count, T->dev.tpo, dbins.p, s));
+    for (uint64_t b0 = 0; b0 < ls.count; b0 += batch) {
+      const uint64_t nb = std::min(batch, ls.count - b0);
+      CZ_HIP(counts.reserve(nb));
+      CZ_HIP(offs.reserve(nb + 1));
+      CZ_HIP(launch_leaf_count(T->dev, ls.res.p + b0, ls.num.p + b0, ls.meta.p + b0, nb, counts.p, s));
+      CZ_HIP(launch_scan(counts.p, nb, offs.p, s));
+      uint64_t ncand = 0;
+      CZ_HIP(hipMemcpyAsync(&ncand, offs.p + nb, 8, hipMemcpyDeviceToHost, s));
+      CZ_HIP(hipStreamSynchronize(s));
+      if (!ncand) continue;
+      CZ_HIP(cands.reserve(ncand));
+      CZ_HIP(launch_leaf_write(T->dev, ls.res.p + b0, ls.num.p + b0, ls.meta.p + b0, nb, offs.p,
+                               cands.p, s));
+      CZ_HIP(hipMemsetAsync(&dtot.p->tile_counter, 0, 4, s));
+      CZ_HIP(launch_ref_iterate(limbs, cands.p, ncand, ls.res.p + b0, ls.num.p + b0,
+                                ls.meta.p + b0, mbi, lim, T->pow3.p, 4096, nblocks, dtot.p,
+                                ovf_cap ? dovf.p : nullptr, ovf_cap, s));
+    }
+    CZ_HIP(hipStreamSynchronize(s));
+  }
+  RefTotalsHost tot;
+  unsigned long long bins[17];
+  CZ_HIP(hipMemcpyAsync(&tot, dtot.p, sizeof(tot), hipMemcpyDeviceToHost, s));
+  CZ_HIP(hipMemcpyAsync(bins, dbins.p, sizeof(bins), hipMemcpyDeviceToHost, s));
+  CZ_HIP(hipStreamSynchronize(s));
+  if (tot.stall) return fail(COLLATZ_EDEVICE, "a trajectory did not descend within the check bound");
+  out->complete_hash = tot.hash;
+  out->leaves = leaves;
+  out->candidates = tot.count;
+  out->overflow_count = tot.overflow;
+  out->launches = 0;
+  for (int b = 0; b < 17; ++b) out->bin_leaves[b] = bins[b];
+  for (int b = 0; b < 16; ++b) out->launches += (bins[b] + 65535) / 65536;
+  out->overflow_written = tot.overflow;
+  if (ovf_cap) {
+    const uint64_t nov = std::min<uint64_t>(tot.overflow, ovf_cap);
+    if (nov) CZ_HIP(hipMemcpy(out->overflow_starts, dovf.p, nov * 16, hipMemcpyDeviceToHost));
+  }
+  out->device_ms = now_ms() - t0;
+  if (ovf_cap && tot.overflow > ovf_cap)
+    return fail(COLLATZ_ECAPACITY, "overflow buffer smaller than the result");
+  return COLLATZ_OK;
+}

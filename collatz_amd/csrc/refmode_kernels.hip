@@ -1,0 +1,327 @@
+// Reference mode: the reference program's Steps 2 and 3 for the leaves of a case line.
+//
+//   k_bv_build       precompute() of the BV_i look-ahead bit vectors (collatz_cudav7.cu:423-480,
+//                    collatz_openclv2.cpp:294-353, main.rs:125-181): one thread per residue
+//                    n0 < 2^B, its B-step parity word and max dip; a bit at n0 * 3^{-f} mod 2^B
+//                    in every BV_i whose cutoff it meets.  The first words of each vector are
+//                    mirrored after the end so windows never wrap.
+//   k_leaf_count /   Step 2 (collatz_cudav7.cu:219-235): per leaf, the 2^A-bit window of its
+//   k_leaf_write     BV_i AND the mod 9 row of its base; the set bits are the candidates, counted
+//                    then written as (leaf, a) pairs at scanned offsets.
+//   k_ref_iterate    Step 3: persistent waves iterate the candidate list with the schedule of
+//                    compute_kernel (collatz_cudav7.cu:280-366) and fold hash, count and overflow
+//                    count with one atomic per wave and tile.
+#include "schedule.hpp"
+
+namespace cz {
+
+
+// doubleeven(): main.rs:101-107.  The C++ programs evaluate `x & 0b11 == 0` as `x & 0`
+// (collatz_cudav7.cu:418), which is never true; only main.rs tests the two bits.
+__device__ __forceinline__ bool dbl_even(uint32_t upordown, uint32_t i, uint32_t B, bool rust) {
+  if (!rust) return false;
+  const uint32_t a = __builtin_ctz((upordown >> i) + 1u);
+  if (i + a + 2 >= B) return false;
+  return ((upordown >> (i + a)) & 3u) == 0;
+}
+
+__device__ __forceinline__ bool gt_sem(int32_t x, int32_t y, bool uns) {
+  return uns ? ((uint32_t)x > (uint32_t)y) : (x > y);
+}
+
+__global__ __launch_bounds__(256) void k_bv_build(uint32_t B, uint32_t num_bv, int32_t first,
+                                                  uint32_t sem, const uint32_t* magic,
+                                                  uint32_t* bv, uint64_t words_per_bv,
+                                                  uint32_t mirror_words) {
+  const uint64_t n0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n0 >= (1ull << B)) return;
+  const bool rust = sem == 1, uns = sem == 2;
+  uint32_t upordown = 0;
+  uint64_t n = n0;
+  for (uint32_t i = 0; i < B; ++i) {
+    if ((n & 1) == 0) {
+      n = n + (n >> 1);
+      upordown |= 1u << i;
+    } else {
+      n = (n >> 1) + 1;
+    }
+  }
+  int32_t cur = 0, mx = 0;
+  uint32_t ne = 0;
+  for (uint32_t i = 0; i < B; ++i) {
+    if (upordown & (1u << i)) {
+      if (gt_sem(cur + (int32_t)kNumerator, mx, uns) && dbl_even(upordown, i, B, rust))
+        mx = cur + (int32_t)kNumerator;
+      cur -= (int32_t)(kDenominator - kNumerator);
+      ne = 0;
+    } else {
+      cur += (int32_t)kNumerator;
+      ne += 1;
+      if ((ne & 1) == 0 && gt_sem(cur + (int32_t)(kDenominator - kNumerator), mx, uns))
+        mx = cur + (int32_t)(kDenominator - kNumerator);
+    }
+    if (cur > mx) mx = cur;
+  }
+  const uint32_t mask = (1u << B) - 1u;
+  for (uint32_t a = 0; a < num_bv; ++a) {
+    if (mx <= first + (int32_t)(a * kDenominator)) {
+      const uint32_t idx = ((uint32_t)n0 * magic[a]) & mask;
+      uint32_t* v = bv + a * words_per_bv;
+      atomicOr(&v[idx >> 5], 1u << (idx & 31));
+      if ((idx >> 5) < mirror_words) atomicOr(&v[(1u << (B - 5)) + (idx >> 5)], 1u << (idx & 31));
+    }
+  }
+}
+
+// Word w of a leaf's candidate mask: bit j set iff a = 32 w + j is a candidate.
+__device__ __forceinline__ uint32_t leaf_mask_word(const RefTablesDev& T, uint64_t base,
+                                                   uint64_t number, uint32_t meta, uint32_t w) {
+  const uint32_t f = meta_f(meta);
+  uint32_t m = T.mod9[meta_mod9(meta) * T.nw + w];
+  const uint32_t b = f - T.tpo;
+  if (b < T.num_bv) {
+    const uint32_t idx = (T.magic[b] * (uint32_t)number) & ((1u << T.B) - 1u);
+    const uint32_t* v = T.bv + (uint64_t)b * T.words_per_bv + (idx >> 5) + w;
+    m &= __builtin_amdgcn_alignbit(v[1], v[0], idx & 31);
+  }
+  if (T.rust) {
+    // main.rs:309-313, with a = 32 w + j, so a mod 4 = j mod 4.
+    const uint32_t de = meta_de(meta);
+    const uint32_t nl = (uint32_t)number;
+    if (de == 2) m &= (nl & 1) ? 0xAAAAAAAAu : 0x55555555u;
+    if (de == 1) {
+      const uint32_t bad = (f % 2 == 0) ? ((1u - nl) & 3u) : ((3u * (1u - nl)) & 3u);
+      m &= ~(0x11111111u << bad);
+    }
+  }
+  return m;
+}
+
+__global__ __launch_bounds__(256) void k_leaf_count(RefTablesDev T, const uint64_t* base,
+                                                    const uint64_t* num, const uint32_t* meta,
+                                                    uint64_t nleaves, uint32_t* counts) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nleaves) return;
+  const uint64_t b = base[t], n = num[t];
+  const uint32_t mt = meta[t];
+  uint32_t c = 0;
+  for (uint32_t w = 0; w < T.nw; ++w) c += __popc(leaf_mask_word(T, b, n, mt, w));
+  counts[t] = c;
+}
+
+// Candidate entry: leaf (bits 0-31), a (bits 32-47), CPU-leaf flag (bit 48).
+__global__ __launch_bounds__(256) void k_leaf_write(RefTablesDev T, const uint64_t* base,
+                                                    const uint64_t* num, const uint32_t* meta,
+                                                    uint64_t nleaves, const uint64_t* offs,
+                                                    uint64_t* cands) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nleaves) return;
+  const uint64_t b = base[t], n = num[t];
+  const uint32_t mt = meta[t];
+  // Leaves without a bit vector run on the CPU in the reference (search2/search3,
+  // collatz_cudav7.cu:517-519), whose second overflow test is '>' (:702).
+  const uint64_t cpu = (meta_f(mt) - T.tpo) >= 16 ? (1ull << 48) : 0;
+  uint64_t o = offs[t];
+  for (uint32_t w = 0; w < T.nw; ++w) {
+    uint32_t m = leaf_mask_word(T, b, n, mt, w);
+    while (m) {
+      const uint32_t j = __builtin_ctz(m);
+      m &= m - 1;
+      cands[o++] = (uint64_t)t | ((uint64_t)(32 * w + j) << 32) | cpu;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_leaf_bins(const uint32_t* meta, uint64_t n, uint32_t tpo,
+                                                   unsigned long long* bins) {
+  __shared__ unsigned long long lb[17];
+  if (threadIdx.x < 17) lb[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t b = meta_f(meta[t]) - tpo;
+    atomicAdd(&lb[b < 16 ? b : 16], 1ull);
+  }
+  __syncthreads();
+  if (threadIdx.x < 17 && lb[threadIdx.x]) atomicAdd(&bins[threadIdx.x], lb[threadIdx.x]);
+}
+
+struct RefTotals {
+  unsigned long long count;
+  unsigned long long overflow;
+  unsigned long long stall;
+  unsigned int hash;
+  unsigned int tile_counter;
+};
+
+// num >= 2^c, or (strict) num > 2^c.
+template <int L>
+__device__ __forceinline__ bool reaches(const Num<L>& v, uint32_t c, bool strict) {
+  if (!ge_pow2<L>(v, c)) return false;
+  if (!strict) return true;
+  const uint32_t cw = c >> 5, cb = c & 31u;
+  bool exact = true;
+#pragma unroll
+  for (int i = 0; i < L; ++i) exact = exact && v.w[i] == ((uint32_t)i == cw ? (1u << cb) : 0u);
+  return !exact;
+}
+
+template <int L, bool LOWCUT>
+__global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint64_t ncand,
+                                                     const uint64_t* base, const uint64_t* num,
+                                                     const uint32_t* meta, uint32_t mbi,
+                                                     Limits lim, const uint64_t* pow3,
+                                                     uint32_t tile, RefTotals* tot,
+                                                     uint64_t* ovf, uint64_t ovf_cap) {
+  __shared__ Pow3Lds p3s;
+  stage_pow3(p3s);
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t ntiles = (ncand + tile - 1) / tile;
+  const uint32_t cutoff = lim.cutoff;
+  const uint64_t thr = lim.fast_thr;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(&tot->tile_counter, 1u);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= ntiles) break;
+    uint64_t cur = (uint64_t)t * tile;
+    const uint64_t end = cur + tile < ncand ? cur + tile : ncand;
+    uint32_t h = 0, cnt = 0, ov = 0, stall = 0;
+    bool act = false, strict = false;
+    Num<L> v;
+    uint32_t n[4];
+    uint32_t checks = 0;
+    for (;;) {
+      const uint64_t need = __ballot(!act);
+      if (need && cur < end) {
+        const uint32_t want = __popcll(need);
+        const uint64_t avail = end - cur;
+        const uint32_t take = avail < want ? (uint32_t)avail : want;
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+        if (!act && rank < take) {
+          const uint64_t c = cands[cur + rank];
+          const uint32_t leaf = (uint32_t)c;
+          const uint64_t a = (c >> 32) & 0xFFFFu;
+          strict = (c >> 48) & 1u;
+          const uint64_t nn = base[leaf] + (a << mbi);
+          const uint32_t f = meta_f(meta[leaf]);
+          // number + a * 3^f (Lemma 2.3; collatz_cudav7.cu:262-278)
+          const unsigned __int128 x = (unsigned __int128)a * pow3[f] + num[leaf];
+          v.w[0] = (uint32_t)x;
+          v.w[1] = (uint32_t)(x >> 32);
+          v.w[2] = (uint32_t)(x >> 64);
+          v.w[3] = (uint32_t)(x >> 96);
+#pragma unroll
+          for (int i = 4; i < L; ++i) v.w[i] = 0;
+          n[0] = (uint32_t)nn;
+          n[1] = (uint32_t)(nn >> 32);
+          n[2] = n[3] = 0;
+          checks = 0;
+          act = true;
+          cnt += 1;
+        }
+        cur += take;
+      }
+      if (__ballot(act) == 0) break;
+      if (act) {
+        bool done = false;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+          odd_run<L>(v, p3s);
+          if (over_fast<L, LOWCUT>(v, thr) && reaches<L>(v, cutoff, strict && half == 1)) {
+            ov += 1;
+            if (ovf) {
+              const unsigned long long i = atomicAdd(&tot->overflow, 1ull);
+              if (i < ovf_cap) {
+                ovf[2 * i] = pair(n[0], n[1]);
+                ovf[2 * i + 1] = pair(n[2], n[3]);
+              }
+            }
+            v.w[0] = 2;
+#pragma unroll
+            for (int i = 1; i < L; ++i) v.w[i] = 0;
+          }
+          even_run<L>(v);
+        }
+        h ^= v.w[0];
+        done = lt_base<L>(v, n);
+        if (++checks >= kMaxChecks && !done) {
+          stall += 1;
+          done = true;
+        }
+        if (done) act = false;
+      }
+    }
+    // One atomic per wave and tile.
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      h ^= __shfl_xor(h, m, 64);
+      cnt += __shfl_xor(cnt, m, 64);
+      stall += __shfl_xor(stall, m, 64);
+      if (!ovf) ov += __shfl_xor(ov, m, 64);
+    }
+    if (lane == 0) {
+      atomicXor(&tot->hash, h);
+      atomicAdd(&tot->count, (unsigned long long)cnt);
+      if (stall) atomicAdd(&tot->stall, (unsigned long long)stall);
+      if (!ovf && ov) atomicAdd(&tot->overflow, (unsigned long long)ov);
+    }
+  }
+}
+
+static inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+hipError_t launch_bv_build(uint32_t B, uint32_t num_bv, int32_t first, uint32_t sem,
+                           const uint32_t* magic, uint32_t* bv, uint64_t words_per_bv,
+                           uint32_t mirror_words, hipStream_t s) {
+  hipLaunchKernelGGL(k_bv_build, dim3(blocks_for(1ull << B, 256)), dim3(256), 0, s, B, num_bv,
+                     first, sem, magic, bv, words_per_bv, mirror_words);
+  return hipGetLastError();
+}
+
+hipError_t launch_leaf_count(const RefTablesDev& T, const uint64_t* base, const uint64_t* num,
+                             const uint32_t* meta, uint64_t n, uint32_t* counts, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_leaf_count, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, base, num, meta,
+                     n, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_leaf_write(const RefTablesDev& T, const uint64_t* base, const uint64_t* num,
+                             const uint32_t* meta, uint64_t n, const uint64_t* offs,
+                             uint64_t* cands, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_leaf_write, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, base, num, meta,
+                     n, offs, cands);
+  return hipGetLastError();
+}
+
+hipError_t launch_leaf_bins(const uint32_t* meta, uint64_t n, uint32_t tpo,
+                            unsigned long long* bins, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_leaf_bins, dim3(512), dim3(256), 0, s, meta, n, tpo, bins);
+  return hipGetLastError();
+}
+
+// limbs = 2 requires cutoff <= 123 (no escalation in this mode); three limbs otherwise.
+hipError_t launch_ref_iterate(int limbs, const uint64_t* cands, uint64_t ncand,
+                              const uint64_t* base, const uint64_t* num, const uint32_t* meta,
+                              uint32_t mbi, const Limits& lim, const uint64_t* pow3, uint32_t tile,
+                              uint32_t nblocks, void* tot, uint64_t* ovf, uint64_t ovf_cap,
+                              hipStream_t s) {
+  if (!ncand) return hipSuccess;
+  if (limbs == 2)
+    hipLaunchKernelGGL((k_ref_iterate<4, false>), dim3(nblocks), dim3(256), 0, s, cands, ncand,
+                       base, num, meta, mbi, lim, pow3, tile, (RefTotals*)tot, ovf, ovf_cap);
+  else if (lim.cutoff >= 128)
+    hipLaunchKernelGGL((k_ref_iterate<6, false>), dim3(nblocks), dim3(256), 0, s, cands, ncand,
+                       base, num, meta, mbi, lim, pow3, tile, (RefTotals*)tot, ovf, ovf_cap);
+  else
+    hipLaunchKernelGGL((k_ref_iterate<6, true>), dim3(nblocks), dim3(256), 0, s, cands, ncand,
+                       base, num, meta, mbi, lim, pow3, tile, (RefTotals*)tot, ovf, ovf_cap);
+  return hipGetLastError();
+}
+
+}  // namespace cz

@@ -4,10 +4,6 @@
 using namespace cz;
 
 extern "C" {
-int collatz_reference_run(const collatz_ref_params*, const collatz_node*, void*,
-                          collatz_ref_result*) {
-  return fail(COLLATZ_EINVAL, "collatz_reference_run: not available in this build");
-}
 int collatz_server_create(const collatz_sieve*, const collatz_server_opts*, collatz_server**) {
   return fail(COLLATZ_EINVAL, "collatz_server_create: not available in this build");
 }

@@ -181,6 +181,11 @@ typedef struct {
 int collatz_reference_run(const collatz_ref_params* params, const collatz_node* root,
                           void* stream, collatz_ref_result* out);
 
+/* The look-ahead tables reference mode uses, built on the device: precompute()
+ * (collatz_cudav7.cu:423-495).  bv receives num_bv vectors of 2^B bits as 2^(B-5) u32 words
+ * each (the mirrored tail is not copied), mod9 nine rows of ceil(2^A / 32) words. */
+int collatz_reference_tables(const collatz_ref_params* params, uint32_t* bv, uint32_t* mod9);
+
 /* ---- served form ---------------------------------------------------------------------
  * A request loop: submit ranges at any time; a host thread packs the pending work of all
  * admitted requests into back-to-back persistent launches (continuous batching) while the
