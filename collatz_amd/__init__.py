@@ -278,6 +278,51 @@ class Sieve:
         return result
 
 
+class Server:
+    """Served form: submit ranges at any time; a worker thread batches the pending work of all
+    admitted requests into back-to-back persistent launches (collatz_server_*)."""
+
+    def __init__(self, sieve: Sieve, chunk_log2=24, cutoff_log2=187, max_device_bytes=0,
+                 slice_candidates=0, max_inflight=0):
+        h = C.c_void_p()
+        opts = ServerOpts(chunk_log2, cutoff_log2, max_device_bytes, slice_candidates,
+                          max_inflight, 0)
+        _check(load().collatz_server_create(sieve.handle, C.byref(opts), C.byref(h)))
+        self._h = h
+        self.sieve = sieve
+        self.chunk_log2 = chunk_log2
+
+    def submit(self, lo: int, hi: int) -> int:
+        iv = Interval((C.c_uint64 * 2)(lo & MASK64, lo >> 64), (C.c_uint64 * 2)(hi & MASK64, hi >> 64))
+        t = load().collatz_server_submit(self._h, C.byref(iv))
+        if t < 0:
+            _check(int(t))
+        self._meta = getattr(self, "_meta", {})
+        self._meta[t] = (lo, hi)
+        return t
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0):
+        lo, hi = self._meta.pop(ticket)
+        n = ((hi - 1) >> self.chunk_log2) - (lo >> self.chunk_log2) + 1
+        chunks = np.zeros(n, CHUNK_DTYPE)
+        out = RequestResult()
+        _check(load().collatz_server_wait(self._h, ticket, timeout_ms, C.byref(out),
+                                          chunks.ctypes.data_as(C.POINTER(Chunk)), n))
+        total = np.frombuffer(bytearray(out.total), CHUNK_DTYPE)[0]
+        return total, chunks, (out.submit_ms, out.admit_ms, out.done_ms)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().collatz_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def reference_tables(N=45, A=10, B=24, num_bv=16, semantics=SEM_CUDA, device=-1):
     """BV_i vectors (num_bv x 2^(B-5) u32 words) and mod 9 rows (9 x ceil(2^A/32))."""
     bv = np.zeros((num_bv, 1 << (B - 5)), np.uint32)

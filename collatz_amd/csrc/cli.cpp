@@ -1,0 +1,178 @@
+// collatz_run: command-line front end with the reference program's process protocol.
+//
+//   echo "3 2 2 9 1 0" | collatz_run [--N 45] [--A 10] [--cutoff 100] [--semantics cuda]
+// reads one case line "base sigfig threepower number doubleeven numevens" from stdin
+// (collatz_cudav7.cu:95-101), runs the whole reference method on the GPU, prints every start
+// whose trajectory reached 2^cutoff as hex on stdout (collatz_cudav7.cu:298) and the summary
+// line on stderr (collatz_cudav7.cu:181-182; the OpenCL program's two-line form with
+// --semantics opencl, collatz_openclv2.cpp:279-280).  No CPU spot checks are run, so
+// "Spot checks" is always 0.
+//
+//   collatz_run --range LO HI [--k 32] [--chunk 24] [--cutoff 187] [--form plain|persistent]
+// verifies [LO, HI) in range mode and prints one line per chunk and a total line.
+#include <chrono>
+#include <cinttypes>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/collatz.h"
+
+using u128 = unsigned __int128;
+
+static u128 parse_u128(const char* s) {
+  u128 v = 0;
+  for (; *s; ++s) {
+    if (*s < '0' || *s > '9') {
+      std::fprintf(stderr, "not a decimal number: %s\n", s);
+      std::exit(2);
+    }
+    v = v * 10 + (unsigned)(*s - '0');
+  }
+  return v;
+}
+
+static std::string to_dec(u128 v) {
+  if (!v) return "0";
+  std::string s;
+  while (v) {
+    s.insert(s.begin(), (char)('0' + (int)(v % 10)));
+    v /= 10;
+  }
+  return s;
+}
+
+static std::string hex_start(uint64_t lo, uint64_t hi) {
+  // "%X%08X%08X" of the start's 32-bit limbs 2, 1, 0 (collatz_cudav7.cu:298); for starts
+  // below 2^96 the same text as the CPU form "%lX%016lX" (:691).
+  char buf[64];
+  if (hi >> 32) std::snprintf(buf, sizeof(buf), "%X%08X%08X%08X", (unsigned)(hi >> 32), (unsigned)hi,
+                              (unsigned)(lo >> 32), (unsigned)lo);
+  else std::snprintf(buf, sizeof(buf), "%X%08X%08X", (unsigned)hi, (unsigned)(lo >> 32), (unsigned)lo);
+  return buf;
+}
+
+static int usage() {
+  std::fprintf(stderr,
+               "usage: collatz_run [--N n] [--A a] [--B b] [--cutoff c] [--semantics cuda|opencl|rust]"
+               " < case line\n"
+               "       collatz_run --range LO HI [--k k] [--chunk c] [--cutoff c] [--limbs 2|3]"
+               " [--form plain|persistent]\n");
+  return 2;
+}
+
+int main(int argc, char** argv) {
+  uint32_t N = 45, A = 10, B = 24, cutoff = 0, k = 32, chunk = 24, limbs = 2, form = 0;
+  uint32_t sem = COLLATZ_SEM_CUDA;
+  bool range = false;
+  u128 lo = 0, hi = 0;
+  for (int i = 1; i < argc; ++i) {
+    const char* a = argv[i];
+    auto next = [&]() -> const char* {
+      if (i + 1 >= argc) std::exit(usage());
+      return argv[++i];
+    };
+    if (!std::strcmp(a, "--N")) N = std::atoi(next());
+    else if (!std::strcmp(a, "--A")) A = std::atoi(next());
+    else if (!std::strcmp(a, "--B")) B = std::atoi(next());
+    else if (!std::strcmp(a, "--cutoff")) cutoff = std::atoi(next());
+    else if (!std::strcmp(a, "--k")) k = std::atoi(next());
+    else if (!std::strcmp(a, "--chunk")) chunk = std::atoi(next());
+    else if (!std::strcmp(a, "--limbs")) limbs = std::atoi(next());
+    else if (!std::strcmp(a, "--form")) form = !std::strcmp(next(), "persistent") ? 1 : 0;
+    else if (!std::strcmp(a, "--semantics")) {
+      const char* v = next();
+      sem = !std::strcmp(v, "rust") ? COLLATZ_SEM_RUST
+            : !std::strcmp(v, "opencl") ? COLLATZ_SEM_OPENCL : COLLATZ_SEM_CUDA;
+    } else if (!std::strcmp(a, "--range")) {
+      range = true;
+      lo = parse_u128(next());
+      hi = parse_u128(next());
+    } else {
+      return usage();
+    }
+  }
+  if (range) {
+    collatz_sieve_params sp = {k, 1, -1, 0};
+    collatz_sieve* sv = nullptr;
+    if (collatz_sieve_build(&sp, &sv) != COLLATZ_OK) {
+      std::fprintf(stderr, "sieve: %s\n", collatz_last_error());
+      return 1;
+    }
+    const uint64_t nch = (uint64_t)(((hi - 1) >> chunk) - (lo >> chunk) + 1);
+    std::vector<collatz_chunk> chunks(nch);
+    std::vector<uint64_t> ovf(2 * 4096);
+    collatz_range_result res;
+    std::memset(&res, 0, sizeof(res));
+    res.chunks = chunks.data();
+    res.chunks_cap = nch;
+    res.overflow_starts = ovf.data();
+    res.overflow_cap = 4096;
+    collatz_verify_opts o = {chunk, limbs, cutoff ? cutoff : 187u, form};
+    const uint64_t l2[2] = {(uint64_t)lo, (uint64_t)(lo >> 64)};
+    const uint64_t h2[2] = {(uint64_t)hi, (uint64_t)(hi >> 64)};
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = collatz_verify_range(sv, l2, h2, &o, nullptr, &res);
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rc != COLLATZ_OK && rc != COLLATZ_ECAPACITY) {
+      std::fprintf(stderr, "verify: %s\n", collatz_last_error());
+      return 1;
+    }
+    for (uint64_t i = 0; i < nch; ++i) {
+      const collatz_chunk& c = chunks[i];
+      const u128 ms = ((u128)c.max_start[1] << 64) | c.max_start[0];
+      std::printf("chunk %" PRIu64 " n %" PRIu64 " count %" PRIu64 " stop_sum %" PRIu64
+                  " stop_xor %" PRIu64 " hash %08X max_start %s flags %u\n",
+                  c.chunk_index, c.n_checked, c.count, c.stop_sum, c.stop_xor, c.hash,
+                  to_dec(ms).c_str(), c.flags);
+    }
+    for (uint64_t i = 0; i < std::min<uint64_t>(res.overflow_written, 4096); ++i)
+      std::printf("overflow %s\n", hex_start(ovf[2 * i], ovf[2 * i + 1]).c_str());
+    std::fprintf(stderr,
+                 "Verified %s integers, %" PRIu64 " iterated, hash %08X, %.3f s (%.3e n/s).\n",
+                 to_dec(hi - lo).c_str(), res.total.count, res.total.hash, s,
+                 (double)(hi - lo) / s);
+    collatz_sieve_free(sv);
+    return 0;
+  }
+  std::string line;
+  std::getline(std::cin, line);
+  std::istringstream iss(line);
+  uint64_t t[6] = {0};
+  for (int i = 0; i < 6; ++i) iss >> t[i];
+  collatz_ref_params p = {N, A, B, sem == COLLATZ_SEM_RUST ? 8u : 16u, sem,
+                          cutoff ? cutoff : 100u, -1, 0};
+  collatz_node node;
+  std::memset(&node, 0, sizeof(node));
+  node.base[0] = t[0];
+  node.sigfig = (uint32_t)t[1];
+  node.threepower = (uint32_t)t[2];
+  node.number[0] = t[3];
+  node.doubleeven = (uint32_t)t[4];
+  node.numevens = (uint32_t)t[5];
+  std::vector<uint64_t> ovf(2 * (1 << 16));
+  collatz_ref_result r;
+  std::memset(&r, 0, sizeof(r));
+  r.overflow_starts = ovf.data();
+  r.overflow_cap = 1 << 16;
+  const int rc = collatz_reference_run(&p, &node, nullptr, &r);
+  if (rc != COLLATZ_OK && rc != COLLATZ_ECAPACITY) {
+    std::fprintf(stderr, "collatz_run: %s\n", collatz_last_error());
+    return 1;
+  }
+  for (uint64_t i = 0; i < std::min<uint64_t>(r.overflow_written, 1 << 16); ++i)
+    std::printf("%s\n", hex_start(ovf[2 * i], ovf[2 * i + 1]).c_str());
+  std::fflush(stdout);
+  if (sem == COLLATZ_SEM_OPENCL) {
+    std::fprintf(stderr, "Number of kernel launches = %" PRIu64 " \n", r.launches);
+    std::fprintf(stderr, "Spot checks = %u. Complete hash = %X.\n", 0u, r.complete_hash);
+  } else {
+    std::fprintf(stderr, "Number of kernel launches = %" PRIu64 ". ", r.launches);
+    std::fprintf(stderr, "Spot checks = %u. Complete hash = %X.\n", 0u, r.complete_hash);
+  }
+  return 0;
+}

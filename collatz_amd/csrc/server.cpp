@@ -1,0 +1,301 @@
+// C ABI: the served form -- a request loop with continuous batching.
+//
+// Requests are submitted at any time.  A worker thread owns an Engine (its own device
+// buffers, stream and captured graph) and runs back-to-back persistent launches; each
+// launch takes the remaining rows of every admitted request, oldest first, up to a work
+// budget.  A request that arrives while a launch runs is admitted at the next launch as its
+// footprint fits the budget, and its rows share that launch with the rows of earlier
+// requests.  Per-launch records are folded into each request's chunk records on the host, so
+// a request's result does not depend on what it was batched with.  The sieve stays resident.
+// The reference has no counterpart: it is a single synchronous process per case line.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "engine.hpp"
+#include "host_common.hpp"
+#include "tuning.hpp"
+
+namespace cz {
+
+struct Request {
+  int64_t ticket = 0;
+  u128 lo = 0, hi = 0;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  RowPlan plan;
+  std::vector<uint64_t> row_est;
+  size_t next_row = 0;
+  std::vector<DevRec> recs;
+  uint64_t escalated = 0;
+  uint64_t footprint = 0;
+  bool admitted = false, done = false;
+  int status = COLLATZ_OK;
+  std::string error;
+};
+
+}  // namespace cz
+
+struct collatz_server {
+  const collatz_sieve* sv = nullptr;
+  collatz_server_opts opts{};
+  cz::Engine* engine = nullptr;
+  hipStream_t stream = nullptr;
+  std::thread worker;
+  std::mutex mu;
+  std::condition_variable cv, done_cv;
+  std::deque<cz::Request*> pending;
+  std::vector<cz::Request*> active;
+  std::map<int64_t, cz::Request*> all;
+  int64_t next_ticket = 0;
+  bool stop = false;
+  double t0 = 0;
+  uint64_t bytes_in_use = 0;
+  cz::Tuning tn{16384, 28, 2};
+};
+
+namespace cz {
+
+static void server_loop(collatz_server* S) {
+  (void)hipSetDevice(S->sv->device);
+  const uint64_t budget = S->opts.slice_candidates;
+  for (;;) {
+    LaunchPlan lp;
+    struct SlotRef {
+      Request* r;
+      uint64_t chunk;
+    };
+    std::vector<SlotRef> slots;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->active.empty(); });
+      if (S->stop && S->pending.empty() && S->active.empty()) return;
+      // Admission: oldest first, while the footprint budget and the in-flight cap allow.
+      while (!S->pending.empty() && S->active.size() < S->opts.max_inflight) {
+        Request* r = S->pending.front();
+        if (!S->active.empty() && S->bytes_in_use + r->footprint > S->opts.max_device_bytes) break;
+        S->pending.pop_front();
+        r->admitted = true;
+        r->t_admit = now_ms() - S->t0;
+        S->bytes_in_use += r->footprint;
+        S->active.push_back(r);
+      }
+      // Continuous batching: every admitted request gets an equal share of the launch's work
+      // budget, and at least one row, so a short request finishes within one launch however
+      // long the requests ahead of it are; what the shares leave goes to the oldest.
+      uint64_t taken = 0;
+      auto take = [&](Request* r, uint64_t want, bool one_row_always) {
+        uint64_t mine = 0, last_chunk = UINT64_MAX;
+        while (r->next_row < r->plan.rows.size() &&
+               (mine == 0 && one_row_always ? true : (mine < want && taken < budget))) {
+          const size_t i = r->next_row;
+          RowDesc row = r->plan.rows[i];
+          // The row's slot is its chunk within the request; rows of one chunk are contiguous.
+          const uint64_t chunk = row.slot;
+          if (chunk != last_chunk) {
+            slots.push_back({r, chunk});
+            lp.part.push_back(lp.ntiles);
+            last_chunk = chunk;
+          }
+          row.slot = (uint32_t)(slots.size() - 1);
+          const uint32_t c = row.cls;
+          if (r->plan.key_lo[i] < 0) {
+            row.i_lo = S->sv->class_start[c];
+          } else {
+            row.i_lo = kResolveFlag | lp.keys.size();
+            lp.keys.push_back(r->plan.keys[r->plan.key_lo[i]]);
+          }
+          if (r->plan.key_hi[i] < 0) {
+            row.i_hi = S->sv->class_start[c + 1];
+          } else {
+            row.i_hi = kResolveFlag | lp.keys.size();
+            lp.keys.push_back(r->plan.keys[r->plan.key_hi[i]]);
+          }
+          uint64_t pieces = (uint64_t)((double)r->row_est[i] / S->tn.tile + 0.5);
+          pieces = std::min<uint64_t>(std::max<uint64_t>(pieces, 1), 1u << 20);
+          row.block_start = lp.ntiles;
+          lp.ntiles += pieces;
+          lp.rows.push_back(row);
+          mine += r->row_est[i] + 1;
+          taken += r->row_est[i] + 1;
+          r->next_row++;
+        }
+      };
+      const uint64_t share = std::max<uint64_t>(budget / std::max<size_t>(S->active.size(), 1), 1);
+      for (Request* r : S->active) take(r, share, true);
+      for (Request* r : S->active) {
+        if (taken >= budget) break;
+        take(r, budget - taken, false);
+      }
+    }
+    lp.nslots = slots.size();
+    lp.part.push_back(lp.ntiles);
+    LaunchResult res;
+    int rc = COLLATZ_OK;
+    if (!lp.rows.empty()) {
+      rc = S->engine->run(lp, S->tn, 2, S->opts.cutoff_log2, true, 0, S->stream, res);
+      // More escalations than the list holds: run the same launch on three limbs.
+      if (rc == COLLATZ_ECAPACITY)
+        rc = S->engine->run(lp, S->tn, 3, S->opts.cutoff_log2, true, 0, S->stream, res);
+    }
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      if (rc != COLLATZ_OK) {
+        // Fail every request that had rows in this launch.
+        for (const SlotRef& sr : slots) {
+          sr.r->status = rc;
+          sr.r->error = collatz_last_error();
+          sr.r->next_row = sr.r->plan.rows.size();
+        }
+      } else {
+        for (size_t i = 0; i < slots.size(); ++i) rec_merge_host(slots[i].r->recs[slots[i].chunk], res.recs[i]);
+        for (uint32_t sl : res.esc_slots)
+          if (sl < slots.size()) slots[sl].r->escalated++;
+      }
+      for (auto it = S->active.begin(); it != S->active.end();) {
+        Request* r = *it;
+        if (r->next_row >= r->plan.rows.size()) {
+          r->done = true;
+          r->t_done = now_ms() - S->t0;
+          S->bytes_in_use -= r->footprint;
+          it = S->active.erase(it);
+        } else {
+          ++it;
+        }
+      }
+    }
+    S->done_cv.notify_all();
+  }
+}
+
+}  // namespace cz
+
+using namespace cz;
+
+extern "C" {
+
+int collatz_server_create(const collatz_sieve* sv, const collatz_server_opts* opts,
+                          collatz_server** out) {
+  if (!sv || !out) return fail(COLLATZ_EINVAL, "null argument");
+  *out = nullptr;
+  collatz_server* S = new (std::nothrow) collatz_server();
+  if (!S) return fail(COLLATZ_ENOMEM, "out of host memory");
+  S->sv = sv;
+  if (opts) S->opts = *opts;
+  if (!S->opts.chunk_log2) S->opts.chunk_log2 = 24;
+  if (!S->opts.cutoff_log2) S->opts.cutoff_log2 = 187;
+  if (!S->opts.max_device_bytes) S->opts.max_device_bytes = 1ull << 30;
+  if (!S->opts.slice_candidates) S->opts.slice_candidates = 1ull << 26;
+  if (!S->opts.max_inflight) S->opts.max_inflight = 1024;
+  if (S->opts.chunk_log2 > 63 || S->opts.cutoff_log2 < 64 || S->opts.cutoff_log2 > 187) {
+    delete S;
+    return fail(COLLATZ_EINVAL, "bad server options");
+  }
+  HipDeviceGuard guard(sv->device);
+  if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete S;
+    return fail(COLLATZ_EDEVICE, "stream creation failed");
+  }
+  S->engine = new (std::nothrow) Engine(sv);
+  if (!S->engine) {
+    (void)hipStreamDestroy(S->stream);
+    delete S;
+    return fail(COLLATZ_ENOMEM, "out of host memory");
+  }
+  S->t0 = now_ms();
+  S->worker = std::thread(server_loop, S);
+  *out = S;
+  return COLLATZ_OK;
+}
+
+int64_t collatz_server_submit(collatz_server* S, const collatz_interval* range) {
+  if (!S || !range) return fail(COLLATZ_EINVAL, "null argument");
+  Request* r = new (std::nothrow) Request();
+  if (!r) return fail(COLLATZ_ENOMEM, "out of host memory");
+  r->t_submit = now_ms() - S->t0;
+  r->lo = mk128(range->lo[0], range->lo[1]);
+  r->hi = mk128(range->hi[0], range->hi[1]);
+  if (r->lo >= r->hi) {
+    delete r;
+    return fail(COLLATZ_EINVAL, "empty interval (lo >= hi)");
+  }
+  int rc = plan_rows(S->sv->k, r->lo, r->hi, S->opts.chunk_log2, 0, S->sv->mod9 != 0, r->plan,
+                     1ull << 26);
+  if (rc) {
+    delete r;
+    return rc;
+  }
+  // Expected survivors per row, from the row's residue window and its class size.
+  const double kspan = (double)((u128)1 << S->sv->k);
+  r->row_est.resize(r->plan.rows.size());
+  for (size_t i = 0; i < r->plan.rows.size(); ++i) {
+    const uint32_t c = r->plan.rows[i].cls;
+    const double a = r->plan.key_lo[i] < 0 ? 0.0 : (double)r->plan.keys[r->plan.key_lo[i]];
+    const double b = r->plan.key_hi[i] < 0 ? kspan : (double)r->plan.keys[r->plan.key_hi[i]];
+    r->row_est[i] =
+        (uint64_t)((double)(S->sv->class_start[c + 1] - S->sv->class_start[c]) * (b - a) / kspan);
+  }
+  r->recs.assign(r->plan.nchunks, DevRec{});
+  for (DevRec& d : r->recs) std::memset(&d, 0, sizeof(d));
+  r->footprint = r->plan.rows.size() * (sizeof(RowDesc) + 16) + r->plan.nchunks * sizeof(DevRec);
+  std::lock_guard<std::mutex> lk(S->mu);
+  r->ticket = S->next_ticket++;
+  S->all[r->ticket] = r;
+  S->pending.push_back(r);
+  S->cv.notify_one();
+  return r->ticket;
+}
+
+int collatz_server_wait(collatz_server* S, int64_t ticket, double timeout_ms,
+                        collatz_request_result* out, collatz_chunk* chunks, uint64_t cap) {
+  if (!S || !out) return fail(COLLATZ_EINVAL, "null argument");
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto it = S->all.find(ticket);
+  if (it == S->all.end()) return fail(COLLATZ_EINVAL, "unknown ticket");
+  Request* r = it->second;
+  auto ready = [&] { return r->done; };
+  if (timeout_ms < 0) S->done_cv.wait(lk, ready);
+  else if (!S->done_cv.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready))
+    return fail(COLLATZ_ENOTREADY, "request not finished");
+  std::memset(out, 0, sizeof(*out));
+  out->chunks = r->plan.nchunks;
+  out->submit_ms = r->t_submit;
+  out->admit_ms = r->t_admit;
+  out->done_ms = r->t_done;
+  for (uint64_t c = 0; c < r->plan.nchunks; ++c) {
+    collatz_chunk ch;
+    chunk_from_rec(r->recs[c], r->plan.first_chunk + c, r->plan.n_checked[c], &ch);
+    if (chunks && c < cap) chunks[c] = ch;
+    chunk_merge(&out->total, ch);
+  }
+  out->total.chunk_index = r->plan.first_chunk;
+  const int status = r->status;
+  const std::string err = r->error;
+  S->all.erase(it);
+  delete r;
+  if (status != COLLATZ_OK) return fail(status, err);
+  if (chunks && out->chunks > cap) return fail(COLLATZ_ECAPACITY, "chunk buffer too small");
+  return COLLATZ_OK;
+}
+
+int collatz_server_destroy(collatz_server* S) {
+  if (!S) return COLLATZ_OK;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv.notify_all();
+  if (S->worker.joinable()) S->worker.join();
+  HipDeviceGuard guard(S->sv->device);
+  delete S->engine;
+  if (S->stream) (void)hipStreamDestroy(S->stream);
+  for (auto& kv : S->all) delete kv.second;
+  delete S;
+  return COLLATZ_OK;
+}
+
+}  // extern "C"

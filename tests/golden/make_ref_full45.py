@@ -37,7 +37,7 @@ def work(lines):
 def main():
     lines = open("/root/reference/cases_big").read().split("\n")
     lines = [l for l in lines if l.strip()]
-    nproc = min(8, os.cpu_count() or 1)
+    nproc = min(int(os.environ.get("NPROC", "8")), os.cpu_count() or 1)
     chunks = [lines[i::64] for i in range(64)]
     with mp.Pool(nproc) as pool:
         res = pool.map(work, chunks)

@@ -1,0 +1,72 @@
+"""Served form: requests submitted while others run; every result equals the same range
+verified alone (collatz_verify_range) and the oracle."""
+import random
+import threading
+import time
+
+import pytest
+
+from gpu_util import assert_chunks_equal, assert_total_equal
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def s20():
+    import collatz_amd as cz
+    return cz.Sieve(20)
+
+
+def requests(seed, n):
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n):
+        lo = rng.randrange(1, 1 << rng.choice([40, 62, 64]))
+        out.append((lo, lo + rng.randrange(1 << 10, 1 << rng.choice([16, 22, 26]))))
+    return out
+
+
+def test_results_alone_equal_in_the_mix(s20, oracle):
+    import collatz_amd as cz
+    reqs = requests(3, 60)
+    srv = cz.Server(s20, chunk_log2=22, slice_candidates=1 << 16)
+    tickets = []
+    # Submit from another thread in bursts, so requests arrive while launches run.
+    def producer():
+        for i, (lo, hi) in enumerate(reqs):
+            tickets.append(srv.submit(lo, hi))
+            if i % 7 == 0:
+                time.sleep(0.002)
+    th = threading.Thread(target=producer)
+    th.start()
+    th.join()
+    o20 = oracle.sieve(20)
+    for t, (lo, hi) in zip(tickets, reqs):
+        total, chunks, times = srv.wait(t, 60000)
+        alone = s20.verify_range(lo, hi, 22)
+        assert_chunks_equal(chunks, alone.chunks, f"request [{lo:#x},{hi:#x})")
+        ochunks, ototal = oracle.verify_range(o20, lo, hi, 22)
+        assert_chunks_equal(chunks, ochunks)
+        assert_total_equal(total, ototal)
+        assert times[0] <= times[1] <= times[2]
+    srv.close()
+
+
+def test_server_overlaps_requests(s20):
+    """Short requests submitted behind a long one share its launches and finish first."""
+    import collatz_amd as cz
+    srv = cz.Server(s20, chunk_log2=20, slice_candidates=1 << 16)
+    big = srv.submit(1 << 40, (1 << 40) + (1 << 28))
+    small = [srv.submit((1 << 50) + i * (1 << 20), (1 << 50) + (i + 1) * (1 << 20)) for i in range(5)]
+    tb = srv.wait(big, 60000)[2]
+    ts = [srv.wait(t, 60000)[2] for t in small]
+    assert all(s[2] < tb[2] for s in ts)
+    srv.close()
+
+
+def test_server_bad_request(s20):
+    import collatz_amd as cz
+    srv = cz.Server(s20)
+    with pytest.raises(cz.CollatzError):
+        srv.submit(10, 10)
+    srv.close()
