@@ -1,11 +1,25 @@
-"""Benchmark of the MI355X Collatz range verifier.
+"""Benchmark of the MI355X Collatz range verifier: batched and served forms against the
+plain kernel over the same ranges, bit-exact.
 
-Workload (one step): verify every n in [2^40, 2^40 + 2^36) with the k=32 residue sieve
-(25,005,944 survivors, resident in HBM), the mod 9 sieve, two u64 limbs (three on demand)
-and a record per 2^24-wide chunk: one call of collatz_verify_range.  Inputs (the sieve) are
-built once and stay resident before the timed region.
+Headline workload (one step): the 4096 intervals of the batch workload (SplitMix64 seed 0;
+starts uniform in [2^48, 2^60), widths log-uniform in [2^20, 2^34] rounded up to a multiple
+of 2^32; 2.17e13 integers in all), verified with the k=32 residue sieve (25,005,944
+survivors, resident in HBM), the mod 9 sieve, two u64 limbs (three on demand) and one
+record per 2^32-wide chunk, in ONE collatz_verify_batch call (persistent launch replayed
+from a hipGraph, device work stealing, autotuned launch shape).  `value` is that form's
+integers per second.
 
-Prints one JSON line (see DESIGN.md for the fields).
+Beside it, on the same GPU and sieve:
+  * plain: the same 4096 intervals, one collatz_verify_range call (plain kernel) each; the
+    per-interval records of both forms are compared and must be identical;
+  * range: [2^40, 2^40 + 2^36) with 2^24-wide chunk records, plain and persistent;
+  * served: the request stream (SplitMix64 seed 1; widths log-uniform in [2^16, 2^36];
+    80% of starts in [2^40, 2^62), 20% in [2^62, 2^64)) replayed at 50..500 requests/s
+    through collatz_server_*, with p50/p99 latency, and once all at once (saturated);
+  * cpu_baseline: the CPU restatement (oracle) on a bounded sample of the same workload.
+
+Every call returns after its stream has synchronised, so the host clock around a call
+brackets the device work.  Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -13,111 +27,225 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = ("verified n/sec on 1 GPU (batched/served vs plain, bit-exact); p50/p99 latency")
-LO = 1 << 40
-WIDTH = 1 << 36
+METRIC = "verified n/sec on 1 GPU (batched/served vs plain, bit-exact); p50/p99 latency"
 K = 32
-CHUNK = 24
-# Algorithmic integer operations (32-bit lane ops) per check and per candidate for L
-# 32-bit words: OPS_DR = 13 L + 7, OPS_SETUP = 4 L + 8 (L = 4 for two u64 limbs).
+BATCH_CHUNK = 32
+RANGE_LO, RANGE_WIDTH, RANGE_CHUNK = 1 << 40, 1 << 36, 24
+# Algorithmic integer work, in 32-bit lane operations, for L 32-bit words per value
+# (L = 4 for two u64 limbs): per check (two odd and two even runs, hash and compare)
+# OPS_DR = 13 L + 7, and per candidate (mod 9, start value, base) OPS_SETUP = 4 L + 8.
 L_WORDS = 4
 OPS_DR = 13 * L_WORDS + 7
 OPS_SETUP = 4 * L_WORDS + 8
-# gfx950 int32 VALU issue peak: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz.
+# gfx950 int32 VALU issue peak: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T lane-ops/s.
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+RATES = (50, 100, 200, 500)
 
 
-def cpu_baseline(threads: int, sample_chunks: int):
-    """The oracle (CPU restatement) on a bounded sample of the same workload."""
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def records_equal(a, b) -> bool:
+    return a.shape == b.shape and a.tobytes() == b.tobytes()
+
+
+def cpu_baseline(intervals, threads: int, n_intervals: int, width_log2: int):
+    """The oracle (CPU restatement) on the first `n_intervals` intervals, each cut to its
+    first 2^width_log2 integers."""
     from oracle import bind as B
     t0 = time.time()
     sv = B.sieve(K)
     build_s = time.time() - t0
-    lo = LO + (WIDTH // 2)
-    hi = lo + sample_chunks * (1 << CHUNK)
+    n = 0
     t0 = time.time()
-    _, tot = B.verify_range(sv, lo, hi, CHUNK, threads=threads)
+    for lo, hi in intervals[:n_intervals]:
+        hi = min(hi, lo + (1 << width_log2))
+        B.verify_range(sv, lo, hi, BATCH_CHUNK, threads=threads)
+        n += hi - lo
     dt = time.time() - t0
-    return {"value": (hi - lo) / dt, "unit": "n/s", "cores": threads, "kind": "port",
-            "sample": f"{sample_chunks} chunks of 2^{CHUNK} from the middle of the range "
-                      f"({int(tot['count'])} candidates, {dt:.1f} s on {threads} threads; "
-                      f"sieve build {build_s:.1f} s excluded as on the GPU)"}, tot
+    return {"value": n / dt, "unit": "n/s", "cores": threads, "kind": "port",
+            "sample": f"first {n_intervals} intervals of the batch workload, each cut to "
+                      f"2^{width_log2} integers ({n:.3e} n in {dt:.1f} s on {threads} threads;"
+                      f" oracle sieve build {build_s:.1f} s excluded, as on the GPU)"}
+
+
+def served_leg(sv, reqs, rate, chunk_log2):
+    """Replays `reqs` (gap, lo, hi) at `rate` requests/s (rate None: all at once)."""
+    import collatz_amd as cz
+    srv = cz.Server(sv, chunk_log2=chunk_log2)
+    tickets = []
+    t0 = time.perf_counter()
+
+    def producer():
+        t = 0.0
+        for gap, lo, hi in reqs:
+            if rate:
+                t += gap / rate
+                d = t0 + t - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+            tickets.append((srv.submit(lo, hi), hi - lo))
+
+    th = threading.Thread(target=producer)
+    th.start()
+    lat, n, first_submit, last_done = [], 0, None, 0.0
+    i = 0
+    while True:
+        if i < len(tickets):
+            t, w = tickets[i]
+            _, _, (ts, ta, td) = srv.wait(t)
+            lat.append(td - ts)
+            n += w
+            first_submit = ts if first_submit is None else min(first_submit, ts)
+            last_done = max(last_done, td)
+            i += 1
+        elif not th.is_alive() and i >= len(tickets):
+            break
+        else:
+            time.sleep(0.0005)
+    th.join()
+    srv.close()
+    lat.sort()
+    pct = lambda p: lat[min(len(lat) - 1, int(p * len(lat)))]
+    span_s = (last_done - first_submit) / 1e3
+    return {"requests": len(lat), "p50_ms": pct(0.50), "p99_ms": pct(0.99),
+            "max_ms": lat[-1], "n_per_s": n / span_s, "offered_req_per_s": rate}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--form", choices=["plain", "persistent", "graph"], default="plain")
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--plain-passes", type=int, default=1)
+    ap.add_argument("--range-steps", type=int, default=20)
+    ap.add_argument("--serve-seconds", type=float, default=5.0,
+                    help="length of each served replay at a fixed rate")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-range", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-chunks", type=int, default=48)
+    ap.add_argument("--cpu-intervals", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(the multi-GPU sharding is described in DESIGN.md)"}))
+        print(json.dumps({"error": "this verifier runs on one GPU; --gpus must be 1. Ranges "
+                          "shard across GPUs with no exchange during compute (see DESIGN.md)."}))
         sys.exit(2)
     import collatz_amd as cz
-    form = cz.FORM_PLAIN if args.form == "plain" else cz.FORM_PERSISTENT
+    from collatz_amd import workloads as W
+
+    ivs = W.batch_intervals(0, 4096, K)
+    total_n = sum(hi - lo for lo, hi in ivs)
+    t = time.time()
     sv = cz.Sieve(K)
     info = sv.info()
+    log(f"sieve k={K}: {int(info.survivors)} survivors in {time.time() - t:.1f} s")
 
-    def step():
-        if args.form == "graph":
-            return sv.verify_batch([(LO, LO + WIDTH)], chunk_log2=CHUNK, use_graph=True)[0]
-        return sv.verify_range(LO, LO + WIDTH, CHUNK, form=form)
+    # Headline: the batched form.
+    def batch_step():
+        return sv.verify_batch(ivs, chunk_log2=BATCH_CHUNK, use_graph=True)
 
     for _ in range(args.warmup):
-        step()
+        batch_step()
+    log(f"batched warmup done ({args.warmup} steps)")
     dev_ms = []
     t0 = time.perf_counter()
-    out = None
+    outs = None
     for _ in range(args.steps):
-        out = step()
-        dev_ms.append(out.device_ms)
+        outs = batch_step()
+        dev_ms.append(outs[0].device_ms)
     dt = time.perf_counter() - t0
+    log(f"batched: {args.steps} steps in {dt:.2f} s")
     ms_step = dt / args.steps * 1e3
-    tot = out.total
-    n_per_s = WIDTH / (dt / args.steps)
+    count = sum(int(o.total["count"]) for o in outs)
+    checks = sum(int(o.checks) for o in outs)
     kern_ms = sum(dev_ms) / len(dev_ms)
-    ops = out.checks * OPS_DR + int(tot["count"]) * OPS_SETUP
+    ops = checks * OPS_DR + count * OPS_SETUP
     achieved = ops / (kern_ms * 1e-3) / 1e12
+    value = total_n / (dt / args.steps)
+    h = 0
+    for o in outs:
+        h ^= int(o.total["hash"])
+
     line = {
-        "metric": METRIC,
-        "value": n_per_s,
-        "unit": "n/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32",
-        "data": "synthetic",
-        "config": {"workload": "verify [2^40, 2^40+2^36) with a k=32 sieve, mod 9 on, "
-                               "2x u64 limbs, 2^24-wide chunk records",
-                   "form": args.form, "k": K, "chunk_log2": CHUNK, "width": WIDTH,
-                   "survivors": int(info.survivors)},
-        "survivors_per_s": int(tot["count"]) / (dt / args.steps),
-        "candidates": int(tot["count"]),
-        "checks": int(out.checks),
-        "kernel_ms": kern_ms,
-        "total_hash": "%08X" % int(tot["hash"]),
+        "metric": METRIC, "value": value, "unit": "n/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": "batched: 4096 intervals (SplitMix64 seed 0), starts U[2^48,2^60),"
+                               " widths log-U[2^20,2^34] rounded up to 2^32; k=32 sieve, mod 9 on,"
+                               " 2x u64 limbs, 2^32-wide chunk records, one graph-replayed "
+                               "persistent launch per step",
+                   "form": "batched", "k": K, "chunk_log2": BATCH_CHUNK, "intervals": len(ivs),
+                   "integers_per_step": total_n, "survivors": int(info.survivors)},
+        "survivors_per_s": count / (dt / args.steps),
+        "candidates": count, "checks": checks, "kernel_ms": kern_ms,
+        "total_hash": "%08X" % h,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS,
                      "unit": "TOP/s", "frac": achieved / VALU_PEAK_TOPS, "traffic": None,
-                     "kernel": "k_verify_plain" if form == cz.FORM_PLAIN else "k_verify_persistent"},
+                     "kernel": "k_verify_persistent<4>",
+                     "ops": "checks x (13L+7) + candidates x (4L+8), L=4"},
         "cpu_baseline": None,
     }
+
+    if not args.no_plain:
+        plain_s = 0.0
+        same = True
+        for p in range(args.plain_passes):
+            t0 = time.perf_counter()
+            pouts = [sv.verify_range(lo, hi, BATCH_CHUNK, form=cz.FORM_PLAIN) for lo, hi in ivs]
+            plain_s += time.perf_counter() - t0
+            if p == 0:
+                for a, b in zip(outs, pouts):
+                    if not (records_equal(a.chunks, b.chunks) and
+                            a.total.tobytes() == b.total.tobytes()):
+                        same = False
+        plain_rate = total_n / (plain_s / args.plain_passes)
+        line["plain"] = {"n_per_s": plain_rate, "ms_per_pass": plain_s / args.plain_passes * 1e3,
+                         "calls_per_pass": len(ivs)}
+        line["batched_vs_plain"] = value / plain_rate
+        line["bit_exact_batched_vs_plain"] = same
+        log(f"plain: {plain_rate:.3e} n/s, batched/plain {value / plain_rate:.2f}, "
+            f"identical records {same}")
+
+    if not args.no_range:
+        rng = {}
+        for name, form in (("plain", cz.FORM_PLAIN), ("persistent", cz.FORM_PERSISTENT)):
+            sv.verify_range(RANGE_LO, RANGE_LO + RANGE_WIDTH, RANGE_CHUNK, form=form)
+            ms = []
+            t0 = time.perf_counter()
+            for _ in range(args.range_steps):
+                o = sv.verify_range(RANGE_LO, RANGE_LO + RANGE_WIDTH, RANGE_CHUNK, form=form)
+                ms.append(o.device_ms)
+            d = (time.perf_counter() - t0) / args.range_steps
+            rng[name] = {"n_per_s": RANGE_WIDTH / d, "ms_per_call": d * 1e3,
+                         "device_ms": sum(ms) / len(ms), "hash": "%08X" % int(o.total["hash"])}
+        line["range_2p40_w2p36"] = rng
+        log(f"range: plain {rng['plain']['n_per_s']:.3e}, persistent "
+            f"{rng['persistent']['n_per_s']:.3e} n/s")
+
+    if not args.no_serve:
+        reqs = W.request_stream(1, 20000)
+        served = {}
+        for rate in RATES:
+            n = min(len(reqs), max(50, int(rate * args.serve_seconds)))
+            served[str(rate)] = served_leg(sv, reqs[:n], rate, BATCH_CHUNK)
+            log(f"served at {rate} req/s: {served[str(rate)]}")
+        served["saturated"] = served_leg(sv, reqs[:2000], None, BATCH_CHUNK)
+        log(f"served saturated: {served['saturated']}")
+        line["served"] = served
+
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        line["cpu_baseline"], _ = cpu_baseline(threads, args.cpu_chunks)
-    print(json.dumps(line))
+        line["cpu_baseline"] = cpu_baseline(ivs, threads, args.cpu_intervals, 32)
+        log(f"cpu baseline: {line['cpu_baseline']['value']:.3e} n/s")
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -291,13 +291,13 @@ class Server:
         self._h = h
         self.sieve = sieve
         self.chunk_log2 = chunk_log2
+        self._meta = {}
 
     def submit(self, lo: int, hi: int) -> int:
         iv = Interval((C.c_uint64 * 2)(lo & MASK64, lo >> 64), (C.c_uint64 * 2)(hi & MASK64, hi >> 64))
         t = load().collatz_server_submit(self._h, C.byref(iv))
         if t < 0:
             _check(int(t))
-        self._meta = getattr(self, "_meta", {})
         self._meta[t] = (lo, hi)
         return t
 
