@@ -133,7 +133,19 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-intervals", type=int, default=32)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device tensor copy three times (the "
+                         "reference reading for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4, "ok": bool((b == 1).all())}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this verifier runs on one GPU; --gpus must be 1. Ranges "
                           "shard across GPUs with no exchange during compute (see DESIGN.md)."}))

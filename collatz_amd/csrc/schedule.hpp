@@ -5,10 +5,7 @@
 //   check = odd run, even run, odd run, even run; then hash ^= low word; stop if number < n
 //   odd run : a = ctz(number | 2^8), number = (number >> a) * 3^a
 //   even run: b = ctz((number - 1) | 2^31), number = ((number - 1) >> b) + 1
-// where number = T-value + 1.  The even run is computed here as
-//   number = (number + 2^b - 1) >> b,
-// which is the same integer because number - 1 is a multiple of 2^b, and which needs one
-// carry chain instead of a borrow and a conditional carry.
+// where number = T-value + 1.
 #pragma once
 
 #include "internal.hpp"
@@ -60,16 +57,24 @@ __device__ __forceinline__ uint32_t odd_run(Num<L>& v, const Pow3Lds& p3) {
 
 template <int L>
 __device__ __forceinline__ uint32_t even_run(Num<L>& v) {
-  const uint32_t b = __builtin_ctz((v.w[0] - 1u) | 0x80000000u);
-  const uint32_t add = (1u << b) - 1u;
-  uint32_t x[L];
-  uint32_t c = 0;
-  x[0] = addc(v.w[0], add, 0u, c);
+  // number - 1 is taken in the low word alone, as the reference does: after an odd run
+  // that stopped below a = 8 the number is odd, so nothing borrows; after one that stopped
+  // at the cap it may be even, and then b = 0 and the result is the number itself (a low
+  // word of 0 wraps to 2^32 - 1, is shifted by 0 and wraps back, with no carry).
+  const uint32_t w0 = v.w[0] - 1u;
+  const uint32_t b = __builtin_ctz(w0 | 0x80000000u);
+  const uint32_t x0 = alignr(v.w[1], w0, b) + 1u;
 #pragma unroll
-  for (int i = 1; i < L; ++i) x[i] = addc(v.w[i], 0u, c, c);
+  for (int i = 1; i < L - 1; ++i) v.w[i] = alignr(v.w[i + 1], v.w[i], b);
+  v.w[L - 1] = v.w[L - 1] >> b;
+  v.w[0] = x0;
+  // The + 1 carries out of the low word only if the shifted low word was 2^32 - 1: one
+  // case in 2^31, taken as a branch rather than a carry chain on every run.
+  if (__builtin_expect(x0 == 0u && b != 0u, 0)) {
+    uint32_t c = 1;
 #pragma unroll
-  for (int i = 0; i < L - 1; ++i) v.w[i] = alignr(x[i + 1], x[i], b);
-  v.w[L - 1] = x[L - 1] >> b;
+    for (int i = 1; i < L; ++i) v.w[i] = addc(v.w[i], 0u, c, c);
+  }
   return b;
 }
 

@@ -140,24 +140,27 @@ __device__ __forceinline__ void push_escalate(const DeviceLists& lists, const ui
 }
 
 // After an odd run: offer to the max, then the cutoff / limb limit.  Returns true if the
-// candidate has to be escalated (two limbs only).
+// candidate has to be escalated (two limbs only).  On two limbs LOWCUT means a cutoff below
+// the two-limb bound 2^123 (the top-pair compare is then exact and means overflow);
+// without it the compare means only "escalate", and nothing else changes the state.  On
+// three limbs LOWCUT means a cutoff below 2^128.
 template <int L, bool LOWCUT, class Sink>
 __device__ __forceinline__ bool after_odd(Cand<L>& c, Sink& sink, const Limits& lim,
                                           const DeviceLists& lists) {
   sink.offer(c.v, c.n);
-  if (over_fast<L, LOWCUT>(c.v, lim.fast_thr)) {
-    if (ge_pow2<L>(c.v, lim.cutoff)) {
+  if constexpr (L == 4 && !LOWCUT) {
+    return pair(c.v.w[2], c.v.w[3]) >= lim.fast_thr;
+  } else {
+    if (over_fast<L, LOWCUT>(c.v, lim.fast_thr) && (L == 4 || ge_pow2<L>(c.v, lim.cutoff))) {
       // Overflow: report the start and continue from 2 (collatz_cudav7.cu:295-301).
       c.ovf += 1;
       push_overflow(lists, c.n, c.slot);
       c.v.w[0] = 2;
 #pragma unroll
       for (int i = 1; i < L; ++i) c.v.w[i] = 0;
-    } else if (L == 4) {
-      return true;
     }
+    return false;
   }
-  return false;
 }
 
 // One check.  Returns 0 while running, 1 when the candidate finished (number < n), 2 when
@@ -434,8 +437,11 @@ hipError_t launch_verify_plain(int limbs, const SieveArrays& sv, const RowDesc* 
   if (nblocks == 0) return hipSuccess;
   if (nblocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
   const dim3 grid((uint32_t)nblocks), block(256);
-  if (limbs == 2)
+  if (limbs == 2 && lim.cutoff >= kWideLog2)
     hipLaunchKernelGGL((k_verify_plain<4, false>), grid, block, 0, s, sv, rows, nrows, R, mod9,
+                       lim, pow3, block_recs, lists);
+  else if (limbs == 2)
+    hipLaunchKernelGGL((k_verify_plain<4, true>), grid, block, 0, s, sv, rows, nrows, R, mod9,
                        lim, pow3, block_recs, lists);
   else if (lim.cutoff >= 128)
     hipLaunchKernelGGL((k_verify_plain<6, false>), grid, block, 0, s, sv, rows, nrows, R, mod9,
@@ -452,8 +458,11 @@ hipError_t launch_verify_persistent(int limbs, const SieveArrays& sv, const RowD
                                     uint32_t U, const Limits& lim, const uint64_t* pow3,
                                     DevRec* tile_recs, DeviceLists lists, hipStream_t s) {
   const dim3 grid(nblocks), block(256);
-  if (limbs == 2)
+  if (limbs == 2 && lim.cutoff >= kWideLog2)
     hipLaunchKernelGGL((k_verify_persistent<4, false>), grid, block, 0, s, sv, rows, tiles, ctl,
+                       lim, pow3, tile_recs, lists, U);
+  else if (limbs == 2)
+    hipLaunchKernelGGL((k_verify_persistent<4, true>), grid, block, 0, s, sv, rows, tiles, ctl,
                        lim, pow3, tile_recs, lists, U);
   else if (lim.cutoff >= 128)
     hipLaunchKernelGGL((k_verify_persistent<6, false>), grid, block, 0, s, sv, rows, tiles, ctl,
