@@ -105,7 +105,7 @@ hipError_t Engine::enqueue(const GraphKey& k, const Limits& lim, DeviceLists lis
                            hipStream_t s) {
   hipError_t e = launch_engine_reset(ctl_.p, cnt_.p, 3, s);
   if (e != hipSuccess) return e;
-  e = launch_rows_resolve_ctl(rows_.p, ctl_.p, sv_->res.p, sv_->class_start_dev.p, keys_.p, s);
+  e = launch_rows_resolve_ctl(rows_.p, ctl_.p, sv_->rec.p, sv_->class_start_dev.p, keys_.p, s);
   if (e != hipSuccess) return e;
   e = launch_make_tiles(rows_.p, ctl_.p, tiles_.p, s);
   if (e != hipSuccess) return e;

@@ -331,9 +331,7 @@ int collatz_sieve_build(const collatz_sieve_params* params, collatz_sieve** out)
     DevBuf<uint32_t> counts;
     DevBuf<uint64_t> offs;
     const bool ok =
-        sv->res.alloc(std::max<uint64_t>(1, n)) == hipSuccess &&
-        sv->num.alloc(std::max<uint64_t>(1, n)) == hipSuccess &&
-        sv->meta.alloc(std::max<uint64_t>(1, n)) == hipSuccess &&
+        sv->rec.alloc(std::max<uint64_t>(1, n)) == hipSuccess &&
         counts.alloc(std::max<uint64_t>(1, nb)) == hipSuccess &&
         offs.alloc(nb + 1) == hipSuccess && sv->class_start_dev.alloc(10) == hipSuccess;
     if (!ok) {
@@ -341,7 +339,7 @@ int collatz_sieve_build(const collatz_sieve_params* params, collatz_sieve** out)
       return fail(COLLATZ_ENOMEM, "device allocation failed (sieve)");
     }
     hipError_t e = launch_class_partition(leaves.res.p, leaves.num.p, leaves.meta.p, n,
-                                          sv->res.p, sv->num.p, sv->meta.p, counts.p, offs.p,
+                                          sv->rec.p, counts.p, offs.p,
                                           sv->class_start, s);
     if (e == hipSuccess)
       e = hipMemcpy(sv->class_start_dev.p, sv->class_start, sizeof(sv->class_start),
@@ -374,8 +372,7 @@ int collatz_sieve_query(const collatz_sieve* sv, collatz_sieve_info* info) {
   info->mod9 = sv->mod9;
   info->survivors = sv->count;
   info->bitarray_bits = sv->bits_bits;
-  info->device_bytes = sv->res.bytes() + sv->num.bytes() + sv->meta.bytes() +
-                       sv->bits.bytes() + sv->pow3.bytes() + sv->class_start_dev.bytes();
+  info->device_bytes = sv->rec.bytes() + sv->bits.bytes() + sv->pow3.bytes() + sv->class_start_dev.bytes();
   info->build_ms = sv->build_ms;
   return COLLATZ_OK;
 }
@@ -386,21 +383,18 @@ int collatz_sieve_copy(const collatz_sieve* sv, uint64_t* r, uint64_t* number, u
   HipDeviceGuard guard(sv->device);
   // Survivors are stored class-major; return them sorted by residue.
   const uint64_t all = sv->count;
-  std::vector<uint64_t> hr(all), hn(all);
-  std::vector<uint32_t> hm(all);
-  if (all) {
-    CZ_HIP(hipMemcpy(hr.data(), sv->res.p, all * 8, hipMemcpyDeviceToHost));
-    CZ_HIP(hipMemcpy(hn.data(), sv->num.p, all * 8, hipMemcpyDeviceToHost));
-    CZ_HIP(hipMemcpy(hm.data(), sv->meta.p, all * 4, hipMemcpyDeviceToHost));
-  }
+  std::vector<uint4> hrec(all);
+  if (all) CZ_HIP(hipMemcpy(hrec.data(), sv->rec.p, all * sizeof(uint4), hipMemcpyDeviceToHost));
+  std::vector<uint64_t> hr(all);
+  for (uint64_t i = 0; i < all; ++i) hr[i] = rec_r(hrec[i]);
   std::vector<uint64_t> order(all);
   for (uint64_t i = 0; i < all; ++i) order[i] = i;
   std::sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return hr[a] < hr[b]; });
   const uint64_t n = std::min(cap, all);
   for (uint64_t i = 0; i < n; ++i) {
     if (r) r[i] = hr[order[i]];
-    if (number) number[i] = hn[order[i]];
-    if (f) f[i] = (uint8_t)meta_f(hm[order[i]]);
+    if (number) number[i] = rec_M(hrec[order[i]]);
+    if (f) f[i] = (uint8_t)rec_f(hrec[order[i]]);
   }
   return COLLATZ_OK;
 }

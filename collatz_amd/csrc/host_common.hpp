@@ -153,8 +153,7 @@ struct collatz_sieve {
   int device = 0;
   uint32_t k = 0, mod9 = 1;
   uint64_t count = 0;
-  cz::DevBuf<uint64_t> res, num;
-  cz::DevBuf<uint32_t> meta;
+  cz::DevBuf<uint4> rec;               // packed survivor records (SieveArrays)
   cz::DevBuf<uint32_t> bits;
   uint64_t bits_bits = 0;
   cz::DevBuf<uint64_t> pow3;
@@ -167,9 +166,7 @@ struct collatz_sieve {
   mutable std::mutex engine_mu;
   cz::SieveArrays arrays() const {
     cz::SieveArrays a;
-    a.res = res.p;
-    a.num = num.p;
-    a.meta = meta.p;
+    a.rec = rec.p;
     a.count = count;
     a.k = k;
     return a;

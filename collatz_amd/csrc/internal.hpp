@@ -108,10 +108,17 @@ struct Limits {
   uint64_t fast_thr;     // fast path: pair >= fast_thr, or any word above the pair nonzero
 };
 
+// One survivor as the range kernels read it, 16 bytes in one load:
+//   x = r bits 0..31, y = f | (r bits 32..55) << 8, z:w = T^k(r) + 1.
+// Survivors are stored class-major by r mod 9, each class sorted by r.
+__host__ __device__ inline uint64_t rec_r(const uint4& v) {
+  return v.x | ((uint64_t)(v.y >> 8) << 32);
+}
+__host__ __device__ inline uint32_t rec_f(const uint4& v) { return v.y & 0xFFu; }
+__host__ __device__ inline uint64_t rec_M(const uint4& v) { return v.z | ((uint64_t)v.w << 32); }
+
 struct SieveArrays {
-  const uint64_t* res;    // residues r (sorted)
-  const uint64_t* num;    // T^k(r) + 1
-  const uint32_t* meta;
+  const uint4* rec;
   uint64_t count;
   uint32_t k;
 };
@@ -144,20 +151,19 @@ hipError_t launch_compact_write(const uint32_t* bits, uint64_t nwords,
 hipError_t launch_sieve_fill(const SieveRoot& root, uint32_t p, const uint64_t* pnum,
                              const uint32_t* pmeta, uint64_t* res_inout, uint64_t* num,
                              uint32_t* meta, uint64_t count, hipStream_t s);
-hipError_t launch_lower_bound(const uint64_t* sorted, const uint64_t* class_start,
+hipError_t launch_lower_bound(const uint4* sorted, const uint64_t* class_start,
                               const uint64_t* keys, const uint32_t* kcls, uint64_t nkeys,
                               uint64_t* out, hipStream_t s);
 // Replaces flagged row bounds by their lower_bound in the row's class.
-hipError_t launch_rows_resolve(RowDesc* rows, uint64_t nrows, const uint64_t* sorted,
+hipError_t launch_rows_resolve(RowDesc* rows, uint64_t nrows, const uint4* sorted,
                                const uint64_t* class_start, const uint64_t* keys,
                                hipStream_t s);
 // Same, for ctl->nrows rows, with a fixed grid (graph-capturable).
-hipError_t launch_rows_resolve_ctl(RowDesc* rows, const Ctl* ctl, const uint64_t* sorted,
+hipError_t launch_rows_resolve_ctl(RowDesc* rows, const Ctl* ctl, const uint4* sorted,
                                    const uint64_t* class_start, const uint64_t* keys,
                                    hipStream_t s);
 hipError_t launch_class_partition(const uint64_t* res, const uint64_t* num, const uint32_t* meta,
-                                  uint64_t n, uint64_t* ores, uint64_t* onum, uint32_t* ometa,
-                                  uint32_t* counts, uint64_t* offs, uint64_t* class_start_host,
+                                  uint64_t n, uint4* orec, uint32_t* counts, uint64_t* offs, uint64_t* class_start_host,
                                   hipStream_t s);
 
 hipError_t launch_verify_plain(int limbs, const SieveArrays& sv, const RowDesc* rows,

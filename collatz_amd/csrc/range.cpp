@@ -36,7 +36,7 @@ int resolve_rows(const collatz_sieve* sv, RowPlan& plan, hipStream_t s) {
                           hipMemcpyHostToDevice, s));
     CZ_HIP(hipMemcpyAsync(dcls.p, plan.key_cls.data(), plan.key_cls.size() * 4,
                           hipMemcpyHostToDevice, s));
-    CZ_HIP(launch_lower_bound(sv->res.p, sv->class_start_dev.p, dkeys.p, dcls.p,
+    CZ_HIP(launch_lower_bound(sv->rec.p, sv->class_start_dev.p, dkeys.p, dcls.p,
                               plan.keys.size(), dout.p, s));
     CZ_HIP(hipMemcpyAsync(idx.data(), dout.p, idx.size() * 8, hipMemcpyDeviceToHost, s));
     CZ_HIP(hipStreamSynchronize(s));

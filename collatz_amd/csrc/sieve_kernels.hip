@@ -271,8 +271,7 @@ __global__ __launch_bounds__(256) void k_class_count(const uint32_t* meta, uint6
 __global__ __launch_bounds__(256) void k_class_write(const uint64_t* res, const uint64_t* num,
                                                      const uint32_t* meta, uint64_t n,
                                                      uint32_t cls, const uint64_t* offs,
-                                                     uint64_t out_base, uint64_t* ores,
-                                                     uint64_t* onum, uint32_t* ometa) {
+                                                     uint64_t out_base, uint4* orec) {
   __shared__ uint32_t lds4[4];
   const uint64_t base = (uint64_t)blockIdx.x * kPartItems + threadIdx.x * 8;
   uint32_t m[8];
@@ -289,9 +288,9 @@ __global__ __launch_bounds__(256) void k_class_write(const uint64_t* res, const 
   for (int j = 0; j < 8; ++j) {
     const uint64_t i = base + j;
     if (i < n && meta_mod9(m[j]) == cls) {
-      ores[o] = res[i];
-      onum[o] = num[i];
-      ometa[o] = m[j];
+      const uint64_t r = res[i], M = num[i];
+      orec[o] = make_uint4((uint32_t)r, meta_f(m[j]) | ((uint32_t)(r >> 32) << 8), (uint32_t)M,
+                           (uint32_t)(M >> 32));
       ++o;
     }
   }
@@ -299,7 +298,7 @@ __global__ __launch_bounds__(256) void k_class_write(const uint64_t* res, const 
 
 // keys[t] = residue bound, kcls[t] = class: searches class kcls[t] of a class-major array
 // whose classes start at class_start[0..9].
-__global__ __launch_bounds__(256) void k_lower_bound(const uint64_t* a,
+__global__ __launch_bounds__(256) void k_lower_bound(const uint4* a,
                                                      const uint64_t* class_start,
                                                      const uint64_t* keys, const uint32_t* kcls,
                                                      uint64_t nkeys, uint64_t* out) {
@@ -310,7 +309,7 @@ __global__ __launch_bounds__(256) void k_lower_bound(const uint64_t* a,
   uint64_t lo = class_start[c], hi = class_start[c + 1];
   while (lo < hi) {
     const uint64_t mid = lo + ((hi - lo) >> 1);
-    if (a[mid] < key) lo = mid + 1;
+    if (rec_r(a[mid]) < key) lo = mid + 1;
     else hi = mid;
   }
   out[t] = lo;
@@ -366,7 +365,7 @@ hipError_t launch_sieve_fill(const SieveRoot& root, uint32_t p, const uint64_t* 
   return hipGetLastError();
 }
 
-hipError_t launch_lower_bound(const uint64_t* sorted, const uint64_t* class_start,
+hipError_t launch_lower_bound(const uint4* sorted, const uint64_t* class_start,
                               const uint64_t* keys, const uint32_t* kcls, uint64_t nkeys,
                               uint64_t* out, hipStream_t s) {
   if (nkeys == 0) return hipSuccess;
@@ -376,18 +375,18 @@ hipError_t launch_lower_bound(const uint64_t* sorted, const uint64_t* class_star
 }
 
 
-__device__ __forceinline__ uint64_t lower_bound_dev(const uint64_t* a, uint64_t lo, uint64_t hi,
+__device__ __forceinline__ uint64_t lower_bound_dev(const uint4* a, uint64_t lo, uint64_t hi,
                                                     uint64_t key) {
   while (lo < hi) {
     const uint64_t mid = lo + ((hi - lo) >> 1);
-    if (a[mid] < key) lo = mid + 1;
+    if (rec_r(a[mid]) < key) lo = mid + 1;
     else hi = mid;
   }
   return lo;
 }
 
 __global__ __launch_bounds__(256) void k_rows_resolve(RowDesc* rows, uint64_t nrows,
-                                                      const uint64_t* a,
+                                                      const uint4* a,
                                                       const uint64_t* class_start,
                                                       const uint64_t* keys) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -402,7 +401,7 @@ __global__ __launch_bounds__(256) void k_rows_resolve(RowDesc* rows, uint64_t nr
   r.i_hi = i_hi;
 }
 
-hipError_t launch_rows_resolve(RowDesc* rows, uint64_t nrows, const uint64_t* sorted,
+hipError_t launch_rows_resolve(RowDesc* rows, uint64_t nrows, const uint4* sorted,
                                const uint64_t* class_start, const uint64_t* keys,
                                hipStream_t s) {
   if (nrows == 0) return hipSuccess;
@@ -413,7 +412,7 @@ hipError_t launch_rows_resolve(RowDesc* rows, uint64_t nrows, const uint64_t* so
 
 
 __global__ __launch_bounds__(256) void k_rows_resolve_ctl(RowDesc* rows, const Ctl* ctl,
-                                                          const uint64_t* a,
+                                                          const uint4* a,
                                                           const uint64_t* class_start,
                                                           const uint64_t* keys) {
   const uint64_t n = ctl->nrows;
@@ -430,7 +429,7 @@ __global__ __launch_bounds__(256) void k_rows_resolve_ctl(RowDesc* rows, const C
   }
 }
 
-hipError_t launch_rows_resolve_ctl(RowDesc* rows, const Ctl* ctl, const uint64_t* sorted,
+hipError_t launch_rows_resolve_ctl(RowDesc* rows, const Ctl* ctl, const uint4* sorted,
                                    const uint64_t* class_start, const uint64_t* keys,
                                    hipStream_t s) {
   hipLaunchKernelGGL(k_rows_resolve_ctl, dim3(512), dim3(256), 0, s, rows, ctl, sorted,
@@ -439,8 +438,7 @@ hipError_t launch_rows_resolve_ctl(RowDesc* rows, const Ctl* ctl, const uint64_t
 }
 
 hipError_t launch_class_partition(const uint64_t* res, const uint64_t* num, const uint32_t* meta,
-                                  uint64_t n, uint64_t* ores, uint64_t* onum, uint32_t* ometa,
-                                  uint32_t* counts, uint64_t* offs, uint64_t* class_start_host,
+                                  uint64_t n, uint4* orec, uint32_t* counts, uint64_t* offs, uint64_t* class_start_host,
                                   hipStream_t s) {
   const uint64_t nb = (n + kPartItems - 1) / kPartItems;
   uint64_t base = 0;
@@ -450,7 +448,7 @@ hipError_t launch_class_partition(const uint64_t* res, const uint64_t* num, cons
     hipLaunchKernelGGL(k_class_count, dim3((uint32_t)nb), dim3(256), 0, s, meta, n, c, counts);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nb, offs);
     hipLaunchKernelGGL(k_class_write, dim3((uint32_t)nb), dim3(256), 0, s, res, num, meta, n, c,
-                       offs, base, ores, onum, ometa);
+                       offs, base, orec);
     uint64_t cnt = 0;
     hipError_t e = hipMemcpyAsync(&cnt, offs + nb, 8, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return e;

@@ -91,16 +91,12 @@ template <int L>
 __device__ __forceinline__ bool cand_start(Cand<L>& c, uint64_t M, const uint32_t* q3,
                                            bool q3_big, uint64_t qb_lo, uint64_t qb_hi,
                                            uint64_t r) {
-  uint64_t t = (uint64_t)q3[0] + (uint32_t)M;
-  c.v.w[0] = (uint32_t)t;
-  t = (uint64_t)q3[1] + (uint32_t)(M >> 32) + (t >> 32);
-  c.v.w[1] = (uint32_t)t;
+  uint32_t cy = 0;
+  c.v.w[0] = addc(q3[0], (uint32_t)M, 0u, cy);
+  c.v.w[1] = addc(q3[1], (uint32_t)(M >> 32), cy, cy);
 #pragma unroll
-  for (int i = 2; i < L; ++i) {
-    t = (uint64_t)q3[i] + (t >> 32);
-    c.v.w[i] = (uint32_t)t;
-  }
-  const bool fits = (t >> 32) == 0 && !q3_big;
+  for (int i = 2; i < L; ++i) c.v.w[i] = addc(q3[i], 0u, cy, cy);
+  const bool fits = cy == 0 && !q3_big;
   const uint64_t nlo = qb_lo | r;
   c.n[0] = (uint32_t)nlo;
   c.n[1] = (uint32_t)(nlo >> 32);
@@ -246,11 +242,11 @@ __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowD
   for (uint32_t it = 0; it < R; ++it) {
     const uint64_t i = first + (uint64_t)it * 256;
     if (i >= row.i_hi) break;
-    const uint64_t r = sv.res[i];
-    const uint64_t M = sv.num[i];
-    const uint32_t meta = sv.meta[i];
+    const uint4 sr = sv.rec[i];
+    const uint64_t r = rec_r(sr);
+    const uint64_t M = rec_M(sr);
     Cand<L> c;
-    const uint32_t f = meta_f(meta);
+    const uint32_t f = rec_f(sr);
     c.slot = row.slot;
     const bool big = L == 4 ? (q3[f][4] | q3[f][5]) != 0 : false;
     if (!cand_start<L>(c, M, q3[f], big, row.qb_lo, row.qb_hi, r) ||
@@ -273,9 +269,10 @@ __global__ __launch_bounds__(256) void k_escalated(SieveArrays sv, const EscEntr
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const EscEntry e = esc[t];
-  const uint64_t r = sv.res[e.res_index];
-  const uint64_t M = sv.num[e.res_index];
-  const uint32_t f = meta_f(sv.meta[e.res_index]);
+  const uint4 sr = sv.rec[e.res_index];
+  const uint64_t r = rec_r(sr);
+  const uint64_t M = rec_M(sr);
+  const uint32_t f = rec_f(sr);
   const uint32_t k = sv.k;
   const uint64_t q_lo = k >= 64 ? (e.n_hi >> (k - 64)) : ((e.n_lo >> k) | (k ? e.n_hi << (64 - k) : 0));
   const uint64_t q_hi = k >= 64 ? 0 : (e.n_hi >> k);
@@ -359,14 +356,15 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
     uint64_t cidx = 0;
     // Prefetch slot.
     bool pf = false;
-    uint64_t pf_r = 0, pf_M = 0, pf_idx = 0;
-    uint32_t pf_meta = 0;
+    uint4 pf_rec = make_uint4(0, 0, 0, 0);
+    uint64_t pf_idx = 0;
     for (;;) {
       // (1) Lanes without a candidate start their prefetched one.
       if (!act && pf) {
         pf = false;
-        const uint32_t f = meta_f(pf_meta);
-        const bool fits = cand_start<L>(c, pf_M, T.q3[f], T.big[f] != 0, qb_lo, qb_hi, pf_r);
+        const uint32_t f = rec_f(pf_rec);
+        const bool fits = cand_start<L>(c, rec_M(pf_rec), T.q3[f], T.big[f] != 0, qb_lo, qb_hi,
+                                        rec_r(pf_rec));
         c.slot = slot;
         cidx = pf_idx;
         if (fits && cand_enter<L>(c, sink)) act = true;
@@ -382,9 +380,7 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
             (uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
         if (!pf && rank < take) {
           const uint64_t i = cur + rank;
-          pf_r = sv.res[i];
-          pf_M = sv.num[i];
-          pf_meta = sv.meta[i];
+          pf_rec = sv.rec[i];
           pf_idx = i;
           pf = true;
         }
