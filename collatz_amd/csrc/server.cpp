@@ -55,7 +55,7 @@ struct collatz_server {
   bool stop = false;
   double t0 = 0;
   uint64_t bytes_in_use = 0;
-  cz::Tuning tn{16384, 28, 2};
+  cz::Tuning tn{16384, 32, 2};
 };
 
 namespace cz {

@@ -1,11 +1,12 @@
 // Autotuner for the persistent kernel.  For each (limbs, magnitude bucket) of a sieve it
 // times a short synthetic range of that magnitude under a small grid of launch shapes
-// (tile size = residues per wave visit, waves per CU, checks per refill) and keeps the
-// fastest.  Results are cached in the sieve handle; COLLATZ_AUTOTUNE=0 disables tuning.
+// (waves per CU, checks per refill) and keeps the fastest; the tile size (residues per
+// wave visit) follows the size of each launch.  Results are cached in the sieve handle; COLLATZ_AUTOTUNE=0 disables tuning.
 #include <cstdlib>
 #include <map>
 #include <mutex>
 
+#include "engine.hpp"
 #include "tuning.hpp"
 
 namespace cz {
@@ -26,14 +27,25 @@ static uint32_t bitlen(u128 x) {
   return b;
 }
 
-static const Tuning kDefault = {16384, 24, 2};
+// Waves per CU: 32 is the occupancy limit of the kernel (under 64 VGPRs, eight waves per
+// SIMD).  Checks between refills: 2.
+static const uint32_t kDefaultWaves = 32, kDefaultU = 2;
+
+// Survivors per tile: the largest power of two in [4096, 65536] that still gives every
+// resident wave about 16 tiles, so that work stealing evens out the tail while the per-tile
+// set-up (the q * 3^f table, the record merge) stays small.
+uint32_t adaptive_tile(const collatz_sieve* sv, uint64_t candidates, uint32_t waves) {
+  const uint64_t nwaves = (uint64_t)device_cu_count(sv->device) * waves;
+  uint32_t t = 65536;
+  while (t > 4096 && candidates / t < 16 * nwaves) t >>= 1;
+  return t;
+}
 
 Tuning choose_tuning(const collatz_sieve* sv, int limbs, u128 max_hi, uint64_t candidates,
                      uint32_t tile, uint32_t waves, uint32_t U) {
-  if (tile && waves && U) return Tuning{tile, waves, U};
-  Tuning t = kDefault;
+  Tuning t{0, kDefaultWaves, kDefaultU};
   const char* env = std::getenv("COLLATZ_AUTOTUNE");
-  const bool tune = !(env && env[0] == '0') && candidates > (1u << 22);
+  const bool tune = !(env && env[0] == '0') && candidates > (1u << 22) && !(waves && U);
   const uint32_t bucket = (uint32_t)limbs * 256 + bitlen(max_hi) / 8;
   const auto key = std::make_pair((const void*)sv, bucket);
   if (tune) {
@@ -42,35 +54,41 @@ Tuning choose_tuning(const collatz_sieve* sv, int limbs, u128 max_hi, uint64_t c
     if (it != g_cache.end()) {
       t = it->second;
     } else {
-      // A range of the bucket's magnitude with about 2^26 candidates (enough tiles for the
-      // largest tile size to show its tail).
+      // A range of the bucket's magnitude with about 2^27 candidates, each shape timed
+      // twice (the faster counts).
       const uint32_t bits = std::max<uint32_t>(bitlen(max_hi), sv->k + 1);
       const double density = (double)sv->count / (double)((u128)1 << sv->k) * 5.0 / 9.0;
-      double w = (double)(1u << 26) / (density > 0 ? density : 1e-3);
+      double w = (double)(1u << 27) / (density > 0 ? density : 1e-3);
       u128 width = (u128)w;
       if (width < ((u128)1 << 20)) width = (u128)1 << 20;
       const u128 lo = (u128)1 << (bits - 1);
       const u128 hi = lo + width;
       double best = 1e30;
-      static const uint32_t tiles[] = {4096, 16384, 65536};
-      static const uint32_t wavess[] = {16, 24, 28};
+      static const uint32_t wavess[] = {24, 28, 32};
       static const uint32_t Us[] = {1, 2, 3};
-      for (uint32_t a : tiles)
-        for (uint32_t b : wavess)
-          for (uint32_t c : Us) {
-            double ms = 0;
-            if (verify_batch_tuned(sv, lo, hi, limbs, Tuning{a, b, c}, nullptr, &ms) != 0) continue;
-            if (ms < best) {
-              best = ms;
-              t = Tuning{a, b, c};
+      for (uint32_t b : wavess)
+        for (uint32_t c : Us) {
+          const Tuning cand{adaptive_tile(sv, 1u << 27, b), b, c};
+          double ms = 1e30;
+          for (int rep = 0; rep < 2; ++rep) {
+            double m = 0;
+            if (verify_batch_tuned(sv, lo, hi, limbs, cand, nullptr, &m) != 0) {
+              m = 1e30;
+              rep = 2;
             }
+            ms = std::min(ms, m);
           }
+          if (ms < best) {
+            best = ms;
+            t = Tuning{0, b, c};
+          }
+        }
       g_cache[key] = t;
     }
   }
-  if (tile) t.tile = tile;
   if (waves) t.waves = waves;
   if (U) t.U = U;
+  t.tile = tile ? tile : adaptive_tile(sv, candidates, t.waves);
   return t;
 }
 
