@@ -46,6 +46,13 @@ OPS_SETUP = 4 * L_WORDS + 8
 # gfx950 int32 VALU issue peak: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T lane-ops/s.
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 RATES = (50, 100, 200, 500)
+# HBM traffic of the batched launch, measured with rocprofv3 on this workload
+# (profiles/k_verify_persistent_batch4096_k32.md): FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0,
+# the factors a 1 GiB copy kernel gives.  Reads are the 16-byte survivor record of each
+# candidate (15.85 B per candidate measured, L2 hits make up the rest); writes are the
+# per-tile records.
+READ_BYTES_PER_CANDIDATE = 15.85
+WRITE_BYTES_PER_LAUNCH = 0.155e9
 
 
 def log(msg):
@@ -200,7 +207,10 @@ def main():
         "candidates": count, "checks": checks, "kernel_ms": kern_ms,
         "total_hash": "%08X" % h,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS,
-                     "unit": "TOP/s", "frac": achieved / VALU_PEAK_TOPS, "traffic": None,
+                     "unit": "TOP/s", "frac": achieved / VALU_PEAK_TOPS,
+                     "traffic": int(count * READ_BYTES_PER_CANDIDATE + WRITE_BYTES_PER_LAUNCH),
+                     "hbm_GBps": (count * READ_BYTES_PER_CANDIDATE + WRITE_BYTES_PER_LAUNCH)
+                                 / (kern_ms * 1e-3) / 1e9,
                      "kernel": "k_verify_persistent<4>",
                      "ops": "checks x (13L+7) + candidates x (4L+8), L=4"},
         "cpu_baseline": None,
