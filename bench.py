@@ -138,6 +138,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-range", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--large-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-intervals", type=int, default=32)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -251,6 +253,31 @@ def main():
         line["range_2p40_w2p36"] = rng
         log(f"range: plain {rng['plain']['n_per_s']:.3e}, persistent "
             f"{rng['persistent']['n_per_s']:.3e} n/s")
+
+    if not args.no_large:
+        # Large sieve: [2^60, 2^60 + 2^44) with k = 37 (|S_37| records resident), 2^30-wide
+        # chunks, batched form.
+        t = time.time()
+        s37 = cz.Sieve(37)
+        build_s = time.time() - t
+        lo, hi = 1 << 60, (1 << 60) + (1 << 44)
+        s37.verify_batch([(lo, hi)], chunk_log2=30, use_graph=True)
+        ms = []
+        t0 = time.perf_counter()
+        for _ in range(args.large_steps):
+            o = s37.verify_batch([(lo, hi)], chunk_log2=30, use_graph=True)[0]
+            ms.append(o.device_ms)
+        d = (time.perf_counter() - t0) / args.large_steps
+        kms = sum(ms) / len(ms)
+        ach = (int(o.checks) * OPS_DR + int(o.total["count"]) * OPS_SETUP) / (kms * 1e-3) / 1e12
+        line["large_sieve_k37"] = {
+            "range": "[2^60, 2^60+2^44)", "chunk_log2": 30, "n_per_s": (hi - lo) / d,
+            "ms_per_step": d * 1e3, "kernel_ms": kms, "candidates": int(o.total["count"]),
+            "survivors": int(s37.info().survivors), "sieve_build_s": build_s,
+            "escalated": int(o.escalated), "hash": "%08X" % int(o.total["hash"]),
+            "roofline_frac": ach / VALU_PEAK_TOPS}
+        log(f"large sieve k=37: {line['large_sieve_k37']}")
+        del s37
 
     if not args.no_serve:
         reqs = W.request_stream(1, 20000)

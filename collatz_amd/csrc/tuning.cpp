@@ -46,7 +46,7 @@ Tuning choose_tuning(const collatz_sieve* sv, int limbs, u128 max_hi, uint64_t c
   Tuning t{0, kDefaultWaves, kDefaultU};
   const char* env = std::getenv("COLLATZ_AUTOTUNE");
   const bool tune = !(env && env[0] == '0') && candidates > (1u << 22) && !(waves && U);
-  const uint32_t bucket = (uint32_t)limbs * 256 + bitlen(max_hi) / 8;
+  const uint32_t bucket = (sv->k << 16) + (uint32_t)limbs * 256 + bitlen(max_hi) / 8;
   const auto key = std::make_pair((const void*)sv, bucket);
   if (tune) {
     std::lock_guard<std::mutex> lk(g_mu);
