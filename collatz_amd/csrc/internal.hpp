@@ -100,6 +100,9 @@ struct DeviceLists {
 // computed are a few thousand T-steps; a candidate still running after this many checks
 // means a defect, and is counted instead of looping forever.
 constexpr uint32_t kMaxChecks = 1u << 16;
+// The same bound in T-steps for the persistent kernel: every half-check (odd run, even run)
+// takes at least one step, so 2^18 steps mean at most 2^17 checks.
+constexpr uint32_t kMaxSteps = 1u << 18;
 
 // Limits of the schedule for one launch (cutoff = CUTOFFPOWER, collatz_cudav7.cu:44-46).
 struct Limits {

@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
           even_run<L>(v);
         }
         h ^= v.w[0];
-        done = lt_base<L>(v, n);
+        done = lt_base<L>(v, pair(n[0], n[1]), pair(n[2], n[3]));
         if (++checks >= kMaxChecks && !done) {
           stall += 1;
           done = true;

@@ -69,9 +69,10 @@ __device__ __forceinline__ uint32_t even_run(Num<L>& v) {
   v.w[L - 1] = v.w[L - 1] >> b;
   v.w[0] = x0;
   // The + 1 carries out of the low word only if the shifted low word was 2^32 - 1: one
-  // case in 2^31, taken as a branch rather than a carry chain on every run.
-  if (__builtin_expect(x0 == 0u && b != 0u, 0)) {
-    uint32_t c = 1;
+  // case in 2^31, taken as a branch rather than a carry chain on every run.  With b = 0
+  // the low word wrapped without a carry (see above), so the carry is b != 0.
+  if (__builtin_expect(x0 == 0u, 0)) {
+    uint32_t c = b != 0u ? 1u : 0u;
 #pragma unroll
     for (int i = 1; i < L; ++i) v.w[i] = addc(v.w[i], 0u, c, c);
   }
@@ -82,25 +83,26 @@ __device__ __forceinline__ uint64_t pair(uint32_t lo, uint32_t hi) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// x >= y over L words, as 64-bit pair compares.
+// x >= y over L words, as 64-bit pair compares combined without branches (bitwise, not
+// short-circuit, so every lane takes the same three or five compares).
 template <int L>
 __device__ __forceinline__ bool ge_words(const uint32_t* x, const uint32_t* y) {
+  const uint64_t xh = pair(x[2], x[3]), yh = pair(y[2], y[3]);
+  const bool ge = (xh > yh) | ((xh == yh) & (pair(x[0], x[1]) >= pair(y[0], y[1])));
   if constexpr (L == 4) {
-    const uint64_t xh = pair(x[2], x[3]), yh = pair(y[2], y[3]);
-    return xh > yh || (xh == yh && pair(x[0], x[1]) >= pair(y[0], y[1]));
+    return ge;
   } else {
     const uint64_t xt = pair(x[4], x[5]), yt = pair(y[4], y[5]);
-    const uint64_t xh = pair(x[2], x[3]), yh = pair(y[2], y[3]);
-    return xt > yt || (xt == yt && (xh > yh || (xh == yh && pair(x[0], x[1]) >= pair(y[0], y[1]))));
+    return (xt > yt) | ((xt == yt) & ge);
   }
 }
 
-// number < n where n has 4 words.
+// number < n for n = n_lo + 2^64 n_hi.
 template <int L>
-__device__ __forceinline__ bool lt_base(const Num<L>& v, const uint32_t* n) {
-  const uint64_t vh = pair(v.w[2], v.w[3]), nh = pair(n[2], n[3]);
-  bool lt = vh < nh || (vh == nh && pair(v.w[0], v.w[1]) < pair(n[0], n[1]));
-  if constexpr (L > 4) lt = lt && (v.w[4] | v.w[5]) == 0;
+__device__ __forceinline__ bool lt_base(const Num<L>& v, uint64_t n_lo, uint64_t n_hi) {
+  const uint64_t vh = pair(v.w[2], v.w[3]);
+  bool lt = (vh < n_hi) | ((vh == n_hi) & (pair(v.w[0], v.w[1]) < n_lo));
+  if constexpr (L > 4) lt = lt & ((v.w[4] | v.w[5]) == 0);
   return lt;
 }
 

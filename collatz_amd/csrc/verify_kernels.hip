@@ -30,7 +30,6 @@ struct Cand {
   uint32_t n[4];
   uint32_t hash;
   uint32_t steps;
-  uint32_t checks;
   uint32_t ovf;
   uint32_t slot;   // output record slot (for overflow and escalation reports)
 };
@@ -104,7 +103,6 @@ __device__ __forceinline__ bool cand_start(Cand<L>& c, uint64_t M, const uint32_
   c.n[3] = (uint32_t)(qb_hi >> 32);
   c.hash = 0;
   c.steps = 0;
-  c.checks = 0;
   c.ovf = 0;
   return fits;
 }
@@ -142,8 +140,8 @@ __device__ __forceinline__ void push_escalate(const DeviceLists& lists, const ui
 // three limbs LOWCUT means a cutoff below 2^128.
 template <int L, bool LOWCUT, class Sink>
 __device__ __forceinline__ bool after_odd(Cand<L>& c, Sink& sink, const Limits& lim,
-                                          const DeviceLists& lists) {
-  sink.offer(c.v, c.n);
+                                          const DeviceLists& lists, bool escalating) {
+  if (!escalating) sink.offer(c.v, c.n);
   if constexpr (L == 4 && !LOWCUT) {
     return pair(c.v.w[2], c.v.w[3]) >= lim.fast_thr;
   } else {
@@ -159,21 +157,24 @@ __device__ __forceinline__ bool after_odd(Cand<L>& c, Sink& sink, const Limits& 
   }
 }
 
-// One check.  Returns 0 while running, 1 when the candidate finished (number < n), 2 when
-// it has to be escalated.
+// One check.  `done`: number < n after it.  `esc` (two limbs only): a value reached 2^123
+// and the candidate goes to the three-limb pass; the check still runs to its end, on a
+// value that may no longer fit, so its later offers are skipped and esc wins over done.
+// Flags instead of an early return keep the common path free of branches.
 template <int L, bool LOWCUT, class Sink>
-__device__ __forceinline__ uint32_t one_check(Cand<L>& c, Sink& sink, const Limits& lim,
-                                              const DeviceLists& lists, const Pow3Lds& p3) {
-  uint32_t a1 = odd_run<L>(c.v, p3);
-  if (after_odd<L, LOWCUT>(c, sink, lim, lists)) return 2;
-  uint32_t b1 = even_run<L>(c.v);
-  uint32_t a2 = odd_run<L>(c.v, p3);
-  if (after_odd<L, LOWCUT>(c, sink, lim, lists)) return 2;
-  uint32_t b2 = even_run<L>(c.v);
+__device__ __forceinline__ void one_check(Cand<L>& c, Sink& sink, const Limits& lim,
+                                          const DeviceLists& lists, const Pow3Lds& p3,
+                                          bool& esc, bool& done) {
+  const uint32_t a1 = odd_run<L>(c.v, p3);
+  bool e = after_odd<L, LOWCUT>(c, sink, lim, lists, false);
+  const uint32_t b1 = even_run<L>(c.v);
+  const uint32_t a2 = odd_run<L>(c.v, p3);
+  e = after_odd<L, LOWCUT>(c, sink, lim, lists, e) || e;
+  const uint32_t b2 = even_run<L>(c.v);
   c.steps += a1 + b1 + a2 + b2;
-  c.checks += 1;
   c.hash ^= c.v.w[0];
-  return lt_base<L>(c.v, c.n) ? 1u : 0u;
+  esc = e;
+  done = lt_base<L>(c.v, pair(c.n[0], c.n[1]), pair(c.n[2], c.n[3]));
 }
 
 template <int L>
@@ -184,7 +185,6 @@ __device__ __forceinline__ void finish(const Cand<L>& c, Acc<L>& acc, uint32_t k
   acc.ssum += J;
   acc.sxor ^= J;
   acc.ovf += c.ovf;
-  acc.chk += c.checks;
 }
 
 // Start-of-candidate test: the start value is offered to the max, and on two limbs a start
@@ -204,12 +204,14 @@ __device__ __forceinline__ bool run_candidate(Cand<L>& c, Acc<L>& acc, const Lim
   SinkOne<L> sink{acc};
   if (!cand_enter<L>(c, sink)) return false;
   for (uint32_t guard = 0; guard < kMaxChecks; ++guard) {
-    const uint32_t s = one_check<L, LOWCUT>(c, sink, lim, lists, p3);
-    if (s == 1) {
+    bool esc, done;
+    one_check<L, LOWCUT>(c, sink, lim, lists, p3, esc, done);
+    acc.chk += 1;
+    if (esc) return false;
+    if (done) {
       finish<L>(c, acc, k);
       return true;
     }
-    if (s == 2) return false;
   }
   atomicAdd(lists.stall_count, 1ull);
   return true;
@@ -390,14 +392,12 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
       // (3) Up to U checks.
       if (act) {
         for (uint32_t u = 0; u < U; ++u) {
-          const uint32_t st = one_check<L, LOWCUT>(c, sink, lim, lists, p3s);
-          if (st == 1) {
-            finish<L>(c, acc, sv.k);
-            act = false;
-            break;
-          }
-          if (st == 2 || c.checks >= kMaxChecks) {
-            if (st == 2) push_escalate(lists, c.n, cidx, slot);
+          bool esc, done;
+          one_check<L, LOWCUT>(c, sink, lim, lists, p3s, esc, done);
+          acc.chk += 1;
+          if (esc || done || c.steps >= kMaxSteps) {
+            if (esc) push_escalate(lists, c.n, cidx, slot);
+            else if (done) finish<L>(c, acc, sv.k);
             else atomicAdd(lists.stall_count, 1ull);
             act = false;
             break;
