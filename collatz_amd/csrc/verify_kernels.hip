@@ -304,10 +304,12 @@ __global__ __launch_bounds__(256) void k_escalated(SieveArrays sv, const EscEntr
 // the q * 3^f table) is the same for the whole tile and stays in SGPRs and LDS.
 constexpr int kWavesPerBlock = 4;
 
+// q * 3^f for f < 41 on L words.  Where the product does not fit L words the row is all
+// ones: adding any start value (>= 1) then carries out, which marks the candidate for the
+// three-limb pass without a separate flag.
 template <int L>
 struct WaveTables {
-  uint32_t q3[41][L];   // q * 3^f
-  uint32_t big[41];     // q * 3^f does not fit L words
+  uint32_t q3[41][L];
 };
 
 template <int L, bool LOWCUT>
@@ -332,17 +334,19 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
     const TileDesc td = tiles[t];
     const RowDesc rw = rows[td.row];
     const uint64_t len = rw.i_hi - rw.i_lo;
-    uint64_t cur = rw.i_lo + len * td.piece / td.pieces;
-    const uint64_t end = rw.i_lo + len * (td.piece + 1) / td.pieces;
+    // The tile is survivors [first, first + n) of the row (n < 2^32: a residue class of the
+    // deepest sieve holds fewer than 2^32 survivors); the cursor is an offset into it.
+    const uint64_t first = rw.i_lo + len * td.piece / td.pieces;
+    const uint32_t n = (uint32_t)(rw.i_lo + len * (td.piece + 1) / td.pieces - first);
+    uint32_t cur = 0;
     if (lane < 41) {
       uint32_t w6[6];
       q_times_pow3(rw.q_lo, rw.q_hi, pow3[lane], w6);
-#pragma unroll
-      for (int i = 0; i < L; ++i) T.q3[lane][i] = w6[i];
       uint32_t big = 0;
 #pragma unroll
       for (int i = L; i < 6; ++i) big |= w6[i];
-      T.big[lane] = big != 0;
+#pragma unroll
+      for (int i = 0; i < L; ++i) T.q3[lane][i] = big ? 0xFFFFFFFFu : w6[i];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -365,7 +369,7 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
       if (!act && pf) {
         pf = false;
         const uint32_t f = rec_f(pf_rec);
-        const bool fits = cand_start<L>(c, rec_M(pf_rec), T.q3[f], T.big[f] != 0, qb_lo, qb_hi,
+        const bool fits = cand_start<L>(c, rec_M(pf_rec), T.q3[f], false, qb_lo, qb_hi,
                                         rec_r(pf_rec));
         c.slot = slot;
         cidx = pf_idx;
@@ -374,21 +378,21 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
       }
       // (2) Lanes without a prefetch take the next survivors of the tile.
       const uint64_t need = __ballot(!pf);
-      if (need && cur < end) {
+      if (need && cur < n) {
         const uint32_t want = __popcll(need);
-        const uint64_t avail = end - cur;
-        const uint32_t take = avail < want ? (uint32_t)avail : want;
+        const uint32_t avail = n - cur;
+        const uint32_t take = avail < want ? avail : want;
         const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
             (uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
         if (!pf && rank < take) {
-          const uint64_t i = cur + rank;
+          const uint64_t i = first + cur + rank;
           pf_rec = sv.rec[i];
           pf_idx = i;
           pf = true;
         }
         cur += take;
       }
-      if (__ballot(act) == 0 && __ballot(pf) == 0) break;
+      if (__ballot(act || pf) == 0) break;
       // (3) Up to U checks.
       if (act) {
         for (uint32_t u = 0; u < U; ++u) {
