@@ -37,12 +37,22 @@ METRIC = "verified n/sec on 1 GPU (batched/served vs plain, bit-exact); p50/p99 
 K = 32
 BATCH_CHUNK = 32
 RANGE_LO, RANGE_WIDTH, RANGE_CHUNK = 1 << 40, 1 << 36, 24
-# Algorithmic integer work, in 32-bit lane operations, for L 32-bit words per value
-# (L = 4 for two u64 limbs): per check (two odd and two even runs, hash and compare)
-# OPS_DR = 13 L + 7, and per candidate (mod 9, start value, base) OPS_SETUP = 4 L + 8.
-L_WORDS = 4
-OPS_DR = 13 * L_WORDS + 7
-OPS_SETUP = 4 * L_WORDS + 8
+# Algorithmic integer work, in 32-bit lane operations, for L 32-bit words per value: per
+# check (two odd and two even runs, hash and compare) OPS_DR = 13 L + 7, and per candidate
+# (mod 9, start value, base) OPS_SETUP = 4 L + 8.  Two-limb work whose starts stay below
+# 2^68 runs on L = 3 words (96 bits, escalating at 2^91); otherwise L = 4.
+def ops_dr(L):
+    return 13 * L + 7
+
+
+def ops_setup(L):
+    return 4 * L + 8
+
+
+def words_for(max_hi):
+    return 3 if max_hi <= (1 << 68) else 4
+
+
 # gfx950 int32 VALU issue peak: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T lane-ops/s.
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 RATES = (50, 100, 200, 500)
@@ -188,7 +198,8 @@ def main():
     count = sum(int(o.total["count"]) for o in outs)
     checks = sum(int(o.checks) for o in outs)
     kern_ms = sum(dev_ms) / len(dev_ms)
-    ops = checks * OPS_DR + count * OPS_SETUP
+    Lw = words_for(max(hi for _, hi in ivs))
+    ops = checks * ops_dr(Lw) + count * ops_setup(Lw)
     achieved = ops / (kern_ms * 1e-3) / 1e12
     value = total_n / (dt / args.steps)
     h = 0
@@ -207,14 +218,15 @@ def main():
                    "integers_per_step": total_n, "survivors": int(info.survivors)},
         "survivors_per_s": count / (dt / args.steps),
         "candidates": count, "checks": checks, "kernel_ms": kern_ms,
+        "escalated": sum(int(o.escalated) for o in outs),
         "total_hash": "%08X" % h,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS,
                      "unit": "TOP/s", "frac": achieved / VALU_PEAK_TOPS,
                      "traffic": int(count * READ_BYTES_PER_CANDIDATE + WRITE_BYTES_PER_LAUNCH),
                      "hbm_GBps": (count * READ_BYTES_PER_CANDIDATE + WRITE_BYTES_PER_LAUNCH)
                                  / (kern_ms * 1e-3) / 1e9,
-                     "kernel": "k_verify_persistent<4>",
-                     "ops": "checks x (13L+7) + candidates x (4L+8), L=4"},
+                     "kernel": f"k_verify_persistent<{Lw}>",
+                     "ops": f"checks x (13L+7) + candidates x (4L+8), L={Lw}"},
         "cpu_baseline": None,
     }
 
@@ -269,7 +281,8 @@ def main():
             ms.append(o.device_ms)
         d = (time.perf_counter() - t0) / args.large_steps
         kms = sum(ms) / len(ms)
-        ach = (int(o.checks) * OPS_DR + int(o.total["count"]) * OPS_SETUP) / (kms * 1e-3) / 1e12
+        L37 = words_for(hi)
+        ach = (int(o.checks) * ops_dr(L37) + int(o.total["count"]) * ops_setup(L37)) / (kms * 1e-3) / 1e12
         line["large_sieve_k37"] = {
             "range": "[2^60, 2^60+2^44)", "chunk_log2": 30, "n_per_s": (hi - lo) / d,
             "ms_per_step": d * 1e3, "kernel_ms": kms, "candidates": int(o.total["count"]),

@@ -18,6 +18,7 @@ namespace cz {
 struct BatchArgs {
   uint32_t C = 24, limbs = 2, cutoff = 187, graph = 0;
   uint32_t tile = 0, waves = 0, U = 0;
+  uint32_t width = 0;  // word width of the kernels; 0 = launch_width
 };
 
 static uint64_t estimate_candidates(const collatz_sieve* sv, u128 width) {
@@ -36,7 +37,8 @@ int verify_batch_impl(const collatz_sieve* sv, uint64_t n, const collatz_interva
     if (hi > max_hi) max_hi = hi;
     est += estimate_candidates(sv, hi - lo);
   }
-  const Tuning tn = choose_tuning(sv, (int)a.limbs, max_hi, est, a.tile, a.waves, a.U);
+  const int width = a.width ? (int)a.width : launch_width((int)a.limbs, max_hi, a.cutoff);
+  const Tuning tn = choose_tuning(sv, width, max_hi, est, a.tile, a.waves, a.U);
   Span sp_all("batch total");
   Span* sp_plan = new Span("batch plan");
   LaunchPlan lp;
@@ -61,12 +63,13 @@ int verify_batch_impl(const collatz_sieve* sv, uint64_t n, const collatz_interva
     std::lock_guard<std::mutex> lk(sv->engine_mu);
     if (!sv->engine) sv->engine = new (std::nothrow) Engine(sv);
     if (!sv->engine) return fail(COLLATZ_ENOMEM, "out of host memory");
-    rc = sv->engine->run(lp, tn, (int)a.limbs, a.cutoff, a.graph != 0, ovf_cap, s, res);
+    rc = sv->engine->run(lp, tn, width, a.cutoff, a.graph != 0, ovf_cap, s, res);
   }
-  if (rc == COLLATZ_ECAPACITY && a.limbs == 2 && allow_retry) {
+  if (rc == COLLATZ_ECAPACITY && width < 3 && allow_retry) {
+    // More escalations than the list holds: the same batch one width up.
     BatchArgs b = a;
-    b.limbs = 3;
-    return verify_batch_impl(sv, n, iv, b, s, outs, false);
+    b.width = (uint32_t)width + 1;
+    return verify_batch_impl(sv, n, iv, b, s, outs, true);
   }
   if (rc) return rc;
 
@@ -123,7 +126,8 @@ int verify_batch_tuned(const collatz_sieve* sv, u128 lo, u128 hi, int limbs, Tun
   iv.hi[1] = hi64(hi);
   BatchArgs a;
   a.C = 32;
-  a.limbs = (uint32_t)limbs;
+  a.limbs = limbs == 3 ? 3u : 2u;
+  a.width = (uint32_t)limbs;
   a.tile = t.tile;
   a.waves = t.waves;
   a.U = t.U;

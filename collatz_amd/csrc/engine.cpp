@@ -141,7 +141,7 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   DeviceLists lists;
   lists.esc = esc_.p;
   lists.esc_count = cnt_.p;
-  lists.esc_cap = limbs == 2 ? esc_.n : 0;
+  lists.esc_cap = limbs < 3 ? esc_.n : 0;
   lists.ovf = ovf_.p;
   lists.ovf_count = cnt_.p + 1;
   lists.ovf_cap = ovf_cap;
@@ -198,8 +198,8 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   out.device_ms = ms;
   out.recs.assign(h_recs_.p, h_recs_.p + lp.nslots);
   const uint64_t esc_n = h_cnt_.p[0], stall = h_cnt_.p[2];
-  if (limbs == 2 && esc_n > esc_.n)
-    return fail(COLLATZ_ECAPACITY, "escalation list overflowed; rerun with limbs = 3");
+  if (limbs < 3 && esc_n > esc_.n)
+    return fail(COLLATZ_ECAPACITY, "escalation list overflowed; rerun one width up");
   out.escalated = esc_n;
   out.esc_slots.clear();
   if (esc_n) {

@@ -88,10 +88,18 @@ void chunk_merge(collatz_chunk* a, const collatz_chunk& b) {
   a->flags |= b.flags;
 }
 
+int launch_width(int limbs, u128 max_hi, uint32_t cutoff) {
+  if (limbs == 2 && cutoff >= kNarrowLog2 && max_hi <= ((u128)1 << kNarrowMaxStartLog2)) return 1;
+  return limbs;
+}
+
 Limits make_limits(int limbs, uint32_t cutoff) {
   Limits l;
   l.cutoff = cutoff;
-  if (limbs == 2) {
+  if (limbs == 1) {
+    l.fast_word = 2;
+    l.fast_thr = 1ull << (kNarrowLog2 - 64);
+  } else if (limbs == 2) {
     const uint32_t c = std::min<uint32_t>(cutoff, kWideLog2);
     l.fast_word = 2;
     l.fast_thr = 1ull << (c - 64);

@@ -121,6 +121,11 @@ int plan_rows(uint32_t k, u128 lo, u128 hi, uint32_t C, uint32_t slot_base, bool
 int resolve_rows(const struct ::collatz_sieve* sv, RowPlan& plan, hipStream_t s);
 
 Limits make_limits(int limbs, uint32_t cutoff);
+// Word width of a launch for a two- or three-limb request: two-limb work whose starts are
+// below 2^68 and whose cutoff is at least 2^91 runs on 96 bits (width 1); otherwise the
+// width is the limb count.  A launch whose escalations overflow the list is rerun one width
+// up.
+int launch_width(int limbs, u128 max_hi, uint32_t cutoff);
 
 // Leaves of the search1 tree below a root at root.depth, sorted by residue: the device
 // sieve (sieve_kernels.hip) behind collatz_sieve_build and reference mode.

@@ -99,6 +99,12 @@ struct DeviceLists {
 // Safety bound on checks per candidate.  Stopping times of all n < 2^128 that have been
 // computed are a few thousand T-steps; a candidate still running after this many checks
 // means a defect, and is counted instead of looping forever.
+// Two-limb work whose starts stay below 2^68 runs on three 32-bit words (96 bits) and
+// escalates at 2^91 (an odd run multiplies by less than 2^5).  Survivors climb by 2^x above
+// their start with probability about 2^-(x-5), so below 2^68 fewer than one candidate in
+// 10^6 escalates.
+constexpr uint32_t kNarrowLog2 = 91;
+constexpr uint32_t kNarrowMaxStartLog2 = 68;
 constexpr uint32_t kMaxChecks = 1u << 16;
 // The same bound in T-steps for the persistent kernel: every half-check (odd run, even run)
 // takes at least one step, so 2^18 steps mean at most 2^17 checks.
@@ -169,6 +175,8 @@ hipError_t launch_class_partition(const uint64_t* res, const uint64_t* num, cons
                                   uint64_t n, uint4* orec, uint32_t* counts, uint64_t* offs, uint64_t* class_start_host,
                                   hipStream_t s);
 
+// Range kernels: `limbs` is the word width, 1 = three 32-bit words (96 bits, escalating at
+// 2^91), 2 = two u64 limbs (escalating at 2^123), 3 = three u64 limbs.
 hipError_t launch_verify_plain(int limbs, const SieveArrays& sv, const RowDesc* rows,
                                uint32_t nrows, uint64_t nblocks, uint32_t R, uint32_t mod9,
                                const Limits& lim, const uint64_t* pow3, DevRec* block_recs,

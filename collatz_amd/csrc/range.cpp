@@ -9,6 +9,7 @@ namespace cz {
 
 struct RangeArgs {
   uint32_t C = 24, limbs = 2, cutoff = 187;
+  uint32_t width = 2;  // word width of the kernels (launch_width)
 };
 
 static int check_opts(const collatz_verify_opts* o, RangeArgs& a) {
@@ -117,24 +118,24 @@ static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeAr
   DevBuf<EscEntry> desc;
   DevBuf<uint64_t> dovf;
   DevBuf<unsigned long long> dcnt;
-  CZ_HIP(desc.alloc(a.limbs == 2 ? esc_cap : 1));
+  CZ_HIP(desc.alloc(a.width < 3 ? esc_cap : 1));
   CZ_HIP(dovf.alloc(4 * ovf_cap));
   CZ_HIP(dcnt.alloc(3));
   CZ_HIP(hipMemsetAsync(dcnt.p, 0, 24, s));
   DeviceLists lists;
   lists.esc = desc.p;
   lists.esc_count = dcnt.p;
-  lists.esc_cap = a.limbs == 2 ? esc_cap : 0;
+  lists.esc_cap = a.width < 3 ? esc_cap : 0;
   lists.ovf = dovf.p;
   lists.ovf_count = dcnt.p + 1;
   lists.ovf_cap = ovf_cap;
   lists.stall_count = dcnt.p + 2;
-  const Limits lim = make_limits((int)a.limbs, a.cutoff);
+  const Limits lim = make_limits((int)a.width, a.cutoff);
   hipEvent_t e0, e1;
   CZ_HIP(hipEventCreate(&e0));
   CZ_HIP(hipEventCreate(&e1));
   CZ_HIP(hipEventRecord(e0, s));
-  CZ_HIP(launch_verify_plain((int)a.limbs, sv->arrays(), drows.p, (uint32_t)plan.rows.size(),
+  CZ_HIP(launch_verify_plain((int)a.width, sv->arrays(), drows.p, (uint32_t)plan.rows.size(),
                              nblocks, R, sv->mod9, lim, sv->pow3.p, dblock.p, lists, s));
   CZ_HIP(hipEventRecord(e1, s));
   CZ_HIP(launch_reduce_chunks(dblock.p, dpart.p, (uint32_t)plan.nchunks, dchunk.p, s));
@@ -148,10 +149,10 @@ static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeAr
   (void)hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (a.limbs == 2 && cnt[0] > esc_cap) {
-    // More escalations than the list holds: run the whole range on three limbs.
+  if (a.width < 3 && cnt[0] > esc_cap) {
+    // More escalations than the list holds: run the whole range one width up.
     RangeArgs b = a;
-    b.limbs = 3;
+    b.width = a.width + 1;
     return verify_plain(sv, lo, hi, b, s, out);
   }
   uint64_t ovf_n = cnt[1];
@@ -222,5 +223,6 @@ extern "C" int collatz_verify_range(const collatz_sieve* sv, const uint64_t lo[2
     bo.cutoff_log2 = a.cutoff;
     return collatz_verify_batch(sv, 1, &iv, &bo, stream, out);
   }
+  a.width = (uint32_t)launch_width((int)a.limbs, H, a.cutoff);
   return verify_plain(sv, L, H, a, s, out);
 }

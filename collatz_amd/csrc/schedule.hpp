@@ -1,5 +1,6 @@
-// The iteration schedule of the reference on L 32-bit words (L = 4: two u64 limbs,
-// L = 6: three u64 limbs), plus the per-lane accumulators and wave reductions.
+// The iteration schedule of the reference on L 32-bit words (L = 3: 96 bits, for two-limb
+// work that stays below 2^91; L = 4: two u64 limbs; L = 6: three u64 limbs), plus the
+// per-lane accumulators and wave reductions.
 //
 // The schedule follows compute_kernel (collatz_cudav7.cu:280-366):
 //   check = odd run, even run, odd run, even run; then hash ^= low word; stop if number < n
@@ -87,6 +88,8 @@ __device__ __forceinline__ uint64_t pair(uint32_t lo, uint32_t hi) {
 // short-circuit, so every lane takes the same three or five compares).
 template <int L>
 __device__ __forceinline__ bool ge_words(const uint32_t* x, const uint32_t* y) {
+  if constexpr (L == 3)
+    return (x[2] > y[2]) | ((x[2] == y[2]) & (pair(x[0], x[1]) >= pair(y[0], y[1])));
   const uint64_t xh = pair(x[2], x[3]), yh = pair(y[2], y[3]);
   const bool ge = (xh > yh) | ((xh == yh) & (pair(x[0], x[1]) >= pair(y[0], y[1])));
   if constexpr (L == 4) {
@@ -100,7 +103,9 @@ __device__ __forceinline__ bool ge_words(const uint32_t* x, const uint32_t* y) {
 // number < n for n = n_lo + 2^64 n_hi.
 template <int L>
 __device__ __forceinline__ bool lt_base(const Num<L>& v, uint64_t n_lo, uint64_t n_hi) {
-  const uint64_t vh = pair(v.w[2], v.w[3]);
+  uint64_t vh;
+  if constexpr (L == 3) vh = v.w[2];
+  else vh = pair(v.w[2], v.w[3]);
   bool lt = (vh < n_hi) | ((vh == n_hi) & (pair(v.w[0], v.w[1]) < n_lo));
   if constexpr (L > 4) lt = lt & ((v.w[4] | v.w[5]) == 0);
   return lt;
@@ -241,7 +246,8 @@ __device__ __forceinline__ DevRec acc_to_rec(const Acc<L>& a) {
   r.hash = a.hash;
   r.pad = 0;
   r.maxnum[0] = pair(a.mx[0], a.mx[1]);
-  r.maxnum[1] = pair(a.mx[2], a.mx[3]);
+  if constexpr (L == 3) r.maxnum[1] = a.mx[2];
+  else r.maxnum[1] = pair(a.mx[2], a.mx[3]);
   if constexpr (L > 4) r.maxnum[2] = pair(a.mx[4], a.mx[5]);
   else r.maxnum[2] = 0;
   r.max_start[0] = pair(a.mstart[0], a.mstart[1]);

@@ -65,6 +65,7 @@ static void server_loop(collatz_server* S) {
   const uint64_t budget = S->opts.slice_candidates;
   for (;;) {
     LaunchPlan lp;
+    u128 max_hi = 0;  // largest end of the requests in this launch (sets the word width)
     struct SlotRef {
       Request* r;
       uint64_t chunk;
@@ -120,6 +121,7 @@ static void server_loop(collatz_server* S) {
           row.block_start = lp.ntiles;
           lp.ntiles += pieces;
           lp.rows.push_back(row);
+          if (r->hi > max_hi) max_hi = r->hi;
           mine += r->row_est[i] + 1;
           taken += r->row_est[i] + 1;
           r->next_row++;
@@ -137,10 +139,11 @@ static void server_loop(collatz_server* S) {
     LaunchResult res;
     int rc = COLLATZ_OK;
     if (!lp.rows.empty()) {
-      rc = S->engine->run(lp, S->tn, 2, S->opts.cutoff_log2, true, 0, S->stream, res);
-      // More escalations than the list holds: run the same launch on three limbs.
-      if (rc == COLLATZ_ECAPACITY)
-        rc = S->engine->run(lp, S->tn, 3, S->opts.cutoff_log2, true, 0, S->stream, res);
+      int width = launch_width(2, max_hi, S->opts.cutoff_log2);
+      rc = S->engine->run(lp, S->tn, width, S->opts.cutoff_log2, true, 0, S->stream, res);
+      // More escalations than the list holds: the same launch one width up.
+      while (rc == COLLATZ_ECAPACITY && width < 3)
+        rc = S->engine->run(lp, S->tn, ++width, S->opts.cutoff_log2, true, 0, S->stream, res);
     }
     {
       std::lock_guard<std::mutex> lk(S->mu);

@@ -142,7 +142,9 @@ template <int L, bool LOWCUT, class Sink>
 __device__ __forceinline__ bool after_odd(Cand<L>& c, Sink& sink, const Limits& lim,
                                           const DeviceLists& lists, bool escalating) {
   if (!escalating) sink.offer(c.v, c.n);
-  if constexpr (L == 4 && !LOWCUT) {
+  if constexpr (L == 3) {
+    return c.v.w[2] >= (1u << (kNarrowLog2 - 64));
+  } else if constexpr (L == 4 && !LOWCUT) {
     return pair(c.v.w[2], c.v.w[3]) >= lim.fast_thr;
   } else {
     if (over_fast<L, LOWCUT>(c.v, lim.fast_thr) && (L == 4 || ge_pow2<L>(c.v, lim.cutoff))) {
@@ -187,12 +189,14 @@ __device__ __forceinline__ void finish(const Cand<L>& c, Acc<L>& acc, uint32_t k
   acc.ovf += c.ovf;
 }
 
-// Start-of-candidate test: the start value is offered to the max, and on two limbs a start
-// at or above 2^123 goes to the three-limb pass (an odd run multiplies by at most
-// (3/2)^8 < 2^5, so two limbs hold its product only below that).
+// Start-of-candidate test: the start value is offered to the max, and a start at or above
+// the escalation bound of the word count (2^91 on three words, 2^123 on four) goes to the
+// three-limb pass (an odd run multiplies by at most (3/2)^8 < 2^5, so the words hold its
+// product only below that).
 template <int L, class Sink>
 __device__ __forceinline__ bool cand_enter(const Cand<L>& c, Sink& sink) {
   sink.offer(c.v, c.n);
+  if constexpr (L == 3) return c.v.w[2] < (1u << (kNarrowLog2 - 64));
   return !(L == 4 && pair(c.v.w[2], c.v.w[3]) >= (1ull << (kWideLog2 - 64)));
 }
 
@@ -250,7 +254,10 @@ __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowD
     Cand<L> c;
     const uint32_t f = rec_f(sr);
     c.slot = row.slot;
-    const bool big = L == 4 ? (q3[f][4] | q3[f][5]) != 0 : false;
+    uint32_t bigw = 0;
+#pragma unroll
+    for (int i = L; i < 6; ++i) bigw |= q3[f][i];
+    const bool big = bigw != 0;
     if (!cand_start<L>(c, M, q3[f], big, row.qb_lo, row.qb_hi, r) ||
         !run_candidate<L, LOWCUT>(c, acc, lim, lists, sv.k, p3s)) {
       push_escalate(lists, c.n, i, row.slot);
@@ -437,7 +444,10 @@ hipError_t launch_verify_plain(int limbs, const SieveArrays& sv, const RowDesc* 
   if (nblocks == 0) return hipSuccess;
   if (nblocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
   const dim3 grid((uint32_t)nblocks), block(256);
-  if (limbs == 2 && lim.cutoff >= kWideLog2)
+  if (limbs == 1)
+    hipLaunchKernelGGL((k_verify_plain<3, false>), grid, block, 0, s, sv, rows, nrows, R, mod9,
+                       lim, pow3, block_recs, lists);
+  else if (limbs == 2 && lim.cutoff >= kWideLog2)
     hipLaunchKernelGGL((k_verify_plain<4, false>), grid, block, 0, s, sv, rows, nrows, R, mod9,
                        lim, pow3, block_recs, lists);
   else if (limbs == 2)
@@ -458,7 +468,10 @@ hipError_t launch_verify_persistent(int limbs, const SieveArrays& sv, const RowD
                                     uint32_t U, const Limits& lim, const uint64_t* pow3,
                                     DevRec* tile_recs, DeviceLists lists, hipStream_t s) {
   const dim3 grid(nblocks), block(256);
-  if (limbs == 2 && lim.cutoff >= kWideLog2)
+  if (limbs == 1)
+    hipLaunchKernelGGL((k_verify_persistent<3, false>), grid, block, 0, s, sv, rows, tiles, ctl,
+                       lim, pow3, tile_recs, lists, U);
+  else if (limbs == 2 && lim.cutoff >= kWideLog2)
     hipLaunchKernelGGL((k_verify_persistent<4, false>), grid, block, 0, s, sv, rows, tiles, ctl,
                        lim, pow3, tile_recs, lists, U);
   else if (limbs == 2)

@@ -106,6 +106,25 @@ def test_three_limbs_and_escalation(s20, o20, oracle):
     assert int(total["flags"]) & 2
 
 
+def test_narrow_width_escalation(s20, o20, oracle):
+    """Two-limb ranges below 2^68 run on 96 bits and escalate at 2^91.  n = 0x8b3aed82f55cb4c9b
+    (< 2^68) climbs to a 92-bit excursion, so its candidate is escalated and rerun on three
+    limbs; the records must equal the oracle's (which runs everything on wide integers)."""
+    n = 0x8B3AED82F55CB4C9B
+    outs, _ = check(s20, o20, oracle, n - (1 << 12), n + (1 << 12), C=12)
+    assert outs[0].escalated >= 1 and outs[1].escalated >= 1
+    assert int(outs[0].total["max_exc"][1]) >> 27 != 0
+
+
+@pytest.mark.parametrize("lo,hi", [
+    ((1 << 68) - (1 << 22), 1 << 68),             # last range on 96 bits
+    ((1 << 68) - (1 << 21), (1 << 68) + (1 << 21)),  # first above: two u64 limbs
+    ((1 << 90) - (1 << 20), (1 << 90) + 5),       # starts near the 96-bit escalation bound
+])
+def test_width_boundaries(s20, o20, oracle, lo, hi):
+    check(s20, o20, oracle, lo, hi, C=20)
+
+
 def test_limbs3_low_cutoff(s20, o20, oracle):
     check(s20, o20, oracle, (1 << 62) + 5, (1 << 62) + (1 << 20), 20, limbs=3, cutoff=100)
     check(s20, o20, oracle, (1 << 62) + 5, (1 << 62) + (1 << 20), 20, limbs=3, cutoff=125)
