@@ -143,8 +143,10 @@ struct Acc {
   uint32_t cnt, hash, ssum, sxor, ovf, chk;
   uint32_t mx[L];      // max number seen (T-value + 1), 0 if none
   uint32_t mstart[4];  // its start
+  uint64_t mxh;        // L = 3: words 1 and 2 of mx as one register pair, for the filter
   __device__ void init() {
     cnt = hash = ssum = sxor = ovf = chk = 0;
+    mxh = 0;
 #pragma unroll
     for (int i = 0; i < L; ++i) mx[i] = 0;
 #pragma unroll
@@ -154,18 +156,38 @@ struct Acc {
 
 // Offer (value, start) to the lane max: take it if larger, or equal with a smaller start.
 template <int L>
+__device__ __forceinline__ void offer_take(Acc<L>& acc, const Num<L>& v, const uint32_t* n) {
+  bool eq = true;
+#pragma unroll
+  for (int i = 0; i < L; ++i) eq = eq && v.w[i] == acc.mx[i];
+  const bool take = !eq || !ge_words<4>(n, acc.mstart);
+  if (take) {
+    uint32_t w[L];
+#pragma unroll
+    for (int i = 0; i < L; ++i) w[i] = v.w[i];
+    // An element-wise copy of three words would be merged into a straddling 64-bit copy
+    // that keeps the value's array out of registers; the empty asm keeps the copies apart.
+    if constexpr (L == 3) asm("" : "+v"(w[0]), "+v"(w[1]), "+v"(w[2]));
+#pragma unroll
+    for (int i = 0; i < L; ++i) acc.mx[i] = w[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc.mstart[i] = n[i];
+    if constexpr (L == 3) acc.mxh = ((uint64_t)w[2] << 32) | w[1];
+  }
+}
+
+template <int L>
 __device__ __forceinline__ void offer_max(Acc<L>& acc, const Num<L>& v, const uint32_t* n) {
-  if (ge_words<L>(v.w, acc.mx)) {
-    bool eq = true;
-#pragma unroll
-    for (int i = 0; i < L; ++i) eq = eq && v.w[i] == acc.mx[i];
-    const bool take = !eq || !ge_words<4>(n, acc.mstart);
-    if (take) {
-#pragma unroll
-      for (int i = 0; i < L; ++i) acc.mx[i] = v.w[i];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) acc.mstart[i] = n[i];
+  if constexpr (L == 3) {
+    // v >= max needs (v >> 32) >= (max >> 32): one 64-bit compare that a lane's running max
+    // fails almost always; the exact test runs only behind it.
+    const uint64_t vh = ((uint64_t)v.w[2] << 32) | v.w[1];
+    const uint64_t mh = acc.mxh;
+    if (__builtin_expect(vh >= mh, 0)) {
+      if ((vh > mh) | (v.w[0] >= acc.mx[0])) offer_take<L>(acc, v, n);
     }
+  } else {
+    if (ge_words<L>(v.w, acc.mx)) offer_take<L>(acc, v, n);
   }
 }
 
