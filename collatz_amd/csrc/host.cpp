@@ -90,6 +90,7 @@ void chunk_merge(collatz_chunk* a, const collatz_chunk& b) {
 
 int launch_width(int limbs, u128 max_hi, uint32_t cutoff) {
   if (limbs == 2 && cutoff >= kNarrowLog2 && max_hi <= ((u128)1 << kNarrowMaxStartLog2)) return 1;
+  if (limbs == 2 && cutoff >= kWideLog2 && max_hi > ((u128)1 << kWideMaxStartLog2)) return 3;
   return limbs;
 }
 

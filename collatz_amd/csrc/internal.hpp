@@ -100,11 +100,17 @@ struct DeviceLists {
 // computed are a few thousand T-steps; a candidate still running after this many checks
 // means a defect, and is counted instead of looping forever.
 // Two-limb work whose starts stay below 2^68 runs on three 32-bit words (96 bits) and
-// escalates at 2^91 (an odd run multiplies by less than 2^5).  Survivors climb by 2^x above
-// their start with probability about 2^-(x-5), so below 2^68 fewer than one candidate in
-// 10^6 escalates.
+// escalates to three limbs above 2^86 (see the per-check bounds below).  Survivors climb by
+// 2^x above their start with probability about 2^-(x-5), so below 2^68 about one candidate
+// in 10^6 escalates.
 constexpr uint32_t kNarrowLog2 = 91;
 constexpr uint32_t kNarrowMaxStartLog2 = 68;
+// Per-check escalation bounds (a check multiplies by less than 2^9.4): 96 - 10 and 128 - 10.
+constexpr uint32_t kNarrowCheckLog2 = 86;
+constexpr uint32_t kWideCheckLog2 = 118;
+// Two-limb work whose starts are above 2^100 runs on three limbs from the start (at the
+// four-word check bound of 2^118 too many of its candidates would escalate).
+constexpr uint32_t kWideMaxStartLog2 = 100;
 constexpr uint32_t kMaxChecks = 1u << 16;
 // The same bound in T-steps for the persistent kernel: every half-check (odd run, even run)
 // takes at least one step, so 2^18 steps mean at most 2^17 checks.

@@ -133,20 +133,31 @@ __device__ __forceinline__ void push_escalate(const DeviceLists& lists, const ui
   }
 }
 
-// After an odd run: offer to the max, then the cutoff / limb limit.  Returns true if the
-// candidate has to be escalated (two limbs only).  On two limbs LOWCUT means a cutoff below
-// the two-limb bound 2^123 (the top-pair compare is then exact and means overflow);
-// without it the compare means only "escalate", and nothing else changes the state.  On
-// three limbs LOWCUT means a cutoff below 2^128.
+// Word widths that escalate (three 32-bit words; two u64 limbs with a cutoff at or above
+// 2^123) test the bound once per check, at its end: a check multiplies by at most
+// ((3/2)^8)^2 < 2^9.4 (two odd runs; even runs only decrease), so a check that starts
+// below 2^86 (three words) or 2^118 (four words) never leaves the words, every value it
+// offers is exact, and a candidate that ends a check at or above the bound goes to the
+// three-limb pass, which reruns it from its start.
+template <int L, bool LOWCUT>
+struct Escalates {
+  static constexpr bool value = L == 3 || (L == 4 && !LOWCUT);
+};
+
+template <int L>
+__device__ __forceinline__ bool over_check_bound(const Num<L>& v) {
+  if constexpr (L == 3) return v.w[2] >= (1u << (kNarrowCheckLog2 - 64));
+  else return pair(v.w[2], v.w[3]) >= (1ull << (kWideCheckLog2 - 64));
+}
+
+// After an odd run: offer to the max; where the width overflows instead of escalating,
+// the cutoff test (on four words LOWCUT means a cutoff below 2^123, where the top-pair
+// compare is exact; on six words a cutoff below 2^128).
 template <int L, bool LOWCUT, class Sink>
-__device__ __forceinline__ bool after_odd(Cand<L>& c, Sink& sink, const Limits& lim,
-                                          const DeviceLists& lists, bool escalating) {
-  if (!escalating) sink.offer(c.v, c.n);
-  if constexpr (L == 3) {
-    return c.v.w[2] >= (1u << (kNarrowLog2 - 64));
-  } else if constexpr (L == 4 && !LOWCUT) {
-    return pair(c.v.w[2], c.v.w[3]) >= lim.fast_thr;
-  } else {
+__device__ __forceinline__ void after_odd(Cand<L>& c, Sink& sink, const Limits& lim,
+                                          const DeviceLists& lists) {
+  sink.offer(c.v, c.n);
+  if constexpr (!Escalates<L, LOWCUT>::value) {
     if (over_fast<L, LOWCUT>(c.v, lim.fast_thr) && (L == 4 || ge_pow2<L>(c.v, lim.cutoff))) {
       // Overflow: report the start and continue from 2 (collatz_cudav7.cu:295-301).
       c.ovf += 1;
@@ -155,27 +166,26 @@ __device__ __forceinline__ bool after_odd(Cand<L>& c, Sink& sink, const Limits& 
 #pragma unroll
       for (int i = 1; i < L; ++i) c.v.w[i] = 0;
     }
-    return false;
   }
 }
 
-// One check.  `done`: number < n after it.  `esc` (two limbs only): a value reached 2^123
-// and the candidate goes to the three-limb pass; the check still runs to its end, on a
-// value that may no longer fit, so its later offers are skipped and esc wins over done.
-// Flags instead of an early return keep the common path free of branches.
+// One check.  `done`: number < n after it.  `esc`: the check ended at or above the
+// width's bound, so the candidate goes to the three-limb pass (never together with done:
+// the widths that escalate are used only for starts far below the bound).
 template <int L, bool LOWCUT, class Sink>
 __device__ __forceinline__ void one_check(Cand<L>& c, Sink& sink, const Limits& lim,
                                           const DeviceLists& lists, const Pow3Lds& p3,
                                           bool& esc, bool& done) {
   const uint32_t a1 = odd_run<L>(c.v, p3);
-  bool e = after_odd<L, LOWCUT>(c, sink, lim, lists, false);
+  after_odd<L, LOWCUT>(c, sink, lim, lists);
   const uint32_t b1 = even_run<L>(c.v);
   const uint32_t a2 = odd_run<L>(c.v, p3);
-  e = after_odd<L, LOWCUT>(c, sink, lim, lists, e) || e;
+  after_odd<L, LOWCUT>(c, sink, lim, lists);
   const uint32_t b2 = even_run<L>(c.v);
   c.steps += a1 + b1 + a2 + b2;
   c.hash ^= c.v.w[0];
-  esc = e;
+  if constexpr (Escalates<L, LOWCUT>::value) esc = over_check_bound<L>(c.v);
+  else esc = false;
   done = lt_base<L>(c.v, pair(c.n[0], c.n[1]), pair(c.n[2], c.n[3]));
 }
 
@@ -190,14 +200,12 @@ __device__ __forceinline__ void finish(const Cand<L>& c, Acc<L>& acc, uint32_t k
 }
 
 // Start-of-candidate test: the start value is offered to the max, and a start at or above
-// the escalation bound of the word count (2^91 on three words, 2^123 on four) goes to the
-// three-limb pass (an odd run multiplies by at most (3/2)^8 < 2^5, so the words hold its
-// product only below that).
-template <int L, class Sink>
+// the width's per-check bound goes to the three-limb pass.
+template <int L, bool LOWCUT, class Sink>
 __device__ __forceinline__ bool cand_enter(const Cand<L>& c, Sink& sink) {
   sink.offer(c.v, c.n);
-  if constexpr (L == 3) return c.v.w[2] < (1u << (kNarrowLog2 - 64));
-  return !(L == 4 && pair(c.v.w[2], c.v.w[3]) >= (1ull << (kWideLog2 - 64)));
+  if constexpr (Escalates<L, LOWCUT>::value) return !over_check_bound<L>(c.v);
+  return true;
 }
 
 // Runs candidate c to completion.  Returns false if it was escalated.
@@ -206,7 +214,7 @@ __device__ __forceinline__ bool run_candidate(Cand<L>& c, Acc<L>& acc, const Lim
                                               const DeviceLists& lists, uint32_t k,
                                               const Pow3Lds& p3) {
   SinkOne<L> sink{acc};
-  if (!cand_enter<L>(c, sink)) return false;
+  if (!cand_enter<L, LOWCUT>(c, sink)) return false;
   for (uint32_t guard = 0; guard < kMaxChecks; ++guard) {
     bool esc, done;
     one_check<L, LOWCUT>(c, sink, lim, lists, p3, esc, done);
@@ -380,7 +388,7 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
                                         rec_r(pf_rec));
         c.slot = slot;
         cidx = pf_idx;
-        if (fits && cand_enter<L>(c, sink)) act = true;
+        if (fits && cand_enter<L, LOWCUT>(c, sink)) act = true;
         else push_escalate(lists, c.n, cidx, slot);
       }
       // (2) Lanes without a prefetch take the next survivors of the tile.

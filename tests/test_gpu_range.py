@@ -95,21 +95,30 @@ def test_cutoffs_and_overflow_lists(s20, o20, oracle, cutoff):
 
 
 def test_three_limbs_and_escalation(s20, o20, oracle):
-    """Starts near 2^110 reach 2^123: two limbs escalate to three; three limbs from the
-    start give the same records."""
+    """Starts near 2^110 reach 2^123: two-limb requests that far up run on three limbs from
+    the start, and give the same records as a three-limb request."""
     lo = (1 << 110) + 12345
     hi = lo + (1 << 20)
     outs2, total = check(s20, o20, oracle, lo, hi, 18, limbs=2)
     outs3, _ = check(s20, o20, oracle, lo, hi, 18, limbs=3)
-    assert outs2[0].escalated > 0
     assert outs3[0].escalated == 0
     assert int(total["flags"]) & 2
 
 
+def test_two_limb_escalation(s20, o20, oracle):
+    """n = 0x7737a959485e163f092f585bf (< 2^100, so two u64 limbs) climbs to a 127-bit
+    excursion: it ends a check above 2^118, is escalated and rerun on three limbs."""
+    n = 0x7737A959485E163F092F585BF
+    outs, total = check(s20, o20, oracle, n - (1 << 10), n + (1 << 10), C=10)
+    assert outs[0].escalated >= 1 and outs[1].escalated >= 1
+    assert int(total["flags"]) & 2
+
+
 def test_narrow_width_escalation(s20, o20, oracle):
-    """Two-limb ranges below 2^68 run on 96 bits and escalate at 2^91.  n = 0x8b3aed82f55cb4c9b
-    (< 2^68) climbs to a 92-bit excursion, so its candidate is escalated and rerun on three
-    limbs; the records must equal the oracle's (which runs everything on wide integers)."""
+    """Two-limb ranges below 2^68 run on 96 bits and escalate above 2^86.  n =
+    0x8b3aed82f55cb4c9b (< 2^68) climbs to a 92-bit excursion, so its candidate is escalated
+    and rerun on three limbs; the records must equal the oracle's (which runs everything on
+    wide integers)."""
     n = 0x8B3AED82F55CB4C9B
     outs, _ = check(s20, o20, oracle, n - (1 << 12), n + (1 << 12), C=12)
     assert outs[0].escalated >= 1 and outs[1].escalated >= 1
@@ -119,7 +128,9 @@ def test_narrow_width_escalation(s20, o20, oracle):
 @pytest.mark.parametrize("lo,hi", [
     ((1 << 68) - (1 << 22), 1 << 68),             # last range on 96 bits
     ((1 << 68) - (1 << 21), (1 << 68) + (1 << 21)),  # first above: two u64 limbs
-    ((1 << 90) - (1 << 20), (1 << 90) + 5),       # starts near the 96-bit escalation bound
+    ((1 << 86) - (1 << 20), (1 << 86) + 5),       # starts at the 96-bit check bound (four words)
+    ((1 << 100) - (1 << 20), (1 << 100)),         # last range on four words
+    ((1 << 100) - (1 << 20), (1 << 100) + 1),     # first one on three limbs
 ])
 def test_width_boundaries(s20, o20, oracle, lo, hi):
     check(s20, o20, oracle, lo, hi, C=20)
