@@ -142,25 +142,26 @@ template <int L>
 struct Acc {
   uint32_t cnt, hash, ssum, sxor, ovf, chk;
   uint32_t mx[L];      // max number seen (T-value + 1), 0 if none
-  uint32_t mstart[4];  // its start
+  uint64_t ms_lo, ms_hi;  // its start
   uint64_t mxh;        // L = 3: words 1 and 2 of mx as one register pair, for the filter
   __device__ void init() {
     cnt = hash = ssum = sxor = ovf = chk = 0;
     mxh = 0;
 #pragma unroll
     for (int i = 0; i < L; ++i) mx[i] = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) mstart[i] = 0;
+    ms_lo = ms_hi = 0;
   }
 };
 
 // Offer (value, start) to the lane max: take it if larger, or equal with a smaller start.
 template <int L>
-__device__ __forceinline__ void offer_take(Acc<L>& acc, const Num<L>& v, const uint32_t* n) {
+__device__ __forceinline__ void offer_take(Acc<L>& acc, const Num<L>& v, uint64_t n_lo,
+                                           uint64_t n_hi) {
   bool eq = true;
 #pragma unroll
   for (int i = 0; i < L; ++i) eq = eq && v.w[i] == acc.mx[i];
-  const bool take = !eq || !ge_words<4>(n, acc.mstart);
+  const bool smaller_start = (n_hi < acc.ms_hi) | ((n_hi == acc.ms_hi) & (n_lo < acc.ms_lo));
+  const bool take = !eq || smaller_start;
   if (take) {
     uint32_t w[L];
 #pragma unroll
@@ -170,24 +171,25 @@ __device__ __forceinline__ void offer_take(Acc<L>& acc, const Num<L>& v, const u
     if constexpr (L == 3) asm("" : "+v"(w[0]), "+v"(w[1]), "+v"(w[2]));
 #pragma unroll
     for (int i = 0; i < L; ++i) acc.mx[i] = w[i];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc.mstart[i] = n[i];
+    acc.ms_lo = n_lo;
+    acc.ms_hi = n_hi;
     if constexpr (L == 3) acc.mxh = ((uint64_t)w[2] << 32) | w[1];
   }
 }
 
 template <int L>
-__device__ __forceinline__ void offer_max(Acc<L>& acc, const Num<L>& v, const uint32_t* n) {
+__device__ __forceinline__ void offer_max(Acc<L>& acc, const Num<L>& v, uint64_t n_lo,
+                                          uint64_t n_hi) {
   if constexpr (L == 3) {
     // v >= max needs (v >> 32) >= (max >> 32): one 64-bit compare that a lane's running max
     // fails almost always; the exact test runs only behind it.
     const uint64_t vh = ((uint64_t)v.w[2] << 32) | v.w[1];
     const uint64_t mh = acc.mxh;
     if (__builtin_expect(vh >= mh, 0)) {
-      if ((vh > mh) | (v.w[0] >= acc.mx[0])) offer_take<L>(acc, v, n);
+      if ((vh > mh) | (v.w[0] >= acc.mx[0])) offer_take<L>(acc, v, n_lo, n_hi);
     }
   } else {
-    if (ge_words<L>(v.w, acc.mx)) offer_take<L>(acc, v, n);
+    if (ge_words<L>(v.w, acc.mx)) offer_take<L>(acc, v, n_lo, n_hi);
   }
 }
 
@@ -272,8 +274,8 @@ __device__ __forceinline__ DevRec acc_to_rec(const Acc<L>& a) {
   else r.maxnum[1] = pair(a.mx[2], a.mx[3]);
   if constexpr (L > 4) r.maxnum[2] = pair(a.mx[4], a.mx[5]);
   else r.maxnum[2] = 0;
-  r.max_start[0] = pair(a.mstart[0], a.mstart[1]);
-  r.max_start[1] = pair(a.mstart[2], a.mstart[3]);
+  r.max_start[0] = a.ms_lo;
+  r.max_start[1] = a.ms_hi;
   if (a.cnt == 0) {
     r.maxnum[0] = r.maxnum[1] = r.maxnum[2] = 0;
     r.max_start[0] = r.max_start[1] = 0;

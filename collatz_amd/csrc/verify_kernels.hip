@@ -27,7 +27,7 @@ __device__ __forceinline__ void q_times_pow3(uint64_t q_lo, uint64_t q_hi, uint6
 template <int L>
 struct Cand {
   Num<L> v;
-  uint32_t n[4];
+  uint64_t n_lo, n_hi;  // the start n
   uint32_t hash;
   uint32_t steps;
   uint32_t ovf;
@@ -38,45 +38,8 @@ struct Cand {
 template <int L>
 struct SinkOne {
   Acc<L>& acc;
-  __device__ __forceinline__ void offer(const Num<L>& v, const uint32_t* n) {
-    offer_max<L>(acc, v, n);
-  }
-};
-
-// Two accumulator sets (two tiles in flight per wave); cm mirrors the max of the set the
-// lane's candidate belongs to, so the common case costs one compare.
-template <int L>
-struct SinkTwo {
-  Acc<L>& a0;
-  Acc<L>& a1;
-  uint32_t cset;
-  uint32_t* cm;
-  __device__ __forceinline__ void offer(const Num<L>& v, const uint32_t* n) {
-    if (ge_words<L>(v.w, cm)) {
-      // Value selects (a pointer select between the two sets would put them in scratch).
-      const uint32_t m1 = 0u - cset, m0 = ~m1;
-      uint32_t ms[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) ms[i] = (a1.mstart[i] & m1) | (a0.mstart[i] & m0);
-      bool eq = true;
-#pragma unroll
-      for (int i = 0; i < L; ++i) eq = eq && v.w[i] == cm[i];
-      if (!eq || !ge_words<4>(n, ms)) {
-#pragma unroll
-        for (int i = 0; i < L; ++i) cm[i] = v.w[i];
-        if (cset) {
-#pragma unroll
-          for (int i = 0; i < L; ++i) a1.mx[i] = v.w[i];
-#pragma unroll
-          for (int i = 0; i < 4; ++i) a1.mstart[i] = n[i];
-        } else {
-#pragma unroll
-          for (int i = 0; i < L; ++i) a0.mx[i] = v.w[i];
-#pragma unroll
-          for (int i = 0; i < 4; ++i) a0.mstart[i] = n[i];
-        }
-      }
-    }
+  __device__ __forceinline__ void offer(const Num<L>& v, uint64_t n_lo, uint64_t n_hi) {
+    offer_max<L>(acc, v, n_lo, n_hi);
   }
 };
 
@@ -96,11 +59,8 @@ __device__ __forceinline__ bool cand_start(Cand<L>& c, uint64_t M, const uint32_
 #pragma unroll
   for (int i = 2; i < L; ++i) c.v.w[i] = addc(q3[i], 0u, cy, cy);
   const bool fits = cy == 0 && !q3_big;
-  const uint64_t nlo = qb_lo | r;
-  c.n[0] = (uint32_t)nlo;
-  c.n[1] = (uint32_t)(nlo >> 32);
-  c.n[2] = (uint32_t)qb_hi;
-  c.n[3] = (uint32_t)(qb_hi >> 32);
+  c.n_lo = qb_lo | r;
+  c.n_hi = qb_hi;
   c.hash = 0;
   c.steps = 0;
   c.ovf = 0;
@@ -108,24 +68,24 @@ __device__ __forceinline__ bool cand_start(Cand<L>& c, uint64_t M, const uint32_
 }
 
 // Overflow reports: (n_lo, n_hi, slot, 0) per event, in the order the events happen.
-__device__ __forceinline__ void push_overflow(const DeviceLists& lists, const uint32_t* n,
-                                              uint32_t slot) {
+__device__ __forceinline__ void push_overflow(const DeviceLists& lists, uint64_t n_lo,
+                                              uint64_t n_hi, uint32_t slot) {
   const unsigned long long i = atomicAdd(lists.ovf_count, 1ull);
   if (i < lists.ovf_cap) {
-    lists.ovf[4 * i] = pair(n[0], n[1]);
-    lists.ovf[4 * i + 1] = pair(n[2], n[3]);
+    lists.ovf[4 * i] = n_lo;
+    lists.ovf[4 * i + 1] = n_hi;
     lists.ovf[4 * i + 2] = slot;
     lists.ovf[4 * i + 3] = 0;
   }
 }
 
-__device__ __forceinline__ void push_escalate(const DeviceLists& lists, const uint32_t* n,
-                                              uint64_t res_index, uint32_t slot) {
+__device__ __forceinline__ void push_escalate(const DeviceLists& lists, uint64_t n_lo,
+                                              uint64_t n_hi, uint64_t res_index, uint32_t slot) {
   const unsigned long long i = atomicAdd(lists.esc_count, 1ull);
   if (i < lists.esc_cap) {
     EscEntry e;
-    e.n_lo = pair(n[0], n[1]);
-    e.n_hi = pair(n[2], n[3]);
+    e.n_lo = n_lo;
+    e.n_hi = n_hi;
     e.res_index = res_index;
     e.slot = slot;
     e.pad = 0;
@@ -156,12 +116,12 @@ __device__ __forceinline__ bool over_check_bound(const Num<L>& v) {
 template <int L, bool LOWCUT, class Sink>
 __device__ __forceinline__ void after_odd(Cand<L>& c, Sink& sink, const Limits& lim,
                                           const DeviceLists& lists) {
-  sink.offer(c.v, c.n);
+  sink.offer(c.v, c.n_lo, c.n_hi);
   if constexpr (!Escalates<L, LOWCUT>::value) {
     if (over_fast<L, LOWCUT>(c.v, lim.fast_thr) && (L == 4 || ge_pow2<L>(c.v, lim.cutoff))) {
       // Overflow: report the start and continue from 2 (collatz_cudav7.cu:295-301).
       c.ovf += 1;
-      push_overflow(lists, c.n, c.slot);
+      push_overflow(lists, c.n_lo, c.n_hi, c.slot);
       c.v.w[0] = 2;
 #pragma unroll
       for (int i = 1; i < L; ++i) c.v.w[i] = 0;
@@ -186,7 +146,7 @@ __device__ __forceinline__ void one_check(Cand<L>& c, Sink& sink, const Limits& 
   c.hash ^= c.v.w[0];
   if constexpr (Escalates<L, LOWCUT>::value) esc = over_check_bound<L>(c.v);
   else esc = false;
-  done = lt_base<L>(c.v, pair(c.n[0], c.n[1]), pair(c.n[2], c.n[3]));
+  done = lt_base<L>(c.v, c.n_lo, c.n_hi);
 }
 
 template <int L>
@@ -203,7 +163,7 @@ __device__ __forceinline__ void finish(const Cand<L>& c, Acc<L>& acc, uint32_t k
 // the width's per-check bound goes to the three-limb pass.
 template <int L, bool LOWCUT, class Sink>
 __device__ __forceinline__ bool cand_enter(const Cand<L>& c, Sink& sink) {
-  sink.offer(c.v, c.n);
+  sink.offer(c.v, c.n_lo, c.n_hi);
   if constexpr (Escalates<L, LOWCUT>::value) return !over_check_bound<L>(c.v);
   return true;
 }
@@ -268,7 +228,7 @@ __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowD
     const bool big = bigw != 0;
     if (!cand_start<L>(c, M, q3[f], big, row.qb_lo, row.qb_hi, r) ||
         !run_candidate<L, LOWCUT>(c, acc, lim, lists, sv.k, p3s)) {
-      push_escalate(lists, c.n, i, row.slot);
+      push_escalate(lists, c.n_lo, c.n_hi, i, row.slot);
     }
   }
   DevRec rec = block_merge<4>(acc_to_rec<L>(acc), lrec);
@@ -373,12 +333,14 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
     acc.init();
     SinkOne<L> sink{acc};
     Cand<L> c;
+    c.n_hi = qb_hi;  // the same for every candidate of the tile
     bool act = false;
-    uint64_t cidx = 0;
+    uint32_t cidx = 0;  // offset of the running candidate in the tile
     // Prefetch slot.
     bool pf = false;
     uint4 pf_rec = make_uint4(0, 0, 0, 0);
-    uint64_t pf_idx = 0;
+    uint32_t pf_off = 0;
+    const uint4* tile_rec = sv.rec + first;
     for (;;) {
       // (1) Lanes without a candidate start their prefetched one.
       if (!act && pf) {
@@ -387,9 +349,9 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
         const bool fits = cand_start<L>(c, rec_M(pf_rec), T.q3[f], false, qb_lo, qb_hi,
                                         rec_r(pf_rec));
         c.slot = slot;
-        cidx = pf_idx;
+        cidx = pf_off;
         if (fits && cand_enter<L, LOWCUT>(c, sink)) act = true;
-        else push_escalate(lists, c.n, cidx, slot);
+        else push_escalate(lists, c.n_lo, c.n_hi, first + cidx, slot);
       }
       // (2) Lanes without a prefetch take the next survivors of the tile.
       const uint64_t need = __ballot(!pf);
@@ -400,9 +362,8 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
         const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
             (uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
         if (!pf && rank < take) {
-          const uint64_t i = first + cur + rank;
-          pf_rec = sv.rec[i];
-          pf_idx = i;
+          pf_off = cur + rank;
+          pf_rec = tile_rec[pf_off];
           pf = true;
         }
         cur += take;
@@ -415,7 +376,13 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
           one_check<L, LOWCUT>(c, sink, lim, lists, p3s, esc, done);
           acc.chk += 1;
           if (esc || done || c.steps >= kMaxSteps) {
-            if (esc) push_escalate(lists, c.n, cidx, slot);
+            if (esc) {
+              // (The empty asm keeps the index sum in this rare branch: hoisted, it would
+              // cost two instructions on every refill.)
+              uint32_t off = cidx;
+              asm volatile("" : "+v"(off));
+              push_escalate(lists, c.n_lo, c.n_hi, first + off, slot);
+            }
             else if (done) finish<L>(c, acc, sv.k);
             else atomicAdd(lists.stall_count, 1ull);
             act = false;
