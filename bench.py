@@ -18,8 +18,8 @@ Beside it, on the same GPU and sieve:
     through collatz_server_*, with p50/p99 latency, and once all at once (saturated);
   * cpu_baseline: the CPU restatement (oracle) on a bounded sample of the same workload.
 
-Every call returns after its stream has synchronised, so the host clock around a call
-brackets the device work.  Prints one JSON line on stdout; progress goes to stderr.
+Every call returns after its stream has synchronised; the timed region is also bracketed
+by hipDeviceSynchronize.  Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -67,6 +67,26 @@ WRITE_BYTES_PER_LAUNCH = 0.155e9
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+_hip = None
+
+
+def device_sync():
+    """hipDeviceSynchronize from the HIP runtime the library is linked against (the timed
+    region is bracketed by it; the library's own calls also return after their streams have
+    synchronised)."""
+    global _hip
+    if _hip is None:
+        import ctypes
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                _hip = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+    if _hip is not None and _hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
 
 
 def records_equal(a, b) -> bool:
@@ -187,11 +207,13 @@ def main():
         batch_step()
     log(f"batched warmup done ({args.warmup} steps)")
     dev_ms = []
+    device_sync()
     t0 = time.perf_counter()
     outs = None
     for _ in range(args.steps):
         outs = batch_step()
         dev_ms.append(outs[0].device_ms)
+    device_sync()
     dt = time.perf_counter() - t0
     log(f"batched: {args.steps} steps in {dt:.2f} s")
     ms_step = dt / args.steps * 1e3

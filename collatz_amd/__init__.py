@@ -66,6 +66,14 @@ class RangeResult(C.Structure):
                 ("escalated", C.c_uint64), ("checks", C.c_uint64), ("device_ms", C.c_double)]
 
 
+RESULT_DTYPE = np.dtype([
+    ("total", CHUNK_DTYPE), ("chunks", "<u8"), ("chunks_cap", "<u8"), ("chunks_written", "<u8"),
+    ("overflow_starts", "<u8"), ("overflow_cap", "<u8"), ("overflow_written", "<u8"),
+    ("escalated", "<u8"), ("checks", "<u8"), ("device_ms", "<f8"),
+])
+assert RESULT_DTYPE.itemsize == C.sizeof(RangeResult)
+
+
 class VerifyOpts(C.Structure):
     _fields_ = [("chunk_log2", C.c_uint32), ("limbs", C.c_uint32), ("cutoff_log2", C.c_uint32),
                 ("form", C.c_uint32)]
@@ -245,37 +253,43 @@ class Sieve:
                      cutoff_log2: int = 187, use_graph: bool = False, tile_residues: int = 0,
                      waves_per_cu: int = 0, checks_per_refill: int = 0, overflow_cap: int = 0,
                      stream=None):
+        """collatz_verify_batch over [(lo, hi), ...]; one RangeOutput per interval.  The
+        chunk records of all intervals share one array (each output holds a view)."""
         m = len(intervals)
-        ivs = (Interval * m)()
-        outs = (RangeResult * m)()
-        chunk_arrays, ovf_arrays = [], []
-        for i, (lo, hi) in enumerate(intervals):
-            ivs[i].lo[0], ivs[i].lo[1] = lo & MASK64, lo >> 64
-            ivs[i].hi[0], ivs[i].hi[1] = hi & MASK64, hi >> 64
-            n = ((hi - 1) >> chunk_log2) - (lo >> chunk_log2) + 1
-            arr = np.zeros(n, CHUNK_DTYPE)
-            chunk_arrays.append(arr)
-            outs[i].chunks = arr.ctypes.data_as(C.POINTER(Chunk))
-            outs[i].chunks_cap = n
-            if overflow_cap:
-                ov = np.zeros(2 * overflow_cap, np.uint64)
-                ovf_arrays.append(ov)
-                outs[i].overflow_starts = ov.ctypes.data_as(C.POINTER(C.c_uint64))
-                outs[i].overflow_cap = overflow_cap
+        iv = np.array([(lo & MASK64, lo >> 64, hi & MASK64, hi >> 64) for lo, hi in intervals],
+                      dtype=np.uint64).reshape(m, 4)
+        nch = np.array([((hi - 1) >> chunk_log2) - (lo >> chunk_log2) + 1
+                        for lo, hi in intervals], dtype=np.int64)
+        offs = np.zeros(m + 1, np.int64)
+        np.cumsum(nch, out=offs[1:])
+        chunks = np.zeros(int(offs[-1]), CHUNK_DTYPE)
+        res = np.zeros(m, RESULT_DTYPE)
+        res["chunks"] = chunks.ctypes.data + offs[:-1].astype(np.uint64) * CHUNK_DTYPE.itemsize
+        res["chunks_cap"] = nch
+        ovf = None
+        if overflow_cap:
+            ovf = np.zeros((m, 2 * overflow_cap), np.uint64)
+            res["overflow_starts"] = ovf.ctypes.data + np.arange(m, dtype=np.uint64) * (
+                16 * overflow_cap)
+            res["overflow_cap"] = overflow_cap
         opts = BatchOpts(chunk_log2, limbs, cutoff_log2, int(use_graph), tile_residues,
                          waves_per_cu, checks_per_refill, 0)
-        _check(load().collatz_verify_batch(self._h, m, ivs, C.byref(opts), stream, outs))
-        result = []
-        for i in range(m):
-            total = np.frombuffer(bytearray(outs[i].total), CHUNK_DTYPE)[0]
-            starts = []
-            if overflow_cap:
-                ov = ovf_arrays[i]
-                nov = min(outs[i].overflow_written, overflow_cap)
-                starts = [int(ov[2 * j]) | (int(ov[2 * j + 1]) << 64) for j in range(nov)]
-            result.append(RangeOutput(total, chunk_arrays[i], starts, outs[i].escalated,
-                                      outs[i].device_ms, outs[i].checks))
-        return result
+        _check(load().collatz_verify_batch(self._h, m, iv.ctypes.data_as(C.POINTER(Interval)),
+                                           C.byref(opts), stream,
+                                           res.ctypes.data_as(C.POINTER(RangeResult))))
+        totals = res["total"]
+        esc = res["escalated"].tolist()
+        dms = res["device_ms"].tolist()
+        chk = res["checks"].tolist()
+        ow = res["overflow_written"].tolist()
+        o = offs.tolist()
+        if ovf is None:
+            starts = [[] for _ in range(m)]
+        else:
+            starts = [[int(ovf[i, 2 * j]) | (int(ovf[i, 2 * j + 1]) << 64)
+                       for j in range(min(ow[i], overflow_cap))] for i in range(m)]
+        return [RangeOutput(t, chunks[a:b], st, e, d, c)
+                for t, a, b, st, e, d, c in zip(totals, o[:-1], o[1:], starts, esc, dms, chk)]
 
 
 class Server:
