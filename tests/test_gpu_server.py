@@ -21,7 +21,9 @@ def requests(seed, n):
     rng = random.Random(seed)
     out = []
     for _ in range(n):
-        lo = rng.randrange(1, 1 << rng.choice([40, 62, 64]))
+        # Magnitudes that run on 96 bits, on two limbs and on three limbs, so launches
+        # change word width as the mix changes.
+        lo = rng.randrange(1, 1 << rng.choice([40, 62, 64, 70, 99, 112]))
         out.append((lo, lo + rng.randrange(1 << 10, 1 << rng.choice([16, 22, 26]))))
     return out
 
