@@ -16,6 +16,9 @@ Beside it, on the same GPU and sieve:
   * served: the request stream (SplitMix64 seed 1; widths log-uniform in [2^16, 2^36];
     80% of starts in [2^40, 2^62), 20% in [2^62, 2^64)) replayed at 50..500 requests/s
     through collatz_server_*, with p50/p99 latency, and once all at once (saturated);
+  * large sieve: k = 37 on [2^60, 2^60 + 2^44) with 2^30-wide chunks;
+  * reference program: the reference's own tree-mode run for all n < 2^50 (reference mode),
+    beside the paper's published RTX 3060 time for the same run;
   * cpu_baseline: the CPU restatement (oracle) on a bounded sample of the same workload.
 
 Every call returns after its stream has synchronised; the timed region is also bracketed
@@ -169,6 +172,9 @@ def main():
     ap.add_argument("--no-range", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-tree", action="store_true")
+    ap.add_argument("--tree-n", type=int, default=50,
+                    help="MAXBITS of the reference-program leg (the whole tree below 2^N)")
     ap.add_argument("--large-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-intervals", type=int, default=32)
@@ -313,6 +319,26 @@ def main():
             "roofline_frac": ach / VALU_PEAK_TOPS}
         log(f"large sieve k=37: {line['large_sieve_k37']}")
         del s37
+
+    if not args.no_tree:
+        # The reference's own program (tree mode) for all n < 2^N from the root case line
+        # "3 2 2 9 1 0", its default knobs (LASTBITS 10, STEPSIZEINITIAL 24, 16 bit vectors,
+        # CUTOFFPOWER 100, collatz_cudav7.cu semantics).  The paper times this run at N = 50
+        # on an RTX 3060: 166 s (p.3, section 1.1).
+        N = args.tree_n
+        t = time.perf_counter()
+        r = cz.reference_run((3, 2, 2, 9, 1, 0), N, 10, semantics=cz.SEM_CUDA)
+        d = time.perf_counter() - t
+        tree = {"N": N, "A": 10, "B": 24, "semantics": "cuda", "seconds": d,
+                "n_per_s": (1 << N) / d, "complete_hash": "%X" % r.complete_hash,
+                "launches": int(r.launches), "leaves": int(r.leaves),
+                "candidates": int(r.candidates)}
+        if N == 50:
+            tree["published"] = {"hardware": "RTX 3060", "seconds": 166.0, "n_per_s": 6.8e12,
+                                 "source": "paper p.3 section 1.1"}
+            tree["speedup_vs_published"] = 166.0 / d
+        line["reference_tree_mode"] = tree
+        log(f"reference program, all n < 2^{N}: {tree}")
 
     if not args.no_serve:
         reqs = W.request_stream(1, 20000)

@@ -42,12 +42,12 @@ static std::map<std::tuple<int, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t
     g_ref_cache;
 
 // Tables alone can be built for deeper runs (N - A up to 120, e.g. main.rs's N = 72); a run
-// needs N - A <= 38 (leaves with 64-bit base and number).
-static int check_params(const collatz_ref_params* p, uint32_t max_mbi = 38) {
+// needs N - A <= 40 (leaves with 64-bit base and number: T^40(n) + 1 < 3^40 + 1 < 2^64).
+static int check_params(const collatz_ref_params* p, uint32_t max_mbi = 40) {
   if (!p) return fail(COLLATZ_EINVAL, "null params");
   if (p->A < 1 || p->A > 12) return fail(COLLATZ_EINVAL, "A (LASTBITS) must be in [1, 12]");
   if (p->N < p->A + 4 || p->N - p->A > max_mbi)
-    return fail(COLLATZ_EINVAL, "N - A out of range (4..38 for a run, 4..120 for tables)");
+    return fail(COLLATZ_EINVAL, "N - A out of range (4..40 for a run, 4..120 for tables)");
   if (p->B < 8 || p->B > 28) return fail(COLLATZ_EINVAL, "B must be in [8, 28]");
   if (p->num_bv < 1 || p->num_bv > 32) return fail(COLLATZ_EINVAL, "num_bv must be in [1, 32]");
   if (p->semantics > 2) return fail(COLLATZ_EINVAL, "unknown semantics");

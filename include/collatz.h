@@ -154,7 +154,7 @@ typedef struct {
 } collatz_node;
 
 typedef struct {
-  uint32_t N;            /* MAXBITS (N - A <= 38) */
+  uint32_t N;            /* MAXBITS (N - A <= 40) */
   uint32_t A;            /* LASTBITS, 1..12 */
   uint32_t B;            /* STEPSIZEINITIAL, 24 */
   uint32_t num_bv;       /* bit vectors, 16 (CUDA/OpenCL) or 8 (Rust) */

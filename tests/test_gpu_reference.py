@@ -56,7 +56,9 @@ def test_case_lines_cuda_semantics_match_oracle(oracle, bv_cuda_n45):
 
 
 @pytest.mark.parametrize("N,expected", [(35, 31109345), (36, 56414851), (37, 106551583),
-                                        (38, 205942265), (39, 390055737), (40, 756583624)])
+                                        (38, 205942265), (39, 390055737), (40, 756583624),
+                                        (41, 1408862059), (42, 2658801769), (43, 5143671444),
+                                        (44, 9392592637), (45, 17807287297)])
 def test_fig51_counts(N, expected):
     """paper p.10 Fig 5.1: integers to iterate for N = 35..40 (main.rs semantics)."""
     import collatz_amd as cz
