@@ -239,35 +239,21 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
   CZ_HIP(dovf.alloc(std::max<uint64_t>(1, 2 * ovf_cap)));
   CZ_HIP(hipMemsetAsync(dbins.p, 0, 17 * 8, s));
   CZ_HIP(hipMemsetAsync(dtot.p, 0, sizeof(RefTotalsHost), s));
-  DevBuf<uint32_t> counts;
-  DevBuf<uint64_t> offs, cands;
   uint64_t leaves = 0;
-  const uint64_t batch = 1ull << 22;
-  const uint32_t nblocks = (uint32_t)device_cu_count(dev) * 28 / 4;
+  const uint32_t nblocks = (uint32_t)device_cu_count(dev) * 32 / 4;
+  CZ_HIP(hipMemsetAsync(&dtot.p->tile_counter, 0, 4, s));
   for (const SieveRoot& sr : roots) {
     LeafSet ls;
     rc = build_leaves(sr, s, ls);
     if (rc) return rc;
     leaves += ls.count;
     CZ_HIP(launch_leaf_bins(ls.meta.p, ls.count, T->dev.tpo, dbins.p, s));
-    for (uint64_t b0 = 0; b0 < ls.count; b0 += batch) {
-      const uint64_t nb = std::min(batch, ls.count - b0);
-      CZ_HIP(counts.reserve(nb));
-      CZ_HIP(offs.reserve(nb + 1));
-      CZ_HIP(launch_leaf_count(T->dev, ls.res.p + b0, ls.num.p + b0, ls.meta.p + b0, nb, counts.p, s));
-      CZ_HIP(launch_scan(counts.p, nb, offs.p, s));
-      uint64_t ncand = 0;
-      CZ_HIP(hipMemcpyAsync(&ncand, offs.p + nb, 8, hipMemcpyDeviceToHost, s));
-      CZ_HIP(hipStreamSynchronize(s));
-      if (!ncand) continue;
-      CZ_HIP(cands.reserve(ncand));
-      CZ_HIP(launch_leaf_write(T->dev, ls.res.p + b0, ls.num.p + b0, ls.meta.p + b0, nb, offs.p,
-                               cands.p, s));
-      CZ_HIP(hipMemsetAsync(&dtot.p->tile_counter, 0, 4, s));
-      CZ_HIP(launch_ref_iterate(limbs, cands.p, ncand, ls.res.p + b0, ls.num.p + b0,
-                                ls.meta.p + b0, mbi, lim, T->pow3.p, 4096, nblocks, dtot.p,
-                                ovf_cap ? dovf.p : nullptr, ovf_cap, s));
-    }
+    // Steps 2 and 3 in one persistent launch over the leaves (tiles of 256 leaves, two
+    // checks between refills).
+    CZ_HIP(hipMemsetAsync(&dtot.p->tile_counter, 0, 4, s));
+    CZ_HIP(launch_ref_fused(limbs, T->dev, ls.res.p, ls.num.p, ls.meta.p, ls.count, mbi, lim,
+                            T->pow3.p, 256, 2, nblocks, dtot.p, ovf_cap ? dovf.p : nullptr,
+                            ovf_cap, s));
     CZ_HIP(hipStreamSynchronize(s));
   }
   RefTotalsHost tot;

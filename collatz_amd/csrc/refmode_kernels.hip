@@ -271,6 +271,144 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
   }
 }
 
+// Steps 2 and 3 fused: persistent waves take tiles of leaves; every lane walks the
+// candidate mask of its own leaf word by word (one word per iteration, so a sparse word
+// never stalls the wave), starts the candidate of each set bit (number + a * 3^f, n = base +
+// a * 2^MBI; Lemma 2.3), and runs up to U checks between refills.  No candidate list is
+// written.  Hash, count and overflow count are folded with one atomic per wave and tile.
+template <int L, bool LOWCUT>
+__global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_t* base,
+                                                   const uint64_t* num, const uint32_t* meta,
+                                                   uint64_t nleaves, uint32_t mbi, Limits lim,
+                                                   const uint64_t* pow3, uint32_t tile,
+                                                   uint32_t U, RefTotals* tot, uint64_t* ovf,
+                                                   uint64_t ovf_cap) {
+  __shared__ Pow3Lds p3s;
+  stage_pow3(p3s);
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t ntiles = (nleaves + tile - 1) / tile;
+  const uint32_t cutoff = lim.cutoff;
+  const uint64_t thr = lim.fast_thr;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(&tot->tile_counter, 1u);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= ntiles) break;
+    uint64_t cur = (uint64_t)t * tile;
+    const uint64_t end = cur + tile < nleaves ? cur + tile : nleaves;
+    uint32_t h = 0, cnt = 0, ov = 0, stall = 0;
+    // Leaf of the lane: base, number, meta, 3^f, mask word index and remaining bits.
+    bool has_leaf = false, strict = false;
+    uint64_t lb = 0, ln = 0, lp3 = 0;
+    uint32_t lmeta = 0, w = 0, m = 0;
+    // Candidate of the lane.
+    bool act = false;
+    Num<L> v;
+    uint64_t n_lo = 0;
+    uint32_t checks = 0;
+    for (;;) {
+      // (1) Lanes without a candidate take the next set bit of their leaf's mask, or move
+      //     to the mask's next word (one word per iteration), or give the leaf up.
+      if (!act && has_leaf) {
+        if (m == 0) {
+          if (++w < T.nw) m = leaf_mask_word(T, lb, ln, lmeta, w);
+          else has_leaf = false;
+        }
+        if (m != 0) {
+          const uint32_t j = __builtin_ctz(m);
+          m &= m - 1;
+          const uint64_t a = 32u * w + j;
+          // number + a * 3^f (Lemma 2.3; collatz_cudav7.cu:262-278)
+          const unsigned __int128 x = (unsigned __int128)a * lp3 + ln;
+          v.w[0] = (uint32_t)x;
+          v.w[1] = (uint32_t)(x >> 32);
+          v.w[2] = (uint32_t)(x >> 64);
+          v.w[3] = (uint32_t)(x >> 96);
+#pragma unroll
+          for (int i = 4; i < L; ++i) v.w[i] = 0;
+          n_lo = lb + (a << mbi);
+          checks = 0;
+          act = true;
+          cnt += 1;
+        }
+      }
+      // (2) Lanes without a leaf take the next leaves of the tile.
+      const uint64_t need = __ballot(!has_leaf);
+      if (need && cur < end) {
+        const uint32_t want = __popcll(need);
+        const uint64_t avail = end - cur;
+        const uint32_t take = avail < want ? (uint32_t)avail : want;
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+        if (!has_leaf && rank < take) {
+          const uint64_t leaf = cur + rank;
+          lb = base[leaf];
+          ln = num[leaf];
+          lmeta = meta[leaf];
+          const uint32_t f = meta_f(lmeta);
+          lp3 = pow3[f];
+          // Leaves without a bit vector run on the CPU in the reference (search2/search3,
+          // collatz_cudav7.cu:517-519), whose second overflow test is '>' (:702).
+          strict = (f - T.tpo) >= 16;
+          w = 0;
+          m = leaf_mask_word(T, lb, ln, lmeta, 0);
+          has_leaf = true;
+        }
+        cur += take;
+      }
+      if (__ballot(act || has_leaf) == 0) break;
+      // (3) Up to U checks.
+      if (act) {
+        for (uint32_t u = 0; u < U; ++u) {
+#pragma unroll
+          for (int half = 0; half < 2; ++half) {
+            odd_run<L>(v, p3s);
+            if (over_fast<L, LOWCUT>(v, thr) && reaches<L>(v, cutoff, strict && half == 1)) {
+              ov += 1;
+              if (ovf) {
+                const unsigned long long i = atomicAdd(&tot->overflow, 1ull);
+                if (i < ovf_cap) {
+                  ovf[2 * i] = n_lo;
+                  ovf[2 * i + 1] = 0;
+                }
+              }
+              v.w[0] = 2;
+#pragma unroll
+              for (int i = 1; i < L; ++i) v.w[i] = 0;
+            }
+            even_run<L>(v);
+          }
+          h ^= v.w[0];
+          bool done = lt_base<L>(v, n_lo, 0);
+          if (++checks >= kMaxChecks && !done) {
+            stall += 1;
+            done = true;
+          }
+          if (done) {
+            act = false;
+            break;
+          }
+        }
+      }
+    }
+    // One atomic per wave and tile.
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1) {
+      h ^= __shfl_xor(h, k, 64);
+      cnt += __shfl_xor(cnt, k, 64);
+      stall += __shfl_xor(stall, k, 64);
+      if (!ovf) ov += __shfl_xor(ov, k, 64);
+    }
+    if (lane == 0) {
+      atomicXor(&tot->hash, h);
+      atomicAdd(&tot->count, (unsigned long long)cnt);
+      if (stall) atomicAdd(&tot->stall, (unsigned long long)stall);
+      if (!ovf && ov) atomicAdd(&tot->overflow, (unsigned long long)ov);
+    }
+  }
+}
+
 static inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
 hipError_t launch_bv_build(uint32_t B, uint32_t num_bv, int32_t first, uint32_t sem,
@@ -321,6 +459,24 @@ hipError_t launch_ref_iterate(int limbs, const uint64_t* cands, uint64_t ncand,
   else
     hipLaunchKernelGGL((k_ref_iterate<6, true>), dim3(nblocks), dim3(256), 0, s, cands, ncand,
                        base, num, meta, mbi, lim, pow3, tile, (RefTotals*)tot, ovf, ovf_cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_fused(int limbs, const RefTablesDev& T, const uint64_t* base,
+                            const uint64_t* num, const uint32_t* meta, uint64_t nleaves,
+                            uint32_t mbi, const Limits& lim, const uint64_t* pow3, uint32_t tile,
+                            uint32_t U, uint32_t nblocks, void* tot, uint64_t* ovf,
+                            uint64_t ovf_cap, hipStream_t s) {
+  if (!nleaves) return hipSuccess;
+  if (limbs == 2)
+    hipLaunchKernelGGL((k_ref_fused<4, false>), dim3(nblocks), dim3(256), 0, s, T, base, num,
+                       meta, nleaves, mbi, lim, pow3, tile, U, (RefTotals*)tot, ovf, ovf_cap);
+  else if (lim.cutoff >= 128)
+    hipLaunchKernelGGL((k_ref_fused<6, false>), dim3(nblocks), dim3(256), 0, s, T, base, num,
+                       meta, nleaves, mbi, lim, pow3, tile, U, (RefTotals*)tot, ovf, ovf_cap);
+  else
+    hipLaunchKernelGGL((k_ref_fused<6, true>), dim3(nblocks), dim3(256), 0, s, T, base, num,
+                       meta, nleaves, mbi, lim, pow3, tile, U, (RefTotals*)tot, ovf, ovf_cap);
   return hipGetLastError();
 }
 
