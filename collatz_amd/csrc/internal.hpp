@@ -214,7 +214,7 @@ hipError_t launch_ref_fused(int limbs, const RefTablesDev& T, const uint64_t* ba
                             const uint64_t* num, const uint32_t* meta, uint64_t nleaves,
                             uint32_t mbi, const Limits& lim, const uint64_t* pow3, uint32_t tile,
                             uint32_t U, uint32_t nblocks, void* tot, uint64_t* ovf,
-                            uint64_t ovf_cap, hipStream_t s);
+                            uint64_t ovf_cap, uint64_t* esc, uint64_t esc_cap, hipStream_t s);
 hipError_t launch_ref_iterate(int limbs, const uint64_t* cands, uint64_t ncand,
                               const uint64_t* base, const uint64_t* num, const uint32_t* meta,
                               uint32_t mbi, const Limits& lim, const uint64_t* pow3, uint32_t tile,

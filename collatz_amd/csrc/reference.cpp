@@ -28,6 +28,7 @@ struct RefTotalsHost {
   unsigned long long stall;
   unsigned int hash;
   unsigned int tile_counter;
+  unsigned long long esc_count;
 };
 
 struct RefTables {
@@ -232,30 +233,53 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
   const Limits lim = make_limits(limbs, params->cutoff_log2);
   DevBuf<unsigned long long> dbins;
   DevBuf<RefTotalsHost> dtot;
-  DevBuf<uint64_t> dovf;
+  DevBuf<uint64_t> dovf, desc;
   const uint64_t ovf_cap = out->overflow_starts ? std::max<uint64_t>(out->overflow_cap, 1) : 0;
+  const uint64_t esc_cap = 1ull << 20;
   CZ_HIP(dbins.alloc(17));
   CZ_HIP(dtot.alloc(1));
   CZ_HIP(dovf.alloc(std::max<uint64_t>(1, 2 * ovf_cap)));
-  CZ_HIP(hipMemsetAsync(dbins.p, 0, 17 * 8, s));
-  CZ_HIP(hipMemsetAsync(dtot.p, 0, sizeof(RefTotalsHost), s));
-  uint64_t leaves = 0;
+  CZ_HIP(desc.alloc(esc_cap));
   const uint32_t nblocks = (uint32_t)device_cu_count(dev) * 32 / 4;
-  CZ_HIP(hipMemsetAsync(&dtot.p->tile_counter, 0, 4, s));
-  for (const SieveRoot& sr : roots) {
-    LeafSet ls;
-    rc = build_leaves(sr, s, ls);
-    if (rc) return rc;
-    leaves += ls.count;
-    CZ_HIP(launch_leaf_bins(ls.meta.p, ls.count, T->dev.tpo, dbins.p, s));
-    // Steps 2 and 3 in one persistent launch over the leaves (tiles of 256 leaves, two
-    // checks between refills).
-    CZ_HIP(hipMemsetAsync(&dtot.p->tile_counter, 0, 4, s));
-    CZ_HIP(launch_ref_fused(limbs, T->dev, ls.res.p, ls.num.p, ls.meta.p, ls.count, mbi, lim,
-                            T->pow3.p, 256, 2, nblocks, dtot.p, ovf_cap ? dovf.p : nullptr,
-                            ovf_cap, s));
-    CZ_HIP(hipStreamSynchronize(s));
-  }
+  uint64_t leaves = 0;
+  // The whole run at one word width of the fused pass: 96 bits (escalating to `limbs`
+  // above 2^86) when the cutoff is at least 2^96, `limbs` otherwise.
+  auto run_all = [&](int width) -> int {
+    CZ_HIP(hipMemsetAsync(dbins.p, 0, 17 * 8, s));
+    CZ_HIP(hipMemsetAsync(dtot.p, 0, sizeof(RefTotalsHost), s));
+    leaves = 0;
+    for (const SieveRoot& sr : roots) {
+      LeafSet ls;
+      int rc2 = build_leaves(sr, s, ls);
+      if (rc2) return rc2;
+      leaves += ls.count;
+      CZ_HIP(launch_leaf_bins(ls.meta.p, ls.count, T->dev.tpo, dbins.p, s));
+      // Steps 2 and 3 in one persistent launch over the leaves (tiles of 256 leaves, two
+      // checks between refills).
+      CZ_HIP(hipMemsetAsync(&dtot.p->tile_counter, 0, 4, s));
+      CZ_HIP(hipMemsetAsync(&dtot.p->esc_count, 0, 8, s));
+      CZ_HIP(launch_ref_fused(width, T->dev, ls.res.p, ls.num.p, ls.meta.p, ls.count, mbi, lim,
+                              T->pow3.p, 256, 2, nblocks, dtot.p, ovf_cap ? dovf.p : nullptr,
+                              ovf_cap, desc.p, esc_cap, s));
+      unsigned long long nesc = 0;
+      CZ_HIP(hipMemcpyAsync(&nesc, &dtot.p->esc_count, 8, hipMemcpyDeviceToHost, s));
+      CZ_HIP(hipStreamSynchronize(s));
+      if (nesc > esc_cap) return fail(COLLATZ_ECAPACITY, "escalation list overflowed");
+      if (nesc) {
+        // Candidates that left 96 bits: rerun from their start on two or three limbs.
+        CZ_HIP(hipMemsetAsync(&dtot.p->tile_counter, 0, 4, s));
+        CZ_HIP(launch_ref_iterate(limbs, desc.p, nesc, ls.res.p, ls.num.p, ls.meta.p, mbi, lim,
+                                  T->pow3.p, 4096, nblocks, dtot.p, ovf_cap ? dovf.p : nullptr,
+                                  ovf_cap, s));
+        CZ_HIP(hipStreamSynchronize(s));
+      }
+    }
+    return COLLATZ_OK;
+  };
+  const bool narrow = params->cutoff_log2 >= 96;
+  rc = run_all(narrow ? 1 : limbs);
+  if (rc == COLLATZ_ECAPACITY && narrow) rc = run_all(limbs);
+  if (rc) return rc;
   RefTotalsHost tot;
   unsigned long long bins[17];
   CZ_HIP(hipMemcpyAsync(&tot, dtot.p, sizeof(tot), hipMemcpyDeviceToHost, s));

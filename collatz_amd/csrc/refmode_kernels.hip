@@ -152,6 +152,7 @@ struct RefTotals {
   unsigned long long stall;
   unsigned int hash;
   unsigned int tile_counter;
+  unsigned long long esc_count;  // candidates handed from 96 bits to the wide pass
 };
 
 // num >= 2^c, or (strict) num > 2^c.
@@ -276,13 +277,20 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
 // never stalls the wave), starts the candidate of each set bit (number + a * 3^f, n = base +
 // a * 2^MBI; Lemma 2.3), and runs up to U checks between refills.  No candidate list is
 // written.  Hash, count and overflow count are folded with one atomic per wave and tile.
+//
+// L = 3 (96 bits, cutoff at or above 2^96) escalates instead of testing for overflow: a
+// check multiplies by less than 2^9.4, so a candidate that starts every check below 2^86
+// never reaches 2^96 (nor the cutoff) inside one; a candidate that ends a check above 2^86
+// is listed as (leaf, a, CPU-leaf flag), its partial hash dropped, and rerun from its start
+// by k_ref_iterate on two or three limbs.
 template <int L, bool LOWCUT>
 __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_t* base,
                                                    const uint64_t* num, const uint32_t* meta,
                                                    uint64_t nleaves, uint32_t mbi, Limits lim,
                                                    const uint64_t* pow3, uint32_t tile,
                                                    uint32_t U, RefTotals* tot, uint64_t* ovf,
-                                                   uint64_t ovf_cap) {
+                                                   uint64_t ovf_cap, uint64_t* esc,
+                                                   uint64_t esc_cap) {
   __shared__ Pow3Lds p3s;
   stage_pow3(p3s);
   __syncthreads();
@@ -306,7 +314,8 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
     bool act = false;
     Num<L> v;
     uint64_t n_lo = 0;
-    uint32_t checks = 0;
+    uint32_t checks = 0, ch = 0, cur_a = 0;
+    uint64_t cur_leaf = 0;
     for (;;) {
       // (1) Lanes without a candidate take the next set bit of their leaf's mask, or move
       //     to the mask's next word (one word per iteration), or give the leaf up.
@@ -324,13 +333,26 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
           v.w[0] = (uint32_t)x;
           v.w[1] = (uint32_t)(x >> 32);
           v.w[2] = (uint32_t)(x >> 64);
-          v.w[3] = (uint32_t)(x >> 96);
+          if constexpr (L > 3) v.w[3] = (uint32_t)(x >> 96);
 #pragma unroll
           for (int i = 4; i < L; ++i) v.w[i] = 0;
           n_lo = lb + (a << mbi);
           checks = 0;
+          ch = 0;
+          cur_a = (uint32_t)a;
           act = true;
           cnt += 1;
+          if constexpr (L == 3) {
+            // A start at or above the 96-bit check bound goes to the wide pass at once
+            // (a * 3^f < 2^76 and number < 2^64, so the start fits 96 bits).
+            if (v.w[2] >= (1u << (kNarrowCheckLog2 - 64))) {
+              const unsigned long long i = atomicAdd(&tot->esc_count, 1ull);
+              if (i < esc_cap)
+                esc[i] = cur_leaf | ((uint64_t)cur_a << 32) | (strict ? (1ull << 48) : 0ull);
+              cnt -= 1;
+              act = false;
+            }
+          }
         }
       }
       // (2) Lanes without a leaf take the next leaves of the tile.
@@ -343,6 +365,7 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
             (uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
         if (!has_leaf && rank < take) {
           const uint64_t leaf = cur + rank;
+          cur_leaf = leaf;
           lb = base[leaf];
           ln = num[leaf];
           lmeta = meta[leaf];
@@ -364,28 +387,40 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
 #pragma unroll
           for (int half = 0; half < 2; ++half) {
             odd_run<L>(v, p3s);
-            if (over_fast<L, LOWCUT>(v, thr) && reaches<L>(v, cutoff, strict && half == 1)) {
-              ov += 1;
-              if (ovf) {
-                const unsigned long long i = atomicAdd(&tot->overflow, 1ull);
-                if (i < ovf_cap) {
-                  ovf[2 * i] = n_lo;
-                  ovf[2 * i + 1] = 0;
+            if constexpr (L > 3) {
+              if (over_fast<L, LOWCUT>(v, thr) && reaches<L>(v, cutoff, strict && half == 1)) {
+                ov += 1;
+                if (ovf) {
+                  const unsigned long long i = atomicAdd(&tot->overflow, 1ull);
+                  if (i < ovf_cap) {
+                    ovf[2 * i] = n_lo;
+                    ovf[2 * i + 1] = 0;
+                  }
                 }
-              }
-              v.w[0] = 2;
+                v.w[0] = 2;
 #pragma unroll
-              for (int i = 1; i < L; ++i) v.w[i] = 0;
+                for (int i = 1; i < L; ++i) v.w[i] = 0;
+              }
             }
             even_run<L>(v);
           }
-          h ^= v.w[0];
+          ch ^= v.w[0];
           bool done = lt_base<L>(v, n_lo, 0);
-          if (++checks >= kMaxChecks && !done) {
+          bool wide = false;
+          if constexpr (L == 3) wide = v.w[2] >= (1u << (kNarrowCheckLog2 - 64));
+          if (++checks >= kMaxChecks && !done && !wide) {
             stall += 1;
             done = true;
           }
-          if (done) {
+          if (done || wide) {
+            if (wide) {
+              const unsigned long long i = atomicAdd(&tot->esc_count, 1ull);
+              if (i < esc_cap)
+                esc[i] = cur_leaf | ((uint64_t)cur_a << 32) | (strict ? (1ull << 48) : 0ull);
+              cnt -= 1;
+            } else {
+              h ^= ch;
+            }
             act = false;
             break;
           }
@@ -466,17 +501,24 @@ hipError_t launch_ref_fused(int limbs, const RefTablesDev& T, const uint64_t* ba
                             const uint64_t* num, const uint32_t* meta, uint64_t nleaves,
                             uint32_t mbi, const Limits& lim, const uint64_t* pow3, uint32_t tile,
                             uint32_t U, uint32_t nblocks, void* tot, uint64_t* ovf,
-                            uint64_t ovf_cap, hipStream_t s) {
+                            uint64_t ovf_cap, uint64_t* esc, uint64_t esc_cap, hipStream_t s) {
   if (!nleaves) return hipSuccess;
-  if (limbs == 2)
+  if (limbs == 1)
+    hipLaunchKernelGGL((k_ref_fused<3, false>), dim3(nblocks), dim3(256), 0, s, T, base, num,
+                       meta, nleaves, mbi, lim, pow3, tile, U, (RefTotals*)tot, ovf, ovf_cap,
+                       esc, esc_cap);
+  else if (limbs == 2)
     hipLaunchKernelGGL((k_ref_fused<4, false>), dim3(nblocks), dim3(256), 0, s, T, base, num,
-                       meta, nleaves, mbi, lim, pow3, tile, U, (RefTotals*)tot, ovf, ovf_cap);
+                       meta, nleaves, mbi, lim, pow3, tile, U, (RefTotals*)tot, ovf, ovf_cap,
+                       esc, 0);
   else if (lim.cutoff >= 128)
     hipLaunchKernelGGL((k_ref_fused<6, false>), dim3(nblocks), dim3(256), 0, s, T, base, num,
-                       meta, nleaves, mbi, lim, pow3, tile, U, (RefTotals*)tot, ovf, ovf_cap);
+                       meta, nleaves, mbi, lim, pow3, tile, U, (RefTotals*)tot, ovf, ovf_cap,
+                       esc, 0);
   else
     hipLaunchKernelGGL((k_ref_fused<6, true>), dim3(nblocks), dim3(256), 0, s, T, base, num,
-                       meta, nleaves, mbi, lim, pow3, tile, U, (RefTotals*)tot, ovf, ovf_cap);
+                       meta, nleaves, mbi, lim, pow3, tile, U, (RefTotals*)tot, ovf, ovf_cap,
+                       esc, 0);
   return hipGetLastError();
 }
 
