@@ -8,6 +8,7 @@
 // one per full batch of 2^16 leaves of a bin plus one flush per non-empty bin
 // (collatz_cudav7.cu:532-533, :177-179).
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -254,13 +255,15 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
       if (rc2) return rc2;
       leaves += ls.count;
       CZ_HIP(launch_leaf_bins(ls.meta.p, ls.count, T->dev.tpo, dbins.p, s));
-      // Steps 2 and 3 in one persistent launch over the leaves (tiles of 256 leaves, two
-      // checks between refills).
+      // Steps 2 and 3 in one persistent launch over the leaves.
       CZ_HIP(hipMemsetAsync(&dtot.p->tile_counter, 0, 4, s));
       CZ_HIP(hipMemsetAsync(&dtot.p->esc_count, 0, 8, s));
+      // Tiles of 2048 leaves (each lane walks whole leaves, whose candidate counts vary a
+      // lot, so a tile needs many leaves per lane to balance) and four checks between
+      // refills (candidates that pass the look-ahead filters run long).
       CZ_HIP(launch_ref_fused(width, T->dev, ls.res.p, ls.num.p, ls.meta.p, ls.count, mbi, lim,
-                              T->pow3.p, 256, 2, nblocks, dtot.p, ovf_cap ? dovf.p : nullptr,
-                              ovf_cap, desc.p, esc_cap, s));
+                              T->pow3.p, 2048, 4, nblocks, dtot.p,
+                              ovf_cap ? dovf.p : nullptr, ovf_cap, desc.p, esc_cap, s));
       unsigned long long nesc = 0;
       CZ_HIP(hipMemcpyAsync(&nesc, &dtot.p->esc_count, 8, hipMemcpyDeviceToHost, s));
       CZ_HIP(hipStreamSynchronize(s));
