@@ -159,6 +159,25 @@ def served_leg(sv, reqs, rate, chunk_log2):
             "max_ms": lat[-1], "n_per_s": n / span_s, "offered_req_per_s": rate}
 
 
+def tree_leg(cz, N):
+    """The reference's own program (tree mode) for all n < 2^N from the root case line
+    "3 2 2 9 1 0", its default knobs (LASTBITS 10, STEPSIZEINITIAL 24, 16 bit vectors,
+    CUTOFFPOWER 100, collatz_cudav7.cu semantics).  The paper times this run at N = 50 on an
+    RTX 3060: 166 s (p.3, section 1.1)."""
+    t = time.perf_counter()
+    r = cz.reference_run((3, 2, 2, 9, 1, 0), N, 10, semantics=cz.SEM_CUDA)
+    d = time.perf_counter() - t
+    tree = {"N": N, "A": 10, "B": 24, "semantics": "cuda", "seconds": d,
+            "n_per_s": (1 << N) / d, "complete_hash": "%X" % r.complete_hash,
+            "launches": int(r.launches), "leaves": int(r.leaves),
+            "candidates": int(r.candidates)}
+    if N == 50:
+        tree["published"] = {"hardware": "RTX 3060", "seconds": 166.0, "n_per_s": 6.8e12,
+                             "source": "paper p.3 section 1.1"}
+        tree["speedup_vs_published"] = 166.0 / d
+    return tree
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +194,8 @@ def main():
     ap.add_argument("--no-tree", action="store_true")
     ap.add_argument("--tree-n", type=int, default=50,
                     help="MAXBITS of the reference-program leg (the whole tree below 2^N)")
+    ap.add_argument("--only-tree", action="store_true",
+                    help="run only the reference-program leg and print its record")
     ap.add_argument("--large-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-intervals", type=int, default=32)
@@ -197,6 +218,10 @@ def main():
         sys.exit(2)
     import collatz_amd as cz
     from collatz_amd import workloads as W
+
+    if args.only_tree:
+        print(json.dumps(tree_leg(cz, args.tree_n)), flush=True)
+        return
 
     ivs = W.batch_intervals(0, 4096, K)
     total_n = sum(hi - lo for lo, hi in ivs)
@@ -321,24 +346,8 @@ def main():
         del s37
 
     if not args.no_tree:
-        # The reference's own program (tree mode) for all n < 2^N from the root case line
-        # "3 2 2 9 1 0", its default knobs (LASTBITS 10, STEPSIZEINITIAL 24, 16 bit vectors,
-        # CUTOFFPOWER 100, collatz_cudav7.cu semantics).  The paper times this run at N = 50
-        # on an RTX 3060: 166 s (p.3, section 1.1).
-        N = args.tree_n
-        t = time.perf_counter()
-        r = cz.reference_run((3, 2, 2, 9, 1, 0), N, 10, semantics=cz.SEM_CUDA)
-        d = time.perf_counter() - t
-        tree = {"N": N, "A": 10, "B": 24, "semantics": "cuda", "seconds": d,
-                "n_per_s": (1 << N) / d, "complete_hash": "%X" % r.complete_hash,
-                "launches": int(r.launches), "leaves": int(r.leaves),
-                "candidates": int(r.candidates)}
-        if N == 50:
-            tree["published"] = {"hardware": "RTX 3060", "seconds": 166.0, "n_per_s": 6.8e12,
-                                 "source": "paper p.3 section 1.1"}
-            tree["speedup_vs_published"] = 166.0 / d
-        line["reference_tree_mode"] = tree
-        log(f"reference program, all n < 2^{N}: {tree}")
+        line["reference_tree_mode"] = tree_leg(cz, args.tree_n)
+        log(f"reference program, all n < 2^{args.tree_n}: {line['reference_tree_mode']}")
 
     if not args.no_serve:
         reqs = W.request_stream(1, 20000)
