@@ -170,9 +170,6 @@ hipError_t launch_lower_bound(const uint4* sorted, const uint64_t* class_start,
                               const uint64_t* keys, const uint32_t* kcls, uint64_t nkeys,
                               uint64_t* out, hipStream_t s);
 // Replaces flagged row bounds by their lower_bound in the row's class.
-hipError_t launch_rows_resolve(RowDesc* rows, uint64_t nrows, const uint4* sorted,
-                               const uint64_t* class_start, const uint64_t* keys,
-                               hipStream_t s);
 // Same, for ctl->nrows rows, with a fixed grid (graph-capturable).
 hipError_t launch_rows_resolve_ctl(RowDesc* rows, const Ctl* ctl, const uint4* sorted,
                                    const uint64_t* class_start, const uint64_t* keys,
@@ -203,11 +200,6 @@ hipError_t launch_reduce_slots(const DevRec* parts, const uint64_t* part_start, 
 hipError_t launch_bv_build(uint32_t B, uint32_t num_bv, int32_t first, uint32_t sem,
                            const uint32_t* magic, uint32_t* bv, uint64_t words_per_bv,
                            uint32_t mirror_words, hipStream_t s);
-hipError_t launch_leaf_count(const RefTablesDev& T, const uint64_t* base, const uint64_t* num,
-                             const uint32_t* meta, uint64_t n, uint32_t* counts, hipStream_t s);
-hipError_t launch_leaf_write(const RefTablesDev& T, const uint64_t* base, const uint64_t* num,
-                             const uint32_t* meta, uint64_t n, const uint64_t* offs,
-                             uint64_t* cands, hipStream_t s);
 hipError_t launch_leaf_bins(const uint32_t* meta, uint64_t n, uint32_t tpo,
                             unsigned long long* bins, hipStream_t s);
 hipError_t launch_ref_fused(int limbs, const RefTablesDev& T, const uint64_t* base,

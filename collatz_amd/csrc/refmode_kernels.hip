@@ -5,12 +5,13 @@
 //                    n0 < 2^B, its B-step parity word and max dip; a bit at n0 * 3^{-f} mod 2^B
 //                    in every BV_i whose cutoff it meets.  The first words of each vector are
 //                    mirrored after the end so windows never wrap.
-//   k_leaf_count /   Step 2 (collatz_cudav7.cu:219-235): per leaf, the 2^A-bit window of its
-//   k_leaf_write     BV_i AND the mod 9 row of its base; the set bits are the candidates, counted
-//                    then written as (leaf, a) pairs at scanned offsets.
-//   k_ref_iterate    Step 3: persistent waves iterate the candidate list with the schedule of
-//                    compute_kernel (collatz_cudav7.cu:280-366) and fold hash, count and overflow
-//                    count with one atomic per wave and tile.
+//   k_ref_fused      Steps 2 and 3 (collatz_cudav7.cu:219-235, :280-366): persistent waves
+//                    take tiles of leaves; each lane walks its leaf's 2^A-bit candidate mask
+//                    (BV_i window AND the mod 9 row of its base) and iterates the candidate of
+//                    each set bit with the schedule; hash, count and overflow count are folded
+//                    with one atomic per wave and tile.
+//   k_ref_iterate    Step 3 over a list of (leaf, a) candidates: the two- or three-limb rerun
+//                    of the candidates that left 96 bits in k_ref_fused.
 #include "schedule.hpp"
 
 namespace cz {
@@ -95,41 +96,6 @@ __device__ __forceinline__ uint32_t leaf_mask_word(const RefTablesDev& T, uint64
     }
   }
   return m;
-}
-
-__global__ __launch_bounds__(256) void k_leaf_count(RefTablesDev T, const uint64_t* base,
-                                                    const uint64_t* num, const uint32_t* meta,
-                                                    uint64_t nleaves, uint32_t* counts) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nleaves) return;
-  const uint64_t b = base[t], n = num[t];
-  const uint32_t mt = meta[t];
-  uint32_t c = 0;
-  for (uint32_t w = 0; w < T.nw; ++w) c += __popc(leaf_mask_word(T, b, n, mt, w));
-  counts[t] = c;
-}
-
-// Candidate entry: leaf (bits 0-31), a (bits 32-47), CPU-leaf flag (bit 48).
-__global__ __launch_bounds__(256) void k_leaf_write(RefTablesDev T, const uint64_t* base,
-                                                    const uint64_t* num, const uint32_t* meta,
-                                                    uint64_t nleaves, const uint64_t* offs,
-                                                    uint64_t* cands) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nleaves) return;
-  const uint64_t b = base[t], n = num[t];
-  const uint32_t mt = meta[t];
-  // Leaves without a bit vector run on the CPU in the reference (search2/search3,
-  // collatz_cudav7.cu:517-519), whose second overflow test is '>' (:702).
-  const uint64_t cpu = (meta_f(mt) - T.tpo) >= 16 ? (1ull << 48) : 0;
-  uint64_t o = offs[t];
-  for (uint32_t w = 0; w < T.nw; ++w) {
-    uint32_t m = leaf_mask_word(T, b, n, mt, w);
-    while (m) {
-      const uint32_t j = __builtin_ctz(m);
-      m &= m - 1;
-      cands[o++] = (uint64_t)t | ((uint64_t)(32 * w + j) << 32) | cpu;
-    }
-  }
 }
 
 __global__ __launch_bounds__(256) void k_leaf_bins(const uint32_t* meta, uint64_t n, uint32_t tpo,
@@ -451,23 +417,6 @@ hipError_t launch_bv_build(uint32_t B, uint32_t num_bv, int32_t first, uint32_t 
                            uint32_t mirror_words, hipStream_t s) {
   hipLaunchKernelGGL(k_bv_build, dim3(blocks_for(1ull << B, 256)), dim3(256), 0, s, B, num_bv,
                      first, sem, magic, bv, words_per_bv, mirror_words);
-  return hipGetLastError();
-}
-
-hipError_t launch_leaf_count(const RefTablesDev& T, const uint64_t* base, const uint64_t* num,
-                             const uint32_t* meta, uint64_t n, uint32_t* counts, hipStream_t s) {
-  if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_leaf_count, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, base, num, meta,
-                     n, counts);
-  return hipGetLastError();
-}
-
-hipError_t launch_leaf_write(const RefTablesDev& T, const uint64_t* base, const uint64_t* num,
-                             const uint32_t* meta, uint64_t n, const uint64_t* offs,
-                             uint64_t* cands, hipStream_t s) {
-  if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_leaf_write, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, base, num, meta,
-                     n, offs, cands);
   return hipGetLastError();
 }
 

@@ -385,32 +385,6 @@ __device__ __forceinline__ uint64_t lower_bound_dev(const uint4* a, uint64_t lo,
   return lo;
 }
 
-__global__ __launch_bounds__(256) void k_rows_resolve(RowDesc* rows, uint64_t nrows,
-                                                      const uint4* a,
-                                                      const uint64_t* class_start,
-                                                      const uint64_t* keys) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nrows) return;
-  RowDesc& r = rows[t];
-  const uint64_t lo = class_start[r.cls], hi = class_start[r.cls + 1];
-  uint64_t i_lo = r.i_lo, i_hi = r.i_hi;
-  if (i_lo & kResolveFlag) i_lo = lower_bound_dev(a, lo, hi, keys[i_lo & ~kResolveFlag]);
-  if (i_hi & kResolveFlag) i_hi = lower_bound_dev(a, lo, hi, keys[i_hi & ~kResolveFlag]);
-  if (i_hi < i_lo) i_hi = i_lo;
-  r.i_lo = i_lo;
-  r.i_hi = i_hi;
-}
-
-hipError_t launch_rows_resolve(RowDesc* rows, uint64_t nrows, const uint4* sorted,
-                               const uint64_t* class_start, const uint64_t* keys,
-                               hipStream_t s) {
-  if (nrows == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rows_resolve, dim3(blocks_for(nrows, 256)), dim3(256), 0, s, rows, nrows,
-                     sorted, class_start, keys);
-  return hipGetLastError();
-}
-
-
 __global__ __launch_bounds__(256) void k_rows_resolve_ctl(RowDesc* rows, const Ctl* ctl,
                                                           const uint4* a,
                                                           const uint64_t* class_start,
