@@ -29,6 +29,7 @@ struct Request {
   double t_submit = 0, t_admit = 0, t_done = 0;
   RowPlan plan;
   std::vector<uint64_t> row_est;
+  uint64_t est_left = 0;  // estimated candidates of the rows not yet launched
   size_t next_row = 0;
   std::vector<DevRec> recs;
   uint64_t escalated = 0;
@@ -62,7 +63,6 @@ namespace cz {
 
 static void server_loop(collatz_server* S) {
   (void)hipSetDevice(S->sv->device);
-  const uint64_t budget = S->opts.slice_candidates;
   for (;;) {
     LaunchPlan lp;
     u128 max_hi = 0;  // largest end of the requests in this launch (sets the word width)
@@ -87,7 +87,13 @@ static void server_loop(collatz_server* S) {
       }
       // Continuous batching: every admitted request gets an equal share of the launch's work
       // budget, and at least one row, so a short request finishes within one launch however
-      // long the requests ahead of it are; what the shares leave goes to the oldest.
+      // long the requests ahead of it are; what the shares leave goes to the oldest.  The
+      // budget is slice_candidates, raised up to 16x while the admitted backlog is large
+      // (a quarter of it per launch), so a saturated server runs fewer, longer launches.
+      uint64_t backlog = 0;
+      for (const Request* r : S->active) backlog += r->est_left;
+      const uint64_t budget = std::min<uint64_t>(
+          std::max<uint64_t>(S->opts.slice_candidates, backlog / 4), 16 * S->opts.slice_candidates);
       uint64_t taken = 0;
       auto take = [&](Request* r, uint64_t want, bool one_row_always) {
         uint64_t mine = 0, last_chunk = UINT64_MAX;
@@ -124,6 +130,7 @@ static void server_loop(collatz_server* S) {
           if (r->hi > max_hi) max_hi = r->hi;
           mine += r->row_est[i] + 1;
           taken += r->row_est[i] + 1;
+          r->est_left -= std::min(r->est_left, r->row_est[i]);
           r->next_row++;
         }
       };
@@ -241,6 +248,7 @@ int64_t collatz_server_submit(collatz_server* S, const collatz_interval* range) 
     const double b = r->plan.key_hi[i] < 0 ? kspan : (double)r->plan.keys[r->plan.key_hi[i]];
     r->row_est[i] =
         (uint64_t)((double)(S->sv->class_start[c + 1] - S->sv->class_start[c]) * (b - a) / kspan);
+    r->est_left += r->row_est[i];
   }
   r->recs.assign(r->plan.nchunks, DevRec{});
   for (DevRec& d : r->recs) std::memset(&d, 0, sizeof(d));

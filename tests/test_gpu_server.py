@@ -72,3 +72,50 @@ def test_server_bad_request(s20):
     with pytest.raises(cz.CollatzError):
         srv.submit(10, 10)
     srv.close()
+
+
+def test_admission_budget_serialises(s20):
+    """With a device budget smaller than any request, requests are admitted one at a time
+    (a request is always admitted when nothing else is in flight) and still all finish with
+    the records they have alone."""
+    import collatz_amd as cz
+    srv = cz.Server(s20, chunk_log2=20, max_device_bytes=1)
+    reqs = requests(5, 12)
+    tickets = [srv.submit(lo, hi) for lo, hi in reqs]
+    prev_done = -1.0
+    for t, (lo, hi) in zip(tickets, reqs):
+        total, chunks, (ts, ta, td) = srv.wait(t, 60000)
+        assert ta >= prev_done - 1e-6  # admitted only after the previous one finished
+        prev_done = td
+        assert_chunks_equal(chunks, s20.verify_range(lo, hi, 20).chunks, f"[{lo:#x},{hi:#x})")
+    srv.close()
+
+
+def test_two_servers_and_direct_calls_share_a_sieve(s20):
+    """Two servers and direct calls on one sieve handle at once (each server owns its own
+    engine; direct persistent calls serialise on the sieve's engine)."""
+    import collatz_amd as cz
+    a, b = cz.Server(s20, chunk_log2=22), cz.Server(s20, chunk_log2=22)
+    reqs = requests(9, 16)
+    ta = [a.submit(lo, hi) for lo, hi in reqs[:8]]
+    tb = [b.submit(lo, hi) for lo, hi in reqs[8:]]
+    errors = []
+
+    def direct():
+        try:
+            for lo, hi in reqs[:4]:
+                g = s20.verify_range(lo, hi, 22, form=cz.FORM_PERSISTENT)
+                assert_chunks_equal(g.chunks, s20.verify_range(lo, hi, 22).chunks)
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    th = threading.Thread(target=direct)
+    th.start()
+    for srv, ts, rs in ((a, ta, reqs[:8]), (b, tb, reqs[8:])):
+        for t, (lo, hi) in zip(ts, rs):
+            _, chunks, _ = srv.wait(t, 60000)
+            assert_chunks_equal(chunks, s20.verify_range(lo, hi, 22).chunks)
+    th.join()
+    assert not errors, errors
+    a.close()
+    b.close()
