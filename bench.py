@@ -214,7 +214,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this verifier runs on one GPU; --gpus must be 1. Ranges "
-                          "shard across GPUs with no exchange during compute (see DESIGN.md)."}))
+                          "shard across GPUs with no exchange during compute."}))
         sys.exit(2)
     import collatz_amd as cz
     from collatz_amd import workloads as W
