@@ -342,8 +342,8 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
     uint32_t pf_off = 0;
     const uint4* tile_rec = sv.rec + first;
     for (;;) {
-      // (1) Lanes without a candidate start their prefetched one.
-      if (!act && pf) {
+      // Starts the prefetched candidate of a lane without one.
+      auto start_prefetched = [&]() {
         pf = false;
         const uint32_t f = rec_f(pf_rec);
         const bool fits = cand_start<L>(c, rec_M(pf_rec), T.q3[f], false, qb_lo, qb_hi,
@@ -352,7 +352,9 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
         cidx = pf_off;
         if (fits && cand_enter<L, LOWCUT>(c, sink)) act = true;
         else push_escalate(lists, c.n_lo, c.n_hi, first + cidx, slot);
-      }
+      };
+      // (1) Lanes without a candidate start their prefetched one.
+      if (!act && pf) start_prefetched();
       // (2) Lanes without a prefetch take the next survivors of the tile.
       const uint64_t need = __ballot(!pf);
       if (need && cur < n) {
@@ -369,9 +371,10 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
         cur += take;
       }
       if (__ballot(act || pf) == 0) break;
-      // (3) Up to U checks.
-      if (act) {
-        for (uint32_t u = 0; u < U; ++u) {
+      // (3) U checks; a lane whose candidate ends before the last of them starts its
+      //     prefetched candidate at once instead of idling.
+      for (uint32_t u = 0; u < U; ++u) {
+        if (act) {
           bool esc, done;
           one_check<L, LOWCUT>(c, sink, lim, lists, p3s, esc, done);
           acc.chk += 1;
@@ -386,9 +389,9 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
             else if (done) finish<L>(c, acc, sv.k);
             else atomicAdd(lists.stall_count, 1ull);
             act = false;
-            break;
           }
         }
+        if (u + 1 < U && !act && pf) start_prefetched();
       }
     }
     DevRec r = acc_to_rec<L>(acc);
