@@ -196,7 +196,8 @@ typedef struct {
   uint32_t chunk_log2;       /* 0 = 24 */
   uint32_t cutoff_log2;      /* 0 = 187 */
   uint64_t max_device_bytes; /* admission budget for per-request buffers; 0 = 1 GiB */
-  uint64_t slice_candidates; /* target candidates per launch; 0 = 2^26 */
+  uint64_t slice_candidates; /* candidates per launch (up to 16x more while the backlog is
+                                large); 0 = 2^26 */
   uint32_t max_inflight;     /* admitted requests at once; 0 = 1024 */
   uint32_t reserved;
 } collatz_server_opts;
