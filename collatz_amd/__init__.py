@@ -33,13 +33,15 @@ EXPORTS = [
 
 class SieveParams(C.Structure):
     _fields_ = [("k", C.c_uint32), ("mod9", C.c_uint32), ("device", C.c_int32),
+                ("lookahead_bits", C.c_uint32), ("lookahead_semantics", C.c_uint32),
                 ("reserved", C.c_uint32)]
 
 
 class SieveInfo(C.Structure):
     _fields_ = [("k", C.c_uint32), ("mod9", C.c_uint32), ("survivors", C.c_uint64),
                 ("bitarray_bits", C.c_uint64), ("device_bytes", C.c_uint64),
-                ("build_ms", C.c_double)]
+                ("build_ms", C.c_double), ("lookahead_bits", C.c_uint32),
+                ("lookahead_semantics", C.c_uint32)]
 
 
 class Chunk(C.Structure):
@@ -189,15 +191,24 @@ class RangeOutput:
 
 
 class Sieve:
-    """Resident survivor list of the 2^k residue sieve (collatz_sieve_build)."""
+    """Resident survivor list of the 2^k residue sieve (collatz_sieve_build).
 
-    def __init__(self, k: int, mod9: bool = True, device: int = -1):
+    lookahead_bits = B adds the reference's Step 2 filter at depth k (a candidate is
+    iterated only if the B-step dip of its start value mod 2^B is at most 485 f - 306 k,
+    collatz_cudav7.cu:219-235, :423-480), with the BV_i compare semantics of
+    lookahead_semantics (SEM_CUDA or SEM_OPENCL)."""
+
+    def __init__(self, k: int, mod9: bool = True, device: int = -1, lookahead_bits: int = 0,
+                 lookahead_semantics: int = 0):
         L = load()
         h = C.c_void_p()
-        _check(L.collatz_sieve_build(C.byref(SieveParams(k, int(mod9), device, 0)), C.byref(h)))
+        p = SieveParams(k, int(mod9), device, lookahead_bits, lookahead_semantics, 0)
+        _check(L.collatz_sieve_build(C.byref(p), C.byref(h)))
         self._h = h
         self.k = k
         self.mod9 = mod9
+        self.lookahead_bits = lookahead_bits
+        self.lookahead_semantics = lookahead_semantics
 
     def close(self):
         if self._h:

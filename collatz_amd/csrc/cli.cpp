@@ -9,7 +9,9 @@
 // "Spot checks" is always 0.
 //
 //   collatz_run --range LO HI [--k 32] [--chunk 24] [--cutoff 187] [--form plain|persistent]
-// verifies [LO, HI) in range mode and prints one line per chunk and a total line.
+//               [--lookahead B [--semantics cuda|opencl]]
+// verifies [LO, HI) in range mode and prints one line per chunk and a total line; with
+// --lookahead the sieve carries the Step 2 filter of B steps.
 #include <chrono>
 #include <cinttypes>
 #include <cstdio>
@@ -61,12 +63,13 @@ static int usage() {
                "usage: collatz_run [--N n] [--A a] [--B b] [--cutoff c] [--semantics cuda|opencl|rust]"
                " < case line\n"
                "       collatz_run --range LO HI [--k k] [--chunk c] [--cutoff c] [--limbs 2|3]"
-               " [--form plain|persistent]\n");
+               " [--form plain|persistent] [--lookahead B]\n");
   return 2;
 }
 
 int main(int argc, char** argv) {
   uint32_t N = 45, A = 10, B = 24, cutoff = 0, k = 32, chunk = 24, limbs = 2, form = 0;
+  uint32_t lookahead = 0;
   uint32_t sem = COLLATZ_SEM_CUDA;
   bool range = false;
   u128 lo = 0, hi = 0;
@@ -84,6 +87,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(a, "--chunk")) chunk = std::atoi(next());
     else if (!std::strcmp(a, "--limbs")) limbs = std::atoi(next());
     else if (!std::strcmp(a, "--form")) form = !std::strcmp(next(), "persistent") ? 1 : 0;
+    else if (!std::strcmp(a, "--lookahead")) lookahead = std::atoi(next());
     else if (!std::strcmp(a, "--semantics")) {
       const char* v = next();
       sem = !std::strcmp(v, "rust") ? COLLATZ_SEM_RUST
@@ -97,7 +101,7 @@ int main(int argc, char** argv) {
     }
   }
   if (range) {
-    collatz_sieve_params sp = {k, 1, -1, 0};
+    collatz_sieve_params sp = {k, 1, -1, lookahead, sem, 0};
     collatz_sieve* sv = nullptr;
     if (collatz_sieve_build(&sp, &sv) != COLLATZ_OK) {
       std::fprintf(stderr, "sieve: %s\n", collatz_last_error());

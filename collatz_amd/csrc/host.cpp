@@ -303,6 +303,12 @@ int collatz_sieve_build(const collatz_sieve_params* params, collatz_sieve** out)
   if (!params || !out) return fail(COLLATZ_EINVAL, "null argument");
   *out = nullptr;
   if (params->k < 8 || params->k > 38) return fail(COLLATZ_EINVAL, "k must be in [8, 38]");
+  if (params->lookahead_bits &&
+      (params->lookahead_bits < 8 || params->lookahead_bits > 24 ||
+       (params->lookahead_semantics != COLLATZ_SEM_CUDA &&
+        params->lookahead_semantics != COLLATZ_SEM_OPENCL)))
+    return fail(COLLATZ_EINVAL,
+                "lookahead_bits must be 0 or in [8, 24], with CUDA or OpenCL semantics");
   const auto t0 = std::chrono::steady_clock::now();
   HipDeviceGuard guard(params->device);
   collatz_sieve* sv = new (std::nothrow) collatz_sieve();
@@ -369,6 +375,17 @@ int collatz_sieve_build(const collatz_sieve_params* params, collatz_sieve** out)
     delete sv;
     return fail(COLLATZ_EDEVICE, "pow3 table upload failed");
   }
+  if (params->lookahead_bits) {
+    sv->la_bits = params->lookahead_bits;
+    sv->la_sem = params->lookahead_semantics;
+    hipError_t e = sv->la.alloc(1ull << (sv->la_bits - 3));
+    if (e == hipSuccess) e = launch_la_build(sv->la_bits, sv->k, sv->la_sem, sv->la.p, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      delete sv;
+      return fail(COLLATZ_EDEVICE, std::string("look-ahead table: ") + hipGetErrorString(e));
+    }
+  }
   sv->build_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   *out = sv;
@@ -381,8 +398,11 @@ int collatz_sieve_query(const collatz_sieve* sv, collatz_sieve_info* info) {
   info->mod9 = sv->mod9;
   info->survivors = sv->count;
   info->bitarray_bits = sv->bits_bits;
-  info->device_bytes = sv->rec.bytes() + sv->bits.bytes() + sv->pow3.bytes() + sv->class_start_dev.bytes();
+  info->device_bytes = sv->rec.bytes() + sv->bits.bytes() + sv->pow3.bytes() +
+                       sv->class_start_dev.bytes() + sv->la.bytes();
   info->build_ms = sv->build_ms;
+  info->lookahead_bits = sv->la_bits;
+  info->lookahead_semantics = sv->la_sem;
   return COLLATZ_OK;
 }
 

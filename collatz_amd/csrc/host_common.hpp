@@ -164,6 +164,8 @@ struct collatz_sieve {
   cz::DevBuf<uint64_t> pow3;
   uint64_t class_start[10] = {0};       // survivors are stored class-major by r mod 9
   cz::DevBuf<uint64_t> class_start_dev;
+  uint32_t la_bits = 0, la_sem = 0;     // look-ahead filter (0 bits: none)
+  cz::DevBuf<uint32_t> la;              // its table: 2^B nibbles
   double build_ms = 0;
   // Buffers and captured graph of the persistent form, created on first use; calls that
   // share a sieve handle are serialised on engine_mu.
@@ -174,6 +176,9 @@ struct collatz_sieve {
     a.rec = rec.p;
     a.count = count;
     a.k = k;
+    a.la = la_bits ? la.p : nullptr;
+    a.la_mask = la_bits ? (1u << la_bits) - 1u : 0u;
+    a.la_emin = (306u * k + 484u) / 485u;
     return a;
   }
 };

@@ -136,6 +136,11 @@ struct SieveArrays {
   const uint4* rec;
   uint64_t count;
   uint32_t k;
+  // Look-ahead filter (null: none): nibble g per residue of number mod 2^B, keep iff
+  // f - la_emin >= g.
+  const uint32_t* la;
+  uint32_t la_mask;   // 2^B - 1
+  uint32_t la_emin;   // ceil(306 k / 485)
 };
 
 // Look-ahead tables of reference mode on the device.
@@ -200,6 +205,8 @@ hipError_t launch_reduce_slots(const DevRec* parts, const uint64_t* part_start, 
 hipError_t launch_bv_build(uint32_t B, uint32_t num_bv, int32_t first, uint32_t sem,
                            const uint32_t* magic, uint32_t* bv, uint64_t words_per_bv,
                            uint32_t mirror_words, hipStream_t s);
+hipError_t launch_la_build(uint32_t B, uint32_t k, uint32_t sem, uint32_t* table,
+                           hipStream_t s);
 hipError_t launch_leaf_bins(const uint32_t* meta, uint64_t n, uint32_t tpo,
                             unsigned long long* bins, hipStream_t s);
 hipError_t launch_ref_fused(int limbs, const RefTablesDev& T, const uint64_t* base,

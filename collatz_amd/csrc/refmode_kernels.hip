@@ -5,6 +5,8 @@
 //                    n0 < 2^B, its B-step parity word and max dip; a bit at n0 * 3^{-f} mod 2^B
 //                    in every BV_i whose cutoff it meets.  The first words of each vector are
 //                    mirrored after the end so windows never wrap.
+//   k_la_build       the same dips as a range-mode look-ahead table: per residue of number,
+//                    the least odd-step count above the sieve's minimum that keeps it.
 //   k_ref_fused      Steps 2 and 3 (collatz_cudav7.cu:219-235, :280-366): persistent waves
 //                    take tiles of leaves; each lane walks its leaf's 2^A-bit candidate mask
 //                    (BV_i window AND the mod 9 row of its base) and iterates the candidate of
@@ -30,13 +32,10 @@ __device__ __forceinline__ bool gt_sem(int32_t x, int32_t y, bool uns) {
   return uns ? ((uint32_t)x > (uint32_t)y) : (x > y);
 }
 
-__global__ __launch_bounds__(256) void k_bv_build(uint32_t B, uint32_t num_bv, int32_t first,
-                                                  uint32_t sem, const uint32_t* magic,
-                                                  uint32_t* bv, uint64_t words_per_bv,
-                                                  uint32_t mirror_words) {
-  const uint64_t n0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n0 >= (1ull << B)) return;
-  const bool rust = sem == 1, uns = sem == 2;
+// The B-step dip of residue n0 of number (collatz_cudav7.cu:436-467): the parity word of B
+// steps of the shifted map, then the running maximum of cur = 306 (steps) - 485 (odd steps)
+// with the path-merge terms (doubleeven and even-length runs of even steps).
+__device__ __forceinline__ int32_t la_dip(uint32_t n0, uint32_t B, bool rust, bool uns) {
   uint32_t upordown = 0;
   uint64_t n = n0;
   for (uint32_t i = 0; i < B; ++i) {
@@ -63,6 +62,16 @@ __global__ __launch_bounds__(256) void k_bv_build(uint32_t B, uint32_t num_bv, i
     }
     if (cur > mx) mx = cur;
   }
+  return mx;
+}
+
+__global__ __launch_bounds__(256) void k_bv_build(uint32_t B, uint32_t num_bv, int32_t first,
+                                                  uint32_t sem, const uint32_t* magic,
+                                                  uint32_t* bv, uint64_t words_per_bv,
+                                                  uint32_t mirror_words) {
+  const uint64_t n0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n0 >= (1ull << B)) return;
+  const int32_t mx = la_dip((uint32_t)n0, B, sem == 1, sem == 2);
   const uint32_t mask = (1u << B) - 1u;
   for (uint32_t a = 0; a < num_bv; ++a) {
     if (mx <= first + (int32_t)(a * kDenominator)) {
@@ -72,6 +81,27 @@ __global__ __launch_bounds__(256) void k_bv_build(uint32_t B, uint32_t num_bv, i
       if ((idx >> 5) < mirror_words) atomicOr(&v[(1u << (B - 5)) + (idx >> 5)], 1u << (idx & 31));
     }
   }
+}
+
+// Range-mode look-ahead table of a depth-k sieve: for each residue n0 < 2^B of number, the
+// least g >= 0 with 485 (e_min + g) >= dip(n0) + 306 k, where e_min = ceil(306 k / 485) is
+// the least odd-step count a survivor can have; a candidate with f odd steps is kept iff
+// f - e_min >= g.  Stored as nibbles, eight per word, clamped to 15 (for k <= 38,
+// f - e_min <= 14, so the clamp changes no decision).  One thread per word.
+__global__ __launch_bounds__(256) void k_la_build(uint32_t B, uint32_t k, uint32_t sem,
+                                                  uint32_t* table) {
+  const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= (1ull << (B - 3))) return;
+  const int32_t base = (int32_t)(kNumerator * k) -
+                       (int32_t)(kDenominator * ((kNumerator * k + kDenominator - 1) / kDenominator));
+  uint32_t word = 0;
+  for (uint32_t j = 0; j < 8; ++j) {
+    const int32_t d = la_dip((uint32_t)(w * 8 + j), B, false, sem == 2) + base;
+    uint32_t g = d <= 0 ? 0u : (uint32_t)((d + (int32_t)kDenominator - 1) / (int32_t)kDenominator);
+    if (g > 15) g = 15;
+    word |= g << (4 * j);
+  }
+  table[w] = word;
 }
 
 // Word w of a leaf's candidate mask: bit j set iff a = 32 w + j is a candidate.
@@ -417,6 +447,13 @@ hipError_t launch_bv_build(uint32_t B, uint32_t num_bv, int32_t first, uint32_t 
                            uint32_t mirror_words, hipStream_t s) {
   hipLaunchKernelGGL(k_bv_build, dim3(blocks_for(1ull << B, 256)), dim3(256), 0, s, B, num_bv,
                      first, sem, magic, bv, words_per_bv, mirror_words);
+  return hipGetLastError();
+}
+
+hipError_t launch_la_build(uint32_t B, uint32_t k, uint32_t sem, uint32_t* table,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_la_build, dim3(blocks_for(1ull << (B - 3), 256)), dim3(256), 0, s, B, k,
+                     sem, table);
   return hipGetLastError();
 }
 

@@ -23,6 +23,20 @@ __device__ __forceinline__ void q_times_pow3(uint64_t q_lo, uint64_t q_hi, uint6
   out6[5] = (uint32_t)(b >> 96);
 }
 
+// Look-ahead filter of a sieve that carries one (collatz_cudav7.cu:219-235 at depth k):
+// keep the candidate iff the dip of number mod 2^B is at most 485 f - 306 k, read from the
+// nibble table as f - e_min >= g.  num_lo is the low word of the start value number.
+__device__ __forceinline__ bool la_keep(const SieveArrays& sv, uint32_t f, uint32_t num_lo) {
+  const uint32_t idx = num_lo & sv.la_mask;
+  const uint32_t g = (sv.la[idx >> 3] >> ((idx & 7u) << 2)) & 15u;
+  return f - sv.la_emin >= g;
+}
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
 // State of one candidate in flight.
 template <int L>
 struct Cand {
@@ -221,6 +235,7 @@ __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowD
     const uint64_t M = rec_M(sr);
     Cand<L> c;
     const uint32_t f = rec_f(sr);
+    if (sv.la && !la_keep(sv, f, (uint32_t)M + q3[f][0])) continue;
     c.slot = row.slot;
     uint32_t bigw = 0;
 #pragma unroll
@@ -282,22 +297,33 @@ constexpr int kWavesPerBlock = 4;
 // q * 3^f for f < 41 on L words.  Where the product does not fit L words the row is all
 // ones: adding any start value (>= 1) then carries out, which marks the candidate for the
 // three-limb pass without a separate flag.
-template <int L>
+template <int L, bool LA>
 struct WaveTables {
   uint32_t q3[41][L];
+  uint32_t q3lo[LA ? 41 : 1];  // low word of q * 3^f (the look-ahead index), also where q3
+                               // is all ones
 };
 
-template <int L, bool LOWCUT>
+// Look-ahead form: a wave reads its tile 64 survivors at a time (one coalesced window),
+// tests them against the look-ahead table and keeps the passing ones, in order, in LDS;
+// lanes that need a candidate take them from there.
+struct LaStage {
+  uint4 rec[64];
+  uint32_t off[64];
+};
+
+template <int L, bool LOWCUT, bool LA>
 __global__ __launch_bounds__(256) void k_verify_persistent(
     SieveArrays sv, const RowDesc* rows, const TileDesc* tiles, Ctl* ctl, Limits lim,
     const uint64_t* pow3, DevRec* tile_recs, DeviceLists lists, uint32_t U) {
-  __shared__ WaveTables<L> tabs[kWavesPerBlock];
+  __shared__ WaveTables<L, LA> tabs[kWavesPerBlock];
+  __shared__ LaStage stages[LA ? kWavesPerBlock : 1];
   __shared__ Pow3Lds p3s;
   stage_pow3(p3s);
   __syncthreads();
   const uint32_t wv = threadIdx.x >> 6;
   const uint32_t lane = threadIdx.x & 63;
-  WaveTables<L>& T = tabs[wv];
+  WaveTables<L, LA>& T = tabs[wv];
   const uint32_t ntiles = ctl->ntiles;
   uint32_t* tile_counter = &ctl->tile_counter;
 
@@ -322,6 +348,7 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
       for (int i = L; i < 6; ++i) big |= w6[i];
 #pragma unroll
       for (int i = 0; i < L; ++i) T.q3[lane][i] = big ? 0xFFFFFFFFu : w6[i];
+      if constexpr (LA) T.q3lo[lane] = w6[0];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -341,6 +368,8 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
     uint4 pf_rec = make_uint4(0, 0, 0, 0);
     uint32_t pf_off = 0;
     const uint4* tile_rec = sv.rec + first;
+    // Look-ahead stage: entries [shead, scnt) are kept survivors not yet taken.
+    uint32_t scnt = 0, shead = 0;
     for (;;) {
       // Starts the prefetched candidate of a lane without one.
       auto start_prefetched = [&]() {
@@ -357,12 +386,54 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
       if (!act && pf) start_prefetched();
       // (2) Lanes without a prefetch take the next survivors of the tile.
       const uint64_t need = __ballot(!pf);
-      if (need && cur < n) {
+      if constexpr (LA) {
+        // ... the next kept ones, from the stage, refilled window by window.
+        if (need) {
+          LaStage& S = stages[wv];
+          uint32_t want = __popcll(need);
+          uint32_t rank = lane_rank(need);
+          for (;;) {
+            const uint32_t avail = scnt - shead;
+            const uint32_t take = avail < want ? avail : want;
+            if (!pf) {
+              if (rank < take) {
+                pf_rec = S.rec[shead + rank];
+                pf_off = S.off[shead + rank];
+                pf = true;
+              } else {
+                rank -= take;
+              }
+            }
+            shead += take;
+            want -= take;
+            if (want == 0 || cur >= n) break;
+            const uint32_t o = cur + lane;
+            bool keep = false;
+            uint4 wr = make_uint4(0, 0, 0, 0);
+            if (o < n) {
+              wr = tile_rec[o];
+              const uint32_t f = rec_f(wr);
+              keep = la_keep(sv, f, wr.z + T.q3lo[f]);
+            }
+            const uint64_t km = __ballot(keep);
+            if (keep) {
+              const uint32_t p = lane_rank(km);
+              S.rec[p] = wr;
+              S.off[p] = o;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            scnt = __popcll(km);
+            shead = 0;
+            cur += 64;
+          }
+        }
+      } else if (need && cur < n) {
         const uint32_t want = __popcll(need);
         const uint32_t avail = n - cur;
         const uint32_t take = avail < want ? avail : want;
-        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-            (uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+        const uint32_t rank = lane_rank(need);
         if (!pf && rank < take) {
           pf_off = cur + rank;
           pf_rec = tile_rec[pf_off];
@@ -441,26 +512,39 @@ hipError_t launch_verify_plain(int limbs, const SieveArrays& sv, const RowDesc* 
 }
 
 
+template <bool LA>
+static void launch_persistent_la(int limbs, const SieveArrays& sv, const RowDesc* rows,
+                                 const TileDesc* tiles, Ctl* ctl, uint32_t nblocks, uint32_t U,
+                                 const Limits& lim, const uint64_t* pow3, DevRec* tile_recs,
+                                 DeviceLists lists, hipStream_t s) {
+  const dim3 grid(nblocks), block(256);
+  if (limbs == 1)
+    hipLaunchKernelGGL((k_verify_persistent<3, false, LA>), grid, block, 0, s, sv, rows, tiles,
+                       ctl, lim, pow3, tile_recs, lists, U);
+  else if (limbs == 2 && lim.cutoff >= kWideLog2)
+    hipLaunchKernelGGL((k_verify_persistent<4, false, LA>), grid, block, 0, s, sv, rows, tiles,
+                       ctl, lim, pow3, tile_recs, lists, U);
+  else if (limbs == 2)
+    hipLaunchKernelGGL((k_verify_persistent<4, true, LA>), grid, block, 0, s, sv, rows, tiles,
+                       ctl, lim, pow3, tile_recs, lists, U);
+  else if (lim.cutoff >= 128)
+    hipLaunchKernelGGL((k_verify_persistent<6, false, LA>), grid, block, 0, s, sv, rows, tiles,
+                       ctl, lim, pow3, tile_recs, lists, U);
+  else
+    hipLaunchKernelGGL((k_verify_persistent<6, true, LA>), grid, block, 0, s, sv, rows, tiles,
+                       ctl, lim, pow3, tile_recs, lists, U);
+}
+
 hipError_t launch_verify_persistent(int limbs, const SieveArrays& sv, const RowDesc* rows,
                                     const TileDesc* tiles, Ctl* ctl, uint32_t nblocks,
                                     uint32_t U, const Limits& lim, const uint64_t* pow3,
                                     DevRec* tile_recs, DeviceLists lists, hipStream_t s) {
-  const dim3 grid(nblocks), block(256);
-  if (limbs == 1)
-    hipLaunchKernelGGL((k_verify_persistent<3, false>), grid, block, 0, s, sv, rows, tiles, ctl,
-                       lim, pow3, tile_recs, lists, U);
-  else if (limbs == 2 && lim.cutoff >= kWideLog2)
-    hipLaunchKernelGGL((k_verify_persistent<4, false>), grid, block, 0, s, sv, rows, tiles, ctl,
-                       lim, pow3, tile_recs, lists, U);
-  else if (limbs == 2)
-    hipLaunchKernelGGL((k_verify_persistent<4, true>), grid, block, 0, s, sv, rows, tiles, ctl,
-                       lim, pow3, tile_recs, lists, U);
-  else if (lim.cutoff >= 128)
-    hipLaunchKernelGGL((k_verify_persistent<6, false>), grid, block, 0, s, sv, rows, tiles, ctl,
-                       lim, pow3, tile_recs, lists, U);
+  if (sv.la)
+    launch_persistent_la<true>(limbs, sv, rows, tiles, ctl, nblocks, U, lim, pow3, tile_recs,
+                               lists, s);
   else
-    hipLaunchKernelGGL((k_verify_persistent<6, true>), grid, block, 0, s, sv, rows, tiles, ctl,
-                       lim, pow3, tile_recs, lists, U);
+    launch_persistent_la<false>(limbs, sv, rows, tiles, ctl, nblocks, U, lim, pow3, tile_recs,
+                                lists, s);
   return hipGetLastError();
 }
 

@@ -44,13 +44,23 @@ enum collatz_flags {
  * Replaces the reference's Step 1: search1() from the root "3 2 2 9 1 0" down to depth k
  * (collatz_cudav7.cu:497-563, :173), evaluated on the device for every residue class,
  * stored as a bit array and compacted into a sorted survivor list (r, T^k(r)+1, f) that
- * stays resident in device memory. */
+ * stays resident in device memory.
+ *
+ * With lookahead_bits = B the sieve also carries the reference's Step 2 filter
+ * (precompute() BV_i, collatz_cudav7.cu:423-480, applied at :219-235) at depth k: a
+ * candidate n with survivor (r, f) and start value number = T^k(n)+1 is iterated only if
+ * the B-step dip of number mod 2^B (the BV_i "maxval") is at most 485 f - 306 k.  At
+ * k = N - A over [1, 2^N) this is exactly the reference's candidate set.  Every
+ * verification with such a sieve applies it; counts and checksums cover the candidates it
+ * keeps. */
 typedef struct collatz_sieve collatz_sieve;
 
 typedef struct {
   uint32_t k;       /* sieve depth: residues mod 2^k, 8 <= k <= 38 */
   uint32_t mod9;    /* 1: apply the mod 9 preimage sieve (collatz_cudav7.cu:482-494) */
   int32_t device;   /* HIP device ordinal, -1 = current device */
+  uint32_t lookahead_bits;      /* 0 = none, or B in [8, 24]: a B-step look-ahead filter */
+  uint32_t lookahead_semantics; /* COLLATZ_SEM_CUDA or COLLATZ_SEM_OPENCL */
   uint32_t reserved;
 } collatz_sieve_params;
 
@@ -60,6 +70,7 @@ typedef struct {
   uint64_t bitarray_bits;  /* bits of the dense array (classes r = 3 mod 4) */
   uint64_t device_bytes;   /* device memory held by the handle */
   double build_ms;         /* wall time of collatz_sieve_build */
+  uint32_t lookahead_bits, lookahead_semantics;
 } collatz_sieve_info;
 
 int collatz_sieve_build(const collatz_sieve_params* params, collatz_sieve** out);

@@ -116,6 +116,12 @@ def lib():
         L.or_verify_range.argtypes = [C.c_uint32, C.c_int64, _u64p, _u64p, _P(C.c_uint8),
                                       C.c_int, _u64p, _u64p, C.c_uint32, C.c_uint32, C.c_int,
                                       _P(Chunk), _P(Chunk)]
+        L.or_verify_range_la.argtypes = [C.c_uint32, C.c_int64, _u64p, _u64p,
+                                         _P(C.c_uint8), C.c_int, C.c_uint32, C.c_uint32,
+                                         _u64p, _u64p, C.c_uint32, C.c_uint32, C.c_int,
+                                         _P(Chunk), _P(Chunk)]
+        L.or_la_dip.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
+        L.or_la_dip.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -217,17 +223,23 @@ def nchunks(lo, hi, C_log2):
     return ((hi - 1) >> C_log2) - (lo >> C_log2) + 1
 
 
+def la_dip(n0: int, B: int, semantics: int = SEM_CXX) -> int:
+    return lib().or_la_dip(n0, B, semantics)
+
+
 def verify_range(sv: Sieve, lo: int, hi: int, chunk_log2: int, cutoff: int = 187,
-                 mod9: bool = True, threads: int = 1):
+                 mod9: bool = True, threads: int = 1, lookahead_bits: int = 0,
+                 lookahead_semantics: int = SEM_CXX):
     n = nchunks(lo, hi, chunk_log2)
     chunks = np.zeros(n, CHUNK_DTYPE)
     total = Chunk()
     lo2 = (C.c_uint64 * 2)(lo & MASK64, lo >> 64)
     hi2 = (C.c_uint64 * 2)(hi & MASK64, hi >> 64)
-    rc = lib().or_verify_range(sv.k, len(sv.r), _ptr(sv.r, C.c_uint64),
-                               _ptr(sv.number, C.c_uint64), _ptr(sv.f, C.c_uint8), int(mod9),
-                               lo2, hi2, chunk_log2, cutoff, threads,
-                               chunks.ctypes.data_as(_P(Chunk)), C.byref(total))
+    rc = lib().or_verify_range_la(sv.k, len(sv.r), _ptr(sv.r, C.c_uint64),
+                                  _ptr(sv.number, C.c_uint64), _ptr(sv.f, C.c_uint8),
+                                  int(mod9), lookahead_bits, lookahead_semantics, lo2, hi2,
+                                  chunk_log2, cutoff, threads,
+                                  chunks.ctypes.data_as(_P(Chunk)), C.byref(total))
     if rc != 0:
         raise ValueError("or_verify_range failed")
     tot = np.frombuffer(bytearray(total), CHUNK_DTYPE)[0]

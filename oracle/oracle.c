@@ -207,44 +207,53 @@ static inline int gt_sem(int32_t x, int32_t y, int uns) {
   return uns ? ((uint32_t)x > (uint32_t)y) : (x > y);
 }
 
+/* The B-step dip of one residue n0 of number (collatz_cudav7.cu:436-467): B steps of the
+ * shifted map on n0 give the parity word; then, step by step, cur = 306 (steps) - 485 (odd
+ * steps) and its running maximum with the path-merge terms.  Only the low B bits of number
+ * decide the first B steps, so n0 = number mod 2^B. */
+int32_t or_la_dip(uint64_t n0, uint32_t B, uint32_t semantics) {
+  const int uns = semantics == OR_SEM_OPENCL;
+  uint32_t upordown = 0;
+  uint64_t n = n0;
+  for (uint32_t i = 0; i < B; ++i) {
+    if (n % 2 == 0) {
+      n = n + n / 2;
+      upordown |= 1u << i;
+    } else {
+      n = n / 2 + 1;
+    }
+  }
+  int32_t cur = 0, mx = 0;
+  uint32_t ne = 0;
+  for (uint32_t i = 0; i < B; ++i) {
+    if (upordown & (1u << i)) {
+      if (gt_sem(cur + (int32_t)OR_NUMERATOR, mx, uns) &&
+          or_doubleeven(upordown, i, B, semantics))
+        mx = cur + (int32_t)OR_NUMERATOR;
+      cur -= (int32_t)(OR_DENOMINATOR - OR_NUMERATOR);
+      ne = 0;
+    } else {
+      cur += (int32_t)OR_NUMERATOR;
+      ne += 1;
+      if (ne % 2 == 0 && gt_sem(cur + (int32_t)(OR_DENOMINATOR - OR_NUMERATOR), mx, uns))
+        mx = cur + (int32_t)(OR_DENOMINATOR - OR_NUMERATOR);
+    }
+    if (cur > mx) mx = cur;
+  }
+  return mx;
+}
+
 int or_precompute_bv(const or_knobs* kn, uint64_t* bv) {
   if (!kn || !bv || kn->B < 6 || kn->B > 30 || kn->A > kn->N) return -1;
   const uint32_t B = kn->B, mbi = kn->N - kn->A, tpo = or_tpo(mbi);
   const int32_t first = (int32_t)(int16_t)(OR_DENOMINATOR * tpo - OR_NUMERATOR * mbi);
   const uint64_t words = 1ull << (B - 6);
   const uint32_t mask = (B >= 32) ? 0xFFFFFFFFu : ((1u << B) - 1u);
-  const int uns = kn->semantics == OR_SEM_OPENCL;
   uint32_t magic[64];
   for (uint32_t a = 0; a < kn->num_bv && a < 64; ++a) magic[a] = or_magic(tpo + a);
   memset(bv, 0, (size_t)kn->num_bv * words * 8);
   for (uint64_t n0 = 0; n0 < (1ull << B); ++n0) {
-    uint32_t upordown = 0;
-    uint64_t n = n0;
-    for (uint32_t i = 0; i < B; ++i) {
-      if (n % 2 == 0) {
-        n = n + n / 2;
-        upordown |= 1u << i;
-      } else {
-        n = n / 2 + 1;
-      }
-    }
-    int32_t cur = 0, mx = 0;
-    uint32_t ne = 0;
-    for (uint32_t i = 0; i < B; ++i) {
-      if (upordown & (1u << i)) {
-        if (gt_sem(cur + (int32_t)OR_NUMERATOR, mx, uns) &&
-            or_doubleeven(upordown, i, B, kn->semantics))
-          mx = cur + (int32_t)OR_NUMERATOR;
-        cur -= (int32_t)(OR_DENOMINATOR - OR_NUMERATOR);
-        ne = 0;
-      } else {
-        cur += (int32_t)OR_NUMERATOR;
-        ne += 1;
-        if (ne % 2 == 0 && gt_sem(cur + (int32_t)(OR_DENOMINATOR - OR_NUMERATOR), mx, uns))
-          mx = cur + (int32_t)(OR_DENOMINATOR - OR_NUMERATOR);
-      }
-      if (cur > mx) mx = cur;
-    }
+    const int32_t mx = or_la_dip(n0, B, kn->semantics);
     for (uint32_t a = 0; a < kn->num_bv; ++a) {
       if (mx <= first + (int32_t)(a * OR_DENOMINATOR)) {
         uint32_t idx = ((uint32_t)n0 * magic[a]) & mask;
@@ -532,6 +541,7 @@ static int64_t lower_bound_u64(const uint64_t* a, int64_t n, uint64_t key) {
 typedef struct {
   uint32_t k, C, cutoff;
   int mod9;
+  uint32_t la_bits, la_sem;
   int64_t nres;
   const uint64_t* r;
   const uint64_t* number;
@@ -581,6 +591,12 @@ static void range_chunk(range_ctx* c, uint64_t ci, or_chunk* out) {
       s = (u128)v.w[1] + (uint64_t)(s >> 64);
       v.w[1] = (uint64_t)s;
       v.w[2] += (uint64_t)(s >> 64);
+      /* Step 2 at depth k (collatz_cudav7.cu:219-235 with BV_i of :423-480): keep the
+       * candidate iff the dip of number mod 2^B is at most 485 f - 306 k. */
+      if (c->la_bits) {
+        const int32_t dip = or_la_dip(v.w[0] & ((1ull << c->la_bits) - 1), c->la_bits, c->la_sem);
+        if (dip > (int32_t)(OR_DENOMINATOR * c->f[i]) - (int32_t)(OR_NUMERATOR * c->k)) continue;
+      }
       const uint64_t b2[2] = {(uint64_t)n, (uint64_t)(n >> 64)};
       or_traj_out o;
       or_traj(b2, v.w, c->cutoff, 0, &o);
@@ -623,7 +639,16 @@ int or_verify_range(uint32_t k, int64_t nres, const uint64_t* r, const uint64_t*
                     const uint8_t* f, int mod9, const uint64_t lo[2], const uint64_t hi[2],
                     uint32_t C, uint32_t cutoff, int threads, or_chunk* chunks,
                     or_chunk* total) {
+  return or_verify_range_la(k, nres, r, number, f, mod9, 0, 0, lo, hi, C, cutoff, threads,
+                            chunks, total);
+}
+
+int or_verify_range_la(uint32_t k, int64_t nres, const uint64_t* r, const uint64_t* number,
+                       const uint8_t* f, int mod9, uint32_t la_bits, uint32_t la_sem,
+                       const uint64_t lo[2], const uint64_t hi[2], uint32_t C,
+                       uint32_t cutoff, int threads, or_chunk* chunks, or_chunk* total) {
   if (k < 2 || k > 40 || C > 100 || !chunks || !total || nres < 0) return -1;
+  if (la_bits && (la_bits < 1 || la_bits > 30 || la_sem == OR_SEM_RUST)) return -1;
   const u128 L = mk128(lo[0], lo[1]), H = mk128(hi[0], hi[1]);
   if (L >= H) return -1;
   range_ctx* c = (range_ctx*)calloc(1, sizeof(range_ctx));
@@ -632,6 +657,8 @@ int or_verify_range(uint32_t k, int64_t nres, const uint64_t* r, const uint64_t*
   c->C = C;
   c->cutoff = cutoff;
   c->mod9 = mod9;
+  c->la_bits = la_bits;
+  c->la_sem = la_sem;
   c->nres = nres;
   c->r = r;
   c->number = number;

@@ -20,6 +20,10 @@
  *                      naive step-by-step T iteration with Python integers.
  *   or_precompute_bv   against the compiled reference precompute (C++ semantics, N=45),
  *                      and against the paper's Fig 4.1 popcounts (Rust semantics, N=72).
+ *   or_la_dip          through or_precompute_bv, which is built on it.
+ *   or_verify_range_la over [1, 2^N) at k = N - A: the same candidate set, count and hash
+ *                      as the reference-mode run of the whole tree (or_case_run on the
+ *                      root), which is pinned to the compiled reference.
  *   or_mod9_rows       against the compiled reference, and against n mod 9 computed
  *                      directly.
  *   or_search1         against the compiled reference search1 leaves (case lines, N=45),
@@ -103,6 +107,9 @@ int64_t or_sieve_sorted(uint32_t k, uint64_t* r, uint64_t* number, uint8_t* f, i
 /* --- precompute -------------------------------------------------------------- */
 /* bv: num_bv arrays of 2^B bits each, as 2^(B-6) u64 words per array (logical bit order:
  * bit j of array i is word j/64, bit j%64). */
+/* B-step dip ("maxval") of residue n0 of number, as precompute() computes it per residue
+ * (collatz_cudav7.cu:436-467; unsigned compares for OR_SEM_OPENCL). */
+int32_t or_la_dip(uint64_t n0, uint32_t B, uint32_t semantics);
 int or_precompute_bv(const or_knobs* kn, uint64_t* bv);
 /* rows: 9 rows of 2^A bits as 2^(A-6) (at least 1) u64 words per row. */
 int or_mod9_rows(uint32_t mbi, uint32_t A, uint64_t* rows);
@@ -143,6 +150,14 @@ int or_verify_range(uint32_t k, int64_t nres, const uint64_t* r, const uint64_t*
                     const uint8_t* f, int mod9, const uint64_t lo[2], const uint64_t hi[2],
                     uint32_t C, uint32_t cutoff, int threads, or_chunk* chunks,
                     or_chunk* total);
+/* The same with the Step 2 look-ahead at depth k (la_bits = B > 0): a candidate with
+ * survivor (r, f) and start value number is iterated only if
+ * or_la_dip(number mod 2^B, B, la_sem) <= 485 f - 306 k.  la_sem: OR_SEM_CXX or
+ * OR_SEM_OPENCL. */
+int or_verify_range_la(uint32_t k, int64_t nres, const uint64_t* r, const uint64_t* number,
+                       const uint8_t* f, int mod9, uint32_t la_bits, uint32_t la_sem,
+                       const uint64_t lo[2], const uint64_t hi[2], uint32_t C,
+                       uint32_t cutoff, int threads, or_chunk* chunks, or_chunk* total);
 void or_chunk_merge(or_chunk* into, const or_chunk* from);
 
 #ifdef __cplusplus

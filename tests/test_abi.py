@@ -49,5 +49,5 @@ def test_null_arguments_are_rejected():
 def test_struct_layouts_match_header():
     import collatz_amd as cz
     assert ctypes.sizeof(cz.Chunk) == 11 * 8 + 8
-    assert ctypes.sizeof(cz.SieveParams) == 16
+    assert ctypes.sizeof(cz.SieveParams) == 24
     assert ctypes.sizeof(cz.Interval) == 32
