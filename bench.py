@@ -39,6 +39,7 @@ sys.path.insert(0, ROOT)
 METRIC = "verified n/sec on 1 GPU (batched/served vs plain, bit-exact); p50/p99 latency"
 K = 32
 BATCH_CHUNK = 32
+LOOKAHEAD_BITS = 16
 RANGE_LO, RANGE_WIDTH, RANGE_CHUNK = 1 << 40, 1 << 36, 24
 # Algorithmic integer work, in 32-bit lane operations, for L 32-bit words per value: per
 # check (two odd and two even runs, hash and compare) OPS_DR = 13 L + 7, and per candidate
@@ -191,6 +192,7 @@ def main():
     ap.add_argument("--no-range", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-lookahead", action="store_true")
     ap.add_argument("--no-tree", action="store_true")
     ap.add_argument("--tree-n", type=int, default=50,
                     help="MAXBITS of the reference-program leg (the whole tree below 2^N)")
@@ -344,6 +346,32 @@ def main():
             "roofline_frac": ach / VALU_PEAK_TOPS}
         log(f"large sieve k=37: {line['large_sieve_k37']}")
         del s37
+
+    if not args.no_lookahead:
+        # The same batch with the Step 2 look-ahead of B = LOOKAHEAD_BITS steps at depth 32:
+        # fewer candidates iterated (different counts and checksums by definition), the
+        # same integers verified.
+        sla = cz.Sieve(K, lookahead_bits=LOOKAHEAD_BITS)
+        sla.verify_batch(ivs, chunk_log2=BATCH_CHUNK, use_graph=True)
+        ms = []
+        device_sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            o = sla.verify_batch(ivs, chunk_log2=BATCH_CHUNK, use_graph=True)
+            ms.append(o[0].device_ms)
+        device_sync()
+        d = (time.perf_counter() - t0) / args.steps
+        lah = 0
+        for x in o:
+            lah ^= int(x.total["hash"])
+        line["lookahead"] = {
+            "bits": LOOKAHEAD_BITS, "semantics": "cuda", "n_per_s": total_n / d,
+            "ms_per_step": d * 1e3, "kernel_ms": sum(ms) / len(ms),
+            "candidates": sum(int(x.total["count"]) for x in o),
+            "checks": sum(int(x.checks) for x in o), "total_hash": "%08X" % lah,
+            "vs_headline": (total_n / d) / value}
+        log(f"look-ahead B={LOOKAHEAD_BITS}: {line['lookahead']}")
+        sla.close()
 
     if not args.no_tree:
         line["reference_tree_mode"] = tree_leg(cz, args.tree_n)

@@ -159,8 +159,10 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   key.bufs[7] = (const void*)(uintptr_t)ovf_cap;
   key.limbs = limbs;
   key.cutoff = cutoff;
+  const uint32_t waves =
+      std::min<uint32_t>(tn.waves, persistent_waves_fit(limbs, cutoff, sv_->la_bits != 0));
   key.nblocks =
-      (uint32_t)std::max<uint64_t>(1, (uint64_t)device_cu_count(sv_->device) * tn.waves / 4);
+      (uint32_t)std::max<uint64_t>(1, (uint64_t)device_cu_count(sv_->device) * waves / 4);
   key.U = tn.U;
 
   CZ_HIP(hipEventRecord(e0_, s));

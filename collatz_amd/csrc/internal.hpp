@@ -195,6 +195,8 @@ hipError_t launch_verify_persistent(int limbs, const SieveArrays& sv, const RowD
                                     DevRec* tile_recs, DeviceLists lists, hipStream_t s);
 // Clears the tile counter and the list counters with a kernel (no memset nodes, so the
 // sequence can be graph-captured).
+// Waves per CU that stay resident for the persistent kernel of this width and cutoff.
+uint32_t persistent_waves_fit(int limbs, uint32_t cutoff, bool la);
 hipError_t launch_engine_reset(Ctl* ctl, unsigned long long* cnt, uint32_t ncnt, hipStream_t s);
 // Expands rows into tiles: row r owns tiles [rows[r].block_start, rows[r+1].block_start)
 // (the last row up to ctl->ntiles).

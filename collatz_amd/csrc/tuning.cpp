@@ -33,10 +33,11 @@ static const uint32_t kDefaultWaves = 32, kDefaultU = 2;
 
 // Survivors per tile: the largest power of two in [4096, 65536] that still gives every
 // resident wave about 16 tiles, so that work stealing evens out the tail while the per-tile
-// set-up (the q * 3^f table, the record merge) stays small.
+// set-up (the q * 3^f table, the record merge) stays small.  A look-ahead keeps about a
+// third of the survivors, so tiles of such a sieve may be four times larger.
 uint32_t adaptive_tile(const collatz_sieve* sv, uint64_t candidates, uint32_t waves) {
   const uint64_t nwaves = (uint64_t)device_cu_count(sv->device) * waves;
-  uint32_t t = 65536;
+  uint32_t t = sv->la_bits ? 262144 : 65536;
   while (t > 4096 && candidates / t < 16 * nwaves) t >>= 1;
   return t;
 }
@@ -64,10 +65,14 @@ Tuning choose_tuning(const collatz_sieve* sv, int limbs, u128 max_hi, uint64_t c
       const u128 lo = (u128)1 << (bits - 1);
       const u128 hi = lo + width;
       double best = 1e30;
+      // A look-ahead sieve's refill is dearer (a window of the tile is tested first), so
+      // its grid goes one check further between refills.
       static const uint32_t wavess[] = {24, 28, 32};
-      static const uint32_t Us[] = {1, 2, 3};
+      static const uint32_t Us[] = {1, 2, 3, 4};
+      const int nU = sv->la_bits ? 4 : 3;
       for (uint32_t b : wavess)
-        for (uint32_t c : Us) {
+        for (int ui = 0; ui < nU; ++ui) {
+          const uint32_t c = Us[ui];
           const Tuning cand{adaptive_tile(sv, 1u << 27, b), b, c};
           double ms = 1e30;
           for (int rep = 0; rep < 2; ++rep) {

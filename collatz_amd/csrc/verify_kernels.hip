@@ -5,6 +5,8 @@
 // number = T^k(r) + 1 + q * 3^f (Lemma 2.3, paper p.4; the jump of
 // collatz_cudav7.cu:262-278), and n mod 9 must be in {0,1,3,6,7}
 // (collatz_cudav7.cu:482-494).  The schedule is in schedule.hpp.
+#include <atomic>
+
 #include "schedule.hpp"
 
 namespace cz {
@@ -304,9 +306,10 @@ struct WaveTables {
                                // is all ones
 };
 
-// Look-ahead form: a wave reads its tile 64 survivors at a time (one coalesced window),
-// tests them against the look-ahead table and keeps the passing ones, in order, in LDS;
-// lanes that need a candidate take them from there.
+// Look-ahead form: a wave reads its tile 64 survivors at a time (one coalesced window,
+// loaded one window ahead), tests them against the look-ahead table and keeps the passing
+// records and their offsets, in order, in LDS; lanes that need a candidate take the next
+// one from there.
 struct LaStage {
   uint4 rec[64];
   uint32_t off[64];
@@ -368,8 +371,13 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
     uint4 pf_rec = make_uint4(0, 0, 0, 0);
     uint32_t pf_off = 0;
     const uint4* tile_rec = sv.rec + first;
-    // Look-ahead stage: entries [shead, scnt) are kept survivors not yet taken.
+    // Look-ahead stage: entries [shead, scnt) are kept survivors not yet taken; nx holds the
+    // records of the next window (survivors [cur, cur + 64)), loaded one window ahead.
     uint32_t scnt = 0, shead = 0;
+    uint4 nx = make_uint4(0, 0, 0, 0);
+    if constexpr (LA) {
+      if (lane < n) nx = tile_rec[lane];
+    }
     for (;;) {
       // Starts the prefetched candidate of a lane without one.
       auto start_prefetched = [&]() {
@@ -409,21 +417,21 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
             if (want == 0 || cur >= n) break;
             const uint32_t o = cur + lane;
             bool keep = false;
-            uint4 wr = make_uint4(0, 0, 0, 0);
             if (o < n) {
-              wr = tile_rec[o];
-              const uint32_t f = rec_f(wr);
-              keep = la_keep(sv, f, wr.z + T.q3lo[f]);
+              const uint32_t f = rec_f(nx);
+              keep = la_keep(sv, f, nx.z + T.q3lo[f]);
             }
             const uint64_t km = __ballot(keep);
             if (keep) {
               const uint32_t p = lane_rank(km);
-              S.rec[p] = wr;
+              S.rec[p] = nx;
               S.off[p] = o;
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // A wave's LDS operations execute in order, so other lanes' reads after this
+            // point see the writes; only the compiler must keep them in order.  (A memory
+            // fence would also wait for loads in flight.)
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            if (o + 64 < n) nx = tile_rec[o + 64];
             scnt = __popcll(km);
             shead = 0;
             cur += 64;
@@ -511,6 +519,37 @@ hipError_t launch_verify_plain(int limbs, const SieveArrays& sv, const RowDesc* 
   return hipGetLastError();
 }
 
+
+// Resident waves per CU of the persistent kernel that a launch with these arguments runs
+// (its register use decides: a launch with more waves than fit leaves the extra waves to
+// start late, as a tail).
+template <int L, bool LOWCUT, bool LA>
+static uint32_t waves_fit() {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_verify_persistent<L, LOWCUT, LA>, 256,
+                                                   0) != hipSuccess || nb <= 0)
+    return 32;
+  return (uint32_t)nb * kWavesPerBlock;
+}
+
+uint32_t persistent_waves_fit(int limbs, uint32_t cutoff, bool la) {
+  static std::atomic<uint32_t> cache[2][5];
+  const int w = limbs == 1 ? 0 : limbs == 2 ? (cutoff >= kWideLog2 ? 1 : 2) : (cutoff >= 128 ? 3 : 4);
+  std::atomic<uint32_t>& c = cache[la][w];
+  if (const uint32_t v = c.load(std::memory_order_relaxed)) return v;
+  uint32_t v;
+  if (la) {
+    v = w == 0 ? waves_fit<3, false, true>() : w == 1 ? waves_fit<4, false, true>()
+        : w == 2 ? waves_fit<4, true, true>() : w == 3 ? waves_fit<6, false, true>()
+        : waves_fit<6, true, true>();
+  } else {
+    v = w == 0 ? waves_fit<3, false, false>() : w == 1 ? waves_fit<4, false, false>()
+        : w == 2 ? waves_fit<4, true, false>() : w == 3 ? waves_fit<6, false, false>()
+        : waves_fit<6, true, false>();
+  }
+  c.store(v, std::memory_order_relaxed);
+  return v;
+}
 
 template <bool LA>
 static void launch_persistent_la(int limbs, const SieveArrays& sv, const RowDesc* rows,
