@@ -51,3 +51,15 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(cz.Chunk) == 11 * 8 + 8
     assert ctypes.sizeof(cz.SieveParams) == 24
     assert ctypes.sizeof(cz.Interval) == 32
+
+
+def test_sieve_params_are_checked_before_the_device():
+    """Bad depths and look-ahead settings are refused with EINVAL before any HIP call."""
+    import collatz_amd as cz
+    lib = cz.load()
+    h = ctypes.c_void_p()
+    for k, bits, sem in ((7, 0, 0), (39, 0, 0), (32, 7, 0), (32, 25, 0), (32, 16, cz.SEM_RUST)):
+        p = cz.SieveParams(k, 1, -1, bits, sem, 0)
+        assert lib.collatz_sieve_build(ctypes.byref(p), ctypes.byref(h)) == cz.EINVAL
+        assert b"k must" in lib.collatz_last_error() or b"lookahead" in lib.collatz_last_error()
+    assert ctypes.sizeof(cz.SieveInfo) == 48
