@@ -97,9 +97,13 @@ def records_equal(a, b) -> bool:
     return a.shape == b.shape and a.tobytes() == b.tobytes()
 
 
+CPU_CHUNK = 26
+
+
 def cpu_baseline(intervals, threads: int, n_intervals: int, width_log2: int):
     """The oracle (CPU restatement) on the first `n_intervals` intervals, each cut to its
-    first 2^width_log2 integers."""
+    first 2^width_log2 integers.  The oracle runs chunk records in parallel, so it is given
+    2^26-wide records (the work is the same; only the record granularity differs)."""
     from oracle import bind as B
     t0 = time.time()
     sv = B.sieve(K)
@@ -108,13 +112,35 @@ def cpu_baseline(intervals, threads: int, n_intervals: int, width_log2: int):
     t0 = time.time()
     for lo, hi in intervals[:n_intervals]:
         hi = min(hi, lo + (1 << width_log2))
-        B.verify_range(sv, lo, hi, BATCH_CHUNK, threads=threads)
+        B.verify_range(sv, lo, hi, CPU_CHUNK, threads=threads)
         n += hi - lo
     dt = time.time() - t0
+    # The per-core figure: the first four of the same intervals cut to 2^30, one thread.
+    n1 = 0
+    t1 = time.time()
+    for lo, hi in intervals[:4]:
+        hi = min(hi, lo + (1 << 30))
+        B.verify_range(sv, lo, hi, CPU_CHUNK, threads=1)
+        n1 += hi - lo
+    dt1 = time.time() - t1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": n / dt, "unit": "n/s", "cores": threads, "kind": "port",
             "sample": f"first {n_intervals} intervals of the batch workload, each cut to "
-                      f"2^{width_log2} integers ({n:.3e} n in {dt:.1f} s on {threads} threads;"
-                      f" oracle sieve build {build_s:.1f} s excluded, as on the GPU)"}
+                      f"2^{width_log2} integers, 2^{CPU_CHUNK}-wide records ({n:.3e} n in "
+                      f"{dt:.1f} s on {threads} threads; oracle sieve build {build_s:.1f} s "
+                      f"excluded, as on the GPU)",
+            "single_thread": {"value": n1 / dt1, "unit": "n/s",
+                              "sample": f"first 4 intervals cut to 2^30 ({n1:.3e} n in "
+                                        f"{dt1:.1f} s)"},
+            "cpu_model": model, "host_cpus": os.cpu_count()}
 
 
 def served_leg(sv, reqs, rate, chunk_log2):
@@ -200,7 +226,7 @@ def main():
                     help="run only the reference-program leg and print its record")
     ap.add_argument("--large-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-intervals", type=int, default=32)
+    ap.add_argument("--cpu-intervals", type=int, default=128)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device tensor copy three times (the "
                          "reference reading for the FETCH_SIZE / WRITE_SIZE counters)")
