@@ -258,11 +258,12 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
       // Steps 2 and 3 in one persistent launch over the leaves.
       CZ_HIP(hipMemsetAsync(&dtot.p->tile_counter, 0, 4, s));
       CZ_HIP(hipMemsetAsync(&dtot.p->esc_count, 0, 8, s));
-      // Tiles of 2048 leaves (each lane walks whole leaves, whose candidate counts vary a
-      // lot, so a tile needs many leaves per lane to balance) and four checks between
-      // refills (candidates that pass the look-ahead filters run long).
+      // Tiles of 256 leaves (the wave's candidate queue evens out the leaves' uneven
+      // candidate counts, so small tiles only shorten the launch's tail) and five checks
+      // between refills (candidates that pass the look-ahead filters run long); both
+      // measured best at N = 48.
       CZ_HIP(launch_ref_fused(width, T->dev, ls.res.p, ls.num.p, ls.meta.p, ls.count, mbi, lim,
-                              T->pow3.p, 2048, 4, nblocks, dtot.p,
+                              T->pow3.p, 256, 5, nblocks, dtot.p,
                               ovf_cap ? dovf.p : nullptr, ovf_cap, desc.p, esc_cap, s));
       unsigned long long nesc = 0;
       CZ_HIP(hipMemcpyAsync(&nesc, &dtot.p->esc_count, 8, hipMemcpyDeviceToHost, s));

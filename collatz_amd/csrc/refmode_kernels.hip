@@ -8,10 +8,11 @@
 //   k_la_build       the same dips as a range-mode look-ahead table: per residue of number,
 //                    the least odd-step count above the sieve's minimum that keeps it.
 //   k_ref_fused      Steps 2 and 3 (collatz_cudav7.cu:219-235, :280-366): persistent waves
-//                    take tiles of leaves; each lane walks its leaf's 2^A-bit candidate mask
-//                    (BV_i window AND the mod 9 row of its base) and iterates the candidate of
-//                    each set bit with the schedule; hash, count and overflow count are folded
-//                    with one atomic per wave and tile.
+//                    take tiles of leaves, expand 64 words of their 2^A-bit candidate masks
+//                    (BV_i window AND the mod 9 row of the base) at a time into a queue of
+//                    (leaf, a) in LDS, and iterate the queued candidates with the schedule;
+//                    hash, count and overflow count are folded with one atomic per wave and
+//                    tile.
 //   k_ref_iterate    Step 3 over a list of (leaf, a) candidates: the two- or three-limb rerun
 //                    of the candidates that left 96 bits in k_ref_fused.
 #include "schedule.hpp"
@@ -268,17 +269,24 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
   }
 }
 
-// Steps 2 and 3 fused: persistent waves take tiles of leaves; every lane walks the
-// candidate mask of its own leaf word by word (one word per iteration, so a sparse word
-// never stalls the wave), starts the candidate of each set bit (number + a * 3^f, n = base +
-// a * 2^MBI; Lemma 2.3), and runs up to U checks between refills.  No candidate list is
-// written.  Hash, count and overflow count are folded with one atomic per wave and tile.
+// Steps 2 and 3 fused: persistent waves take tiles of leaves and keep a queue of
+// candidates in LDS.  When the queue runs dry the wave expands the next 64 mask words of
+// its tile at once (lane i takes word i: leaf = first + word / nw, its BV_i window AND mod 9
+// row), counts their set bits, and lists every candidate (leaf, a) in order at its prefix
+// offset; lanes that need a candidate take the next entries and load the leaf's base,
+// number and f one refill ahead.  A candidate starts as number + a * 3^f with
+// n = base + a * 2^MBI (Lemma 2.3) and runs up to U checks between refills; a lane whose
+// candidate ends inside the U checks starts its prefetched one at once.  No candidate list
+// is written to memory.  Hash, count and overflow count are folded with one atomic per
+// wave and tile.
 //
 // L = 3 (96 bits, cutoff at or above 2^96) escalates instead of testing for overflow: a
 // check multiplies by less than 2^9.4, so a candidate that starts every check below 2^86
 // never reaches 2^96 (nor the cutoff) inside one; a candidate that ends a check above 2^86
 // is listed as (leaf, a, CPU-leaf flag), its partial hash dropped, and rerun from its start
 // by k_ref_iterate on two or three limbs.
+constexpr uint32_t kRefQueue = 512;  // queue entries per wave: (leaf, a)
+
 template <int L, bool LOWCUT>
 __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_t* base,
                                                    const uint64_t* num, const uint32_t* meta,
@@ -288,98 +296,142 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
                                                    uint64_t ovf_cap, uint64_t* esc,
                                                    uint64_t esc_cap) {
   __shared__ Pow3Lds p3s;
+  __shared__ uint64_t p3f[kPow3Max];
+  __shared__ uint2 queues[4][kRefQueue];
   stage_pow3(p3s);
+  if (threadIdx.x < kPow3Max) p3f[threadIdx.x] = pow3[threadIdx.x];
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63;
+  uint2* Q = queues[threadIdx.x >> 6];
   const uint64_t ntiles = (nleaves + tile - 1) / tile;
   const uint32_t cutoff = lim.cutoff;
   const uint64_t thr = lim.fast_thr;
+  const uint32_t nw_log2 = __builtin_ctz(T.nw);
   for (;;) {
     uint32_t t = 0;
     if (lane == 0) t = atomicAdd(&tot->tile_counter, 1u);
     t = __builtin_amdgcn_readfirstlane(t);
     if (t >= ntiles) break;
-    uint64_t cur = (uint64_t)t * tile;
-    const uint64_t end = cur + tile < nleaves ? cur + tile : nleaves;
+    const uint64_t first = (uint64_t)t * tile;
+    const uint64_t end = first + tile < nleaves ? first + tile : nleaves;
+    // Mask words of the tile, expanded 64 at a time; queue entries [qhead, qtail).
+    const uint32_t words = (uint32_t)(end - first) << nw_log2;
+    uint32_t wcur = 0, qhead = 0, qtail = 0;
     uint32_t h = 0, cnt = 0, ov = 0, stall = 0;
-    // Leaf of the lane: base, number, meta, 3^f, mask word index and remaining bits.
-    bool has_leaf = false, strict = false;
-    uint64_t lb = 0, ln = 0, lp3 = 0;
-    uint32_t lmeta = 0, w = 0, m = 0;
-    // Candidate of the lane.
-    bool act = false;
+    // Prefetched candidate of the lane: its leaf (index, base, number, meta) and a.
+    bool pf = false;
+    uint32_t pf_leaf = 0, pf_a = 0, pf_meta = 0;
+    uint64_t pf_lb = 0, pf_ln = 0;
+    // Running candidate of the lane.
+    bool act = false, strict = false;
     Num<L> v;
     uint64_t n_lo = 0;
-    uint32_t checks = 0, ch = 0, cur_a = 0;
-    uint64_t cur_leaf = 0;
-    for (;;) {
-      // (1) Lanes without a candidate take the next set bit of their leaf's mask, or move
-      //     to the mask's next word (one word per iteration), or give the leaf up.
-      if (!act && has_leaf) {
-        if (m == 0) {
-          if (++w < T.nw) m = leaf_mask_word(T, lb, ln, lmeta, w);
-          else has_leaf = false;
-        }
-        if (m != 0) {
-          const uint32_t j = __builtin_ctz(m);
-          m &= m - 1;
-          const uint64_t a = 32u * w + j;
-          // number + a * 3^f (Lemma 2.3; collatz_cudav7.cu:262-278)
-          const unsigned __int128 x = (unsigned __int128)a * lp3 + ln;
-          v.w[0] = (uint32_t)x;
-          v.w[1] = (uint32_t)(x >> 32);
-          v.w[2] = (uint32_t)(x >> 64);
-          if constexpr (L > 3) v.w[3] = (uint32_t)(x >> 96);
+    uint32_t checks = 0, ch = 0, cur_a = 0, cur_leaf = 0;
+    auto start_prefetched = [&]() {
+      pf = false;
+      const uint32_t f = meta_f(pf_meta);
+      // number + a * 3^f (Lemma 2.3; collatz_cudav7.cu:262-278)
+      const unsigned __int128 x = (unsigned __int128)pf_a * p3f[f] + pf_ln;
+      v.w[0] = (uint32_t)x;
+      v.w[1] = (uint32_t)(x >> 32);
+      v.w[2] = (uint32_t)(x >> 64);
+      if constexpr (L > 3) v.w[3] = (uint32_t)(x >> 96);
 #pragma unroll
-          for (int i = 4; i < L; ++i) v.w[i] = 0;
-          n_lo = lb + (a << mbi);
-          checks = 0;
-          ch = 0;
-          cur_a = (uint32_t)a;
-          act = true;
-          cnt += 1;
-          if constexpr (L == 3) {
-            // A start at or above the 96-bit check bound goes to the wide pass at once
-            // (a * 3^f < 2^76 and number < 2^64, so the start fits 96 bits).
-            if (v.w[2] >= (1u << (kNarrowCheckLog2 - 64))) {
-              const unsigned long long i = atomicAdd(&tot->esc_count, 1ull);
-              if (i < esc_cap)
-                esc[i] = cur_leaf | ((uint64_t)cur_a << 32) | (strict ? (1ull << 48) : 0ull);
-              cnt -= 1;
-              act = false;
+      for (int i = 4; i < L; ++i) v.w[i] = 0;
+      n_lo = pf_lb + ((uint64_t)pf_a << mbi);
+      // Leaves without a bit vector run on the CPU in the reference (search2/search3,
+      // collatz_cudav7.cu:517-519), whose second overflow test is '>' (:702).
+      strict = (f - T.tpo) >= 16;
+      cur_leaf = pf_leaf;
+      cur_a = pf_a;
+      checks = 0;
+      ch = 0;
+      act = true;
+      cnt += 1;
+      if constexpr (L == 3) {
+        // A start at or above the 96-bit check bound goes to the wide pass at once
+        // (a * 3^f < 2^76 and number < 2^64, so the start fits 96 bits).
+        if (v.w[2] >= (1u << (kNarrowCheckLog2 - 64))) {
+          const unsigned long long i = atomicAdd(&tot->esc_count, 1ull);
+          if (i < esc_cap)
+            esc[i] = cur_leaf | ((uint64_t)cur_a << 32) | (strict ? (1ull << 48) : 0ull);
+          cnt -= 1;
+          act = false;
+        }
+      }
+    };
+    for (;;) {
+      // (1) Lanes without a candidate start their prefetched one.
+      if (!act && pf) start_prefetched();
+      // (2) Lanes without a prefetch take the next queue entries; an empty queue is
+      //     refilled from the next 64 mask words of the tile.
+      const uint64_t need = __ballot(!pf);
+      if (need) {
+        uint32_t want = __popcll(need);
+        uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+        for (;;) {
+          const uint32_t avail = qtail - qhead;
+          const uint32_t take = avail < want ? avail : want;
+          if (!pf) {
+            if (rank < take) {
+              const uint2 e = Q[qhead + rank];
+              pf_leaf = e.x;
+              pf_a = e.y;
+              pf_lb = base[e.x];
+              pf_ln = num[e.x];
+              pf_meta = meta[e.x];
+              pf = true;
+            } else {
+              rank -= take;
             }
           }
+          qhead += take;
+          want -= take;
+          if (want == 0 || wcur >= words) break;
+          // The queue is empty here: expand words [wcur, wcur + 64).
+          const uint32_t wi = wcur + lane;
+          uint32_t m = 0, leaf = 0, w = 0;
+          if (wi < words) {
+            leaf = (uint32_t)first + (wi >> nw_log2);
+            w = wi & (T.nw - 1u);
+            m = leaf_mask_word(T, base[leaf], num[leaf], meta[leaf], w);
+          }
+          const uint32_t pc = __popc(m);
+          // Inclusive prefix sum of the counts across the wave.
+          uint32_t incl = pc;
+#pragma unroll
+          for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(incl, d, 64);
+            if (lane >= (uint32_t)d) incl += y;
+          }
+          // Words whose entries fit the queue (a prefix of the lanes: at least 16 of them,
+          // a word holds at most 32 entries) are listed now; the rest wait for the next
+          // expansion.
+          const bool fits = incl <= kRefQueue;
+          const uint64_t fm = __ballot(fits);
+          const uint32_t nfit = __popcll(fm);
+          if (fits) {
+            uint32_t pos = incl - pc;
+            while (m) {
+              const uint32_t j = __builtin_ctz(m);
+              m &= m - 1u;
+              Q[pos++] = make_uint2(leaf, 32u * w + j);
+            }
+          }
+          // A wave's LDS operations execute in order; only the compiler must keep the
+          // lanes' reads below after these writes.
+          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+          qhead = 0;
+          qtail = __shfl(incl, nfit - 1u, 64);
+          wcur += nfit;
         }
       }
-      // (2) Lanes without a leaf take the next leaves of the tile.
-      const uint64_t need = __ballot(!has_leaf);
-      if (need && cur < end) {
-        const uint32_t want = __popcll(need);
-        const uint64_t avail = end - cur;
-        const uint32_t take = avail < want ? (uint32_t)avail : want;
-        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-            (uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
-        if (!has_leaf && rank < take) {
-          const uint64_t leaf = cur + rank;
-          cur_leaf = leaf;
-          lb = base[leaf];
-          ln = num[leaf];
-          lmeta = meta[leaf];
-          const uint32_t f = meta_f(lmeta);
-          lp3 = pow3[f];
-          // Leaves without a bit vector run on the CPU in the reference (search2/search3,
-          // collatz_cudav7.cu:517-519), whose second overflow test is '>' (:702).
-          strict = (f - T.tpo) >= 16;
-          w = 0;
-          m = leaf_mask_word(T, lb, ln, lmeta, 0);
-          has_leaf = true;
-        }
-        cur += take;
-      }
-      if (__ballot(act || has_leaf) == 0) break;
-      // (3) Up to U checks.
-      if (act) {
-        for (uint32_t u = 0; u < U; ++u) {
+      if (__ballot(act || pf) == 0) break;
+      // (3) Up to U checks; a lane whose candidate ends before the last of them starts its
+      //     prefetched candidate at once.
+      for (uint32_t u = 0; u < U; ++u) {
+        if (act) {
 #pragma unroll
           for (int half = 0; half < 2; ++half) {
             odd_run<L>(v, p3s);
@@ -418,9 +470,9 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
               h ^= ch;
             }
             act = false;
-            break;
           }
         }
+        if (u + 1 < U && !act && pf) start_prefetched();
       }
     }
     // One atomic per wave and tile.
