@@ -9,9 +9,11 @@
 // "Spot checks" is always 0.
 //
 //   collatz_run --range LO HI [--k 32] [--chunk 24] [--cutoff 187] [--form plain|persistent]
-//               [--lookahead B [--semantics cuda|opencl]]
+//               [--lookahead B [--semantics cuda|opencl]] [--out FILE]
 // verifies [LO, HI) in range mode and prints one line per chunk and a total line; with
-// --lookahead the sieve carries the Step 2 filter of B steps.
+// --lookahead the sieve carries the Step 2 filter of B steps; --out also writes the records
+// as a versioned result file (format below).
+#include <algorithm>
 #include <chrono>
 #include <cinttypes>
 #include <cstdio>
@@ -48,6 +50,79 @@ static std::string to_dec(u128 v) {
   return s;
 }
 
+// ---- versioned per-chunk result file ("collatz-chunks 1") --------------------------------
+// Text, one item per line: a header with the knobs of the run, one line per chunk record
+// (the fields of collatz_chunk in header order; counts in decimal, the max excursion and
+// its start as 0x-prefixed lowercase hex, the hash as eight uppercase hex digits), the
+// merged total, the listed overflow starts in ascending order, and a last line
+// "end <fnv>" whose FNV-1a 32-bit hash covers every byte before it.  collatz_amd/chunkfile.py reads and writes the same
+// bytes.
+static std::string hex_words(const uint64_t* w, int n) {
+  int top = n - 1;
+  while (top > 0 && w[top] == 0) --top;
+  char buf[80];
+  int len = std::snprintf(buf, sizeof(buf), "0x%llx", (unsigned long long)w[top]);
+  for (int i = top - 1; i >= 0; --i)
+    len += std::snprintf(buf + len, sizeof(buf) - len, "%016llx", (unsigned long long)w[i]);
+  return std::string(buf, len);
+}
+
+static std::string chunk_fields(const collatz_chunk& c) {
+  char buf[320];
+  std::snprintf(buf, sizeof(buf), "%" PRIu64 " %" PRIu64 " %" PRIu64 " %" PRIu64 " %" PRIu64
+                " %s %s %" PRIu64 " %08X %u",
+                c.chunk_index, c.n_checked, c.count, c.stop_sum, c.stop_xor,
+                hex_words(c.max_exc, 3).c_str(), hex_words(c.max_start, 2).c_str(),
+                c.overflow_count, c.hash, c.flags);
+  return buf;
+}
+
+static uint32_t fnv1a(const std::string& s) {
+  uint32_t h = 2166136261u;
+  for (unsigned char ch : s) {
+    h ^= ch;
+    h *= 16777619u;
+  }
+  return h;
+}
+
+static bool write_chunk_file(const char* path, uint32_t k, uint32_t lookahead, uint32_t sem,
+                             uint32_t chunk, uint32_t cutoff, u128 lo, u128 hi,
+                             const std::vector<collatz_chunk>& chunks, const collatz_chunk& total,
+                             const uint64_t* ovf, uint64_t novf) {
+  std::string b = "collatz-chunks 1\n";
+  char buf[160];
+  const uint64_t l2[2] = {(uint64_t)lo, (uint64_t)(lo >> 64)};
+  const uint64_t h2[2] = {(uint64_t)hi, (uint64_t)(hi >> 64)};
+  std::snprintf(buf, sizeof(buf), "k %u\nmod9 1\nlookahead %u %s\nchunk_log2 %u\ncutoff_log2 %u\n",
+                k, lookahead, sem == COLLATZ_SEM_OPENCL ? "opencl" : "cuda", chunk, cutoff);
+  b += buf;
+  b += "range " + hex_words(l2, 2) + " " + hex_words(h2, 2) + "\n";
+  std::snprintf(buf, sizeof(buf), "chunks %zu\n", chunks.size());
+  b += buf;
+  b += "fields chunk_index n_checked count stop_sum stop_xor max_exc max_start overflow_count "
+       "hash flags\n";
+  for (const collatz_chunk& c : chunks) b += "chunk " + chunk_fields(c) + "\n";
+  b += "total " + chunk_fields(total) + "\n";
+  std::snprintf(buf, sizeof(buf), "overflow %" PRIu64 "\n", novf);
+  b += buf;
+  // The device lists overflow events in the order its waves report them; the file lists
+  // them in ascending order, so that it does not depend on scheduling.
+  std::vector<u128> starts(novf);
+  for (uint64_t i = 0; i < novf; ++i) starts[i] = ((u128)ovf[2 * i + 1] << 64) | ovf[2 * i];
+  std::sort(starts.begin(), starts.end());
+  for (const u128 v : starts) {
+    const uint64_t w[2] = {(uint64_t)v, (uint64_t)(v >> 64)};
+    b += "start " + hex_words(w, 2) + "\n";
+  }
+  std::snprintf(buf, sizeof(buf), "end %08x\n", fnv1a(b));
+  b += buf;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return false;
+  const bool ok = std::fwrite(b.data(), 1, b.size(), f) == b.size();
+  return std::fclose(f) == 0 && ok;
+}
+
 static std::string hex_start(uint64_t lo, uint64_t hi) {
   // "%X%08X%08X" of the start's 32-bit limbs 2, 1, 0 (collatz_cudav7.cu:298); for starts
   // below 2^96 the same text as the CPU form "%lX%016lX" (:691).
@@ -63,13 +138,14 @@ static int usage() {
                "usage: collatz_run [--N n] [--A a] [--B b] [--cutoff c] [--semantics cuda|opencl|rust]"
                " < case line\n"
                "       collatz_run --range LO HI [--k k] [--chunk c] [--cutoff c] [--limbs 2|3]"
-               " [--form plain|persistent] [--lookahead B]\n");
+               " [--form plain|persistent] [--lookahead B] [--out FILE]\n");
   return 2;
 }
 
 int main(int argc, char** argv) {
   uint32_t N = 45, A = 10, B = 24, cutoff = 0, k = 32, chunk = 24, limbs = 2, form = 0;
   uint32_t lookahead = 0;
+  const char* out_path = nullptr;
   uint32_t sem = COLLATZ_SEM_CUDA;
   bool range = false;
   u128 lo = 0, hi = 0;
@@ -88,6 +164,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(a, "--limbs")) limbs = std::atoi(next());
     else if (!std::strcmp(a, "--form")) form = !std::strcmp(next(), "persistent") ? 1 : 0;
     else if (!std::strcmp(a, "--lookahead")) lookahead = std::atoi(next());
+    else if (!std::strcmp(a, "--out")) out_path = next();
     else if (!std::strcmp(a, "--semantics")) {
       const char* v = next();
       sem = !std::strcmp(v, "rust") ? COLLATZ_SEM_RUST
@@ -134,8 +211,15 @@ int main(int argc, char** argv) {
                   c.chunk_index, c.n_checked, c.count, c.stop_sum, c.stop_xor, c.hash,
                   to_dec(ms).c_str(), c.flags);
     }
-    for (uint64_t i = 0; i < std::min<uint64_t>(res.overflow_written, 4096); ++i)
+    const uint64_t nlisted = std::min<uint64_t>(res.overflow_written, 4096);
+    for (uint64_t i = 0; i < nlisted; ++i)
       std::printf("overflow %s\n", hex_start(ovf[2 * i], ovf[2 * i + 1]).c_str());
+    if (out_path &&
+        !write_chunk_file(out_path, k, lookahead, sem, chunk, cutoff ? cutoff : 187u, lo, hi,
+                          chunks, res.total, ovf.data(), nlisted)) {
+      std::fprintf(stderr, "cannot write %s\n", out_path);
+      return 1;
+    }
     std::fprintf(stderr,
                  "Verified %s integers, %" PRIu64 " iterated, hash %08X, %.3f s (%.3e n/s).\n",
                  to_dec(hi - lo).c_str(), res.total.count, res.total.hash, s,

@@ -48,3 +48,34 @@ def test_range_mode(oracle):
         assert int(f[11], 16) == int(c["hash"])
         assert int(f[13]) == int(c["max_start"][0]) | (int(c["max_start"][1]) << 64)
     assert f"hash {int(total['hash']):08X}" in p.stderr
+
+
+@pytest.mark.gpu
+def test_range_result_file(tmp_path, oracle):
+    """--out writes the versioned result file: the library's records for the range (the
+    oracle's, bit for bit), byte-identical to collatz_amd.chunkfile's own writer; with a low
+    cutoff it lists the overflow starts, and with --lookahead it records the filter."""
+    import collatz_amd as cz
+    from collatz_amd import chunkfile as CF
+    o = oracle.sieve(20)
+    lo, hi = (1 << 52) + 11, (1 << 52) + (1 << 23)     # a few hundred values reach 2^64
+    for extra, cutoff, la in (([], 187, 0), (["--cutoff", "64"], 64, 0),
+                              (["--lookahead", "16"], 187, 16)):
+        path = str(tmp_path / f"r{cutoff}_{la}.txt")
+        p = run(["--range", str(lo), str(hi), "--k", "20", "--chunk", "20", "--out", path] + extra)
+        assert p.returncode == 0, p.stderr
+        r = CF.read_chunks(path)
+        chunks, total = oracle.verify_range(o, lo, hi, 20, cutoff, lookahead_bits=la)
+        assert r["chunks"].tobytes() == chunks.tobytes()
+        assert r["total"].tobytes() == total.tobytes()
+        assert r["lookahead_bits"] == la and r["cutoff_log2"] == cutoff
+        sv = cz.Sieve(20, lookahead_bits=la)
+        g = sv.verify_range(lo, hi, 20, cutoff_log2=cutoff)
+        starts = g.overflow_starts
+        if cutoff == 64:
+            assert len(starts) > 0 and r["overflow_starts"] == sorted(starts)
+        same = CF.format_chunks(k=20, chunk_log2=20, cutoff_log2=cutoff, lo=lo, hi=hi,
+                                chunks=g.chunks, total=g.total, overflow_starts=starts,
+                                lookahead_bits=la)
+        assert open(path, "rb").read() == same
+        sv.close()
