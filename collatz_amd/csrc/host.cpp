@@ -380,6 +380,12 @@ int collatz_sieve_build(const collatz_sieve_params* params, collatz_sieve** out)
     sv->la_sem = params->lookahead_semantics;
     hipError_t e = sv->la.alloc(1ull << (sv->la_bits - 3));
     if (e == hipSuccess) e = launch_la_build(sv->la_bits, sv->k, sv->la_sem, sv->la.p, s);
+    // A deeper filter under the CUDA compares also gets a 16-bit first level (the OpenCL
+    // compares can lower a dip later in the walk, so there the levels would not nest).
+    if (e == hipSuccess && sv->la_bits > kLaLevel1Bits && sv->la_sem == COLLATZ_SEM_CUDA) {
+      e = sv->la1.alloc(1ull << (kLaLevel1Bits - 3));
+      if (e == hipSuccess) e = launch_la_build(kLaLevel1Bits, sv->k, sv->la_sem, sv->la1.p, s);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
       delete sv;
@@ -399,7 +405,7 @@ int collatz_sieve_query(const collatz_sieve* sv, collatz_sieve_info* info) {
   info->survivors = sv->count;
   info->bitarray_bits = sv->bits_bits;
   info->device_bytes = sv->rec.bytes() + sv->bits.bytes() + sv->pow3.bytes() +
-                       sv->class_start_dev.bytes() + sv->la.bytes();
+                       sv->class_start_dev.bytes() + sv->la.bytes() + sv->la1.bytes();
   info->build_ms = sv->build_ms;
   info->lookahead_bits = sv->la_bits;
   info->lookahead_semantics = sv->la_sem;

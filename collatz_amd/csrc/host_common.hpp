@@ -166,6 +166,7 @@ struct collatz_sieve {
   cz::DevBuf<uint64_t> class_start_dev;
   uint32_t la_bits = 0, la_sem = 0;     // look-ahead filter (0 bits: none)
   cz::DevBuf<uint32_t> la;              // its table: 2^B nibbles
+  cz::DevBuf<uint32_t> la1;             // first-level table at 16 bits (B > 16, CUDA compares)
   double build_ms = 0;
   // Buffers and captured graph of the persistent form, created on first use; calls that
   // share a sieve handle are serialised on engine_mu.
@@ -179,6 +180,8 @@ struct collatz_sieve {
     a.la = la_bits ? la.p : nullptr;
     a.la_mask = la_bits ? (1u << la_bits) - 1u : 0u;
     a.la_emin = (306u * k + 484u) / 485u;
+    a.la1 = la_bits && la1.p ? la1.p : nullptr;
+    a.la1_mask = a.la1 ? (1u << cz::kLaLevel1Bits) - 1u : 0u;
     return a;
   }
 };

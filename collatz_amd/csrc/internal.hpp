@@ -10,6 +10,7 @@ constexpr uint32_t kNumerator = 306;    // 306/485 ~ log 2 / log 3 (collatz_cuda
 constexpr uint32_t kDenominator = 485;
 constexpr uint32_t kWideLog2 = 123;     // values at or above 2^123 need a third u64 limb
 constexpr uint32_t kMod9Keep = 0xCBu;   // bits {0,1,3,6,7}: residues mod 9 that stay
+constexpr uint32_t kLaLevel1Bits = 16;  // first-level look-ahead table of deeper filters
 constexpr int kPow3Max = 64;            // 3^f tables cover f < 64
 
 // Root node of a sieve walk (a node of the reference's search1 tree,
@@ -141,6 +142,11 @@ struct SieveArrays {
   const uint32_t* la;
   uint32_t la_mask;   // 2^B - 1
   uint32_t la_emin;   // ceil(306 k / 485)
+  // Optional first level (null: none): the same table at 16 bits.  Under the CUDA
+  // compares a dip over 16 steps is at most the dip over B, so a candidate the 16-bit
+  // table drops the B-bit one drops too, and only the survivors read the large table.
+  const uint32_t* la1;
+  uint32_t la1_mask;
 };
 
 // Look-ahead tables of reference mode on the device.

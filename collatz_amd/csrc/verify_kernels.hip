@@ -29,9 +29,13 @@ __device__ __forceinline__ void q_times_pow3(uint64_t q_lo, uint64_t q_hi, uint6
 // keep the candidate iff the dip of number mod 2^B is at most 485 f - 306 k, read from the
 // nibble table as f - e_min >= g.  num_lo is the low word of the start value number.
 __device__ __forceinline__ bool la_keep(const SieveArrays& sv, uint32_t f, uint32_t num_lo) {
+  const uint32_t fe = f - sv.la_emin;
+  if (sv.la1) {
+    const uint32_t i1 = num_lo & sv.la1_mask;
+    if (fe < ((sv.la1[i1 >> 3] >> ((i1 & 7u) << 2)) & 15u)) return false;
+  }
   const uint32_t idx = num_lo & sv.la_mask;
-  const uint32_t g = (sv.la[idx >> 3] >> ((idx & 7u) << 2)) & 15u;
-  return f - sv.la_emin >= g;
+  return fe >= ((sv.la[idx >> 3] >> ((idx & 7u) << 2)) & 15u);
 }
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
