@@ -168,6 +168,7 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   CZ_HIP(hipEventRecord(e0_, s));
   if (graph) {
     if (!gexec_ || !(key == gkey_)) {
+      Span sp("graph capture");
       if (gexec_) (void)hipGraphExecDestroy(gexec_);
       if (graph_) (void)hipGraphDestroy(graph_);
       gexec_ = nullptr;

@@ -56,7 +56,7 @@ struct collatz_server {
   bool stop = false;
   double t0 = 0;
   uint64_t bytes_in_use = 0;
-  cz::Tuning tn{16384, 32, 2};
+  cz::Tuning tn{0, 32, 2};  // waves per CU and checks per refill; tiles follow each launch
 };
 
 namespace cz {
@@ -71,6 +71,8 @@ static void server_loop(collatz_server* S) {
       uint64_t chunk;
     };
     std::vector<SlotRef> slots;
+    std::vector<uint64_t> row_est;     // expected survivors of each row of the launch
+    std::vector<uint8_t> slot_first;   // 1 where a row starts a new slot
     {
       std::unique_lock<std::mutex> lk(S->mu);
       S->cv.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->active.empty(); });
@@ -103,9 +105,9 @@ static void server_loop(collatz_server* S) {
           RowDesc row = r->plan.rows[i];
           // The row's slot is its chunk within the request; rows of one chunk are contiguous.
           const uint64_t chunk = row.slot;
-          if (chunk != last_chunk) {
+          const bool first_of_slot = chunk != last_chunk;
+          if (first_of_slot) {
             slots.push_back({r, chunk});
-            lp.part.push_back(lp.ntiles);
             last_chunk = chunk;
           }
           row.slot = (uint32_t)(slots.size() - 1);
@@ -122,11 +124,9 @@ static void server_loop(collatz_server* S) {
             row.i_hi = kResolveFlag | lp.keys.size();
             lp.keys.push_back(r->plan.keys[r->plan.key_hi[i]]);
           }
-          uint64_t pieces = (uint64_t)((double)r->row_est[i] / S->tn.tile + 0.5);
-          pieces = std::min<uint64_t>(std::max<uint64_t>(pieces, 1), 1u << 20);
-          row.block_start = lp.ntiles;
-          lp.ntiles += pieces;
           lp.rows.push_back(row);
+          row_est.push_back(r->row_est[i]);
+          slot_first.push_back(first_of_slot ? 1 : 0);
           if (r->hi > max_hi) max_hi = r->hi;
           mine += r->row_est[i] + 1;
           taken += r->row_est[i] + 1;
@@ -140,6 +140,18 @@ static void server_loop(collatz_server* S) {
         if (taken >= budget) break;
         take(r, budget - taken, false);
       }
+    }
+    // Tiles for this launch's size (a small launch spreads over all resident waves), then
+    // each row's pieces and each slot's first tile.
+    uint64_t est_total = 0;
+    for (const uint64_t e : row_est) est_total += e;
+    const uint32_t tile = adaptive_tile(S->sv, est_total, S->tn.waves);
+    for (size_t i = 0; i < lp.rows.size(); ++i) {
+      if (slot_first[i]) lp.part.push_back(lp.ntiles);
+      uint64_t pieces = (uint64_t)((double)row_est[i] / tile + 0.5);
+      pieces = std::min<uint64_t>(std::max<uint64_t>(pieces, 1), 1u << 20);
+      lp.rows[i].block_start = lp.ntiles;
+      lp.ntiles += pieces;
     }
     lp.nslots = slots.size();
     lp.part.push_back(lp.ntiles);

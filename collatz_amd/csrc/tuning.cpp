@@ -34,11 +34,14 @@ static const uint32_t kDefaultWaves = 32, kDefaultU = 2;
 // Survivors per tile: the largest power of two in [4096, 65536] that still gives every
 // resident wave about 16 tiles, so that work stealing evens out the tail while the per-tile
 // set-up (the q * 3^f table, the record merge) stays small.  A look-ahead keeps about a
-// third of the survivors, so tiles of such a sieve may be four times larger.
+// third of the survivors, so tiles of such a sieve may be four times larger.  A launch too
+// small to give every resident wave a tile of 4096 takes smaller tiles (down to 256) until
+// it does: its duration is about one tile per wave, so this is its latency.
 uint32_t adaptive_tile(const collatz_sieve* sv, uint64_t candidates, uint32_t waves) {
   const uint64_t nwaves = (uint64_t)device_cu_count(sv->device) * waves;
   uint32_t t = sv->la_bits ? 262144 : 65536;
   while (t > 4096 && candidates / t < 16 * nwaves) t >>= 1;
+  while (t > 256 && candidates / t < nwaves) t >>= 1;
   return t;
 }
 
