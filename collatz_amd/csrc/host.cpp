@@ -90,7 +90,9 @@ void chunk_merge(collatz_chunk* a, const collatz_chunk& b) {
 
 int launch_width(int limbs, u128 max_hi, uint32_t cutoff) {
   if (limbs == 2 && cutoff >= kNarrowLog2 && max_hi <= ((u128)1 << kNarrowMaxStartLog2)) return 1;
-  if (limbs == 2 && cutoff >= kWideLog2 && max_hi > ((u128)1 << kWideMaxStartLog2)) return 3;
+  // Above 2^100 a start value (up to n (3/2)^k) can pass 2^118 (the four-word per-check
+  // bound) or 2^123 (the four-word start bound under a lower cutoff): six words at once.
+  if (limbs == 2 && max_hi > ((u128)1 << kWideMaxStartLog2)) return 3;
   return limbs;
 }
 

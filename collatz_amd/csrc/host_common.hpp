@@ -122,9 +122,9 @@ int resolve_rows(const struct ::collatz_sieve* sv, RowPlan& plan, hipStream_t s)
 
 Limits make_limits(int limbs, uint32_t cutoff);
 // Word width of a launch for a two- or three-limb request: two-limb work whose starts are
-// below 2^68 and whose cutoff is at least 2^91 runs on 96 bits (width 1); otherwise the
-// width is the limb count.  A launch whose escalations overflow the list is rerun one width
-// up.
+// below 2^68 and whose cutoff is at least 2^91 runs on 96 bits (width 1); two-limb work
+// with starts above 2^100 on six words (width 3); otherwise the width is the limb count.
+// A launch whose escalations overflow the list is rerun one width up.
 int launch_width(int limbs, u128 max_hi, uint32_t cutoff);
 
 // Leaves of the search1 tree below a root at root.depth, sorted by residue: the device

@@ -110,7 +110,8 @@ constexpr uint32_t kNarrowMaxStartLog2 = 68;
 constexpr uint32_t kNarrowCheckLog2 = 86;
 constexpr uint32_t kWideCheckLog2 = 118;
 // Two-limb work whose starts are above 2^100 runs on three limbs from the start (at the
-// four-word check bound of 2^118 too many of its candidates would escalate).
+// four-word check bound of 2^118, or the four-word start bound of 2^123 under a lower
+// cutoff, too many of its candidates would escalate).
 constexpr uint32_t kWideMaxStartLog2 = 100;
 constexpr uint32_t kMaxChecks = 1u << 16;
 // The same bound in T-steps for the persistent kernel: every half-check (odd run, even run)

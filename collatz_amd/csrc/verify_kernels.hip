@@ -180,11 +180,16 @@ __device__ __forceinline__ void finish(const Cand<L>& c, Acc<L>& acc, uint32_t k
 }
 
 // Start-of-candidate test: the start value is offered to the max, and a start at or above
-// the width's per-check bound goes to the three-limb pass.
+// the width's per-check bound goes to the three-limb pass.  Four words with a cutoff below
+// 2^123 do not escalate per check: every odd run starts at or below max(start, 2^cutoff)
+// (an odd run that ends at or above the cutoff resets the value to 2; even runs only
+// decrease) and multiplies by at most (3/2)^8 < 2^4.7, so values stay below 2^128 only for
+// starts below 2^123; a larger start goes to the three-limb pass.
 template <int L, bool LOWCUT, class Sink>
 __device__ __forceinline__ bool cand_enter(const Cand<L>& c, Sink& sink) {
   sink.offer(c.v, c.n_lo, c.n_hi);
   if constexpr (Escalates<L, LOWCUT>::value) return !over_check_bound<L>(c.v);
+  if constexpr (L == 4) return c.v.w[3] < (1u << (kWideLog2 - 96));
   return true;
 }
 

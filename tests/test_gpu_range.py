@@ -167,3 +167,23 @@ def test_bad_arguments(s20):
         s20.verify_range(1, 100, cutoff_log2=60)
     with pytest.raises(cz.CollatzError):
         s20.verify_range(1, 100, limbs=4)
+
+
+def test_low_cutoff_start_above_2p123(oracle):
+    """Two limbs with a cutoff of 2^64 and starts near 2^122 (k = 16): start values reach
+    2^128 after the jump and odd runs of starts above 2^123 leave 128 bits before the
+    overflow reset, so such launches run on six words; the max excursion (taken before
+    the reset) must be exact."""
+    import collatz_amd as cz
+    lo, hi = 0x32b68ae4d0284133bd56836441af003, 0x32b68ae4d0284133bd568364427b3e6
+    o16 = oracle.sieve(16)
+    for la, sem in ((0, 0), (8, 2)):
+        chunks, total = oracle.verify_range(o16, lo, hi, 20, 64, threads=16, lookahead_bits=la,
+                                            lookahead_semantics=sem)
+        sv = cz.Sieve(16, lookahead_bits=la, lookahead_semantics=sem)
+        for form in (0, 1):
+            g = sv.verify_range(lo, hi, 20, 2, 64, form=form, overflow_cap=1 << 20)
+            assert_chunks_equal(g.chunks, chunks, f"la {la} form {form}")
+            assert_total_equal(g.total, total, "total")
+        assert int(total["max_exc"][2]) >> 4 != 0   # the excursion itself is above 2^132
+        sv.close()
