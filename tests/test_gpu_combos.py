@@ -26,10 +26,11 @@ def cases(seed, n):
     return out
 
 
-def test_random_combinations(oracle):
+@pytest.mark.parametrize("seed", [2024, 7, 99])
+def test_random_combinations(oracle, seed):
     import collatz_amd as cz
     sieves, osieves = {}, {}
-    for k, la, sem, cutoff, lo, hi, C, limbs in cases(2024, 200):
+    for k, la, sem, cutoff, lo, hi, C, limbs in cases(seed, 200):
         key = (k, la, sem)
         if key not in sieves:
             sieves[key] = cz.Sieve(k, lookahead_bits=la, lookahead_semantics=sem)
