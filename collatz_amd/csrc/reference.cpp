@@ -51,6 +51,10 @@ static int check_params(const collatz_ref_params* p, uint32_t max_mbi = 40) {
   if (p->N < p->A + 4 || p->N - p->A > max_mbi)
     return fail(COLLATZ_EINVAL, "N - A out of range (4..40 for a run, 4..120 for tables)");
   if (p->B < 8 || p->B > 28) return fail(COLLATZ_EINVAL, "B must be in [8, 28]");
+  // A leaf's 2^A-bit candidate window is read from its bit vector with the vector's first
+  // words mirrored after its end (collatz_cudav7.cu:475-477), which wraps a window correctly
+  // only if the window is no longer than the vector.
+  if (p->A > p->B) return fail(COLLATZ_EINVAL, "A (LASTBITS) must not exceed B");
   if (p->num_bv < 1 || p->num_bv > 32) return fail(COLLATZ_EINVAL, "num_bv must be in [1, 32]");
   if (p->semantics > 2) return fail(COLLATZ_EINVAL, "unknown semantics");
   if (p->cutoff_log2 < 64 || p->cutoff_log2 > 187)

@@ -167,7 +167,7 @@ typedef struct {
 typedef struct {
   uint32_t N;            /* MAXBITS (N - A <= 40) */
   uint32_t A;            /* LASTBITS, 1..12 */
-  uint32_t B;            /* STEPSIZEINITIAL, 24 */
+  uint32_t B;            /* STEPSIZEINITIAL, 24; 8..28 and at least A */
   uint32_t num_bv;       /* bit vectors, 16 (CUDA/OpenCL) or 8 (Rust) */
   uint32_t semantics;    /* collatz_semantics */
   uint32_t cutoff_log2;  /* CUTOFFPOWER, 64..187 */

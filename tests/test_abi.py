@@ -63,3 +63,19 @@ def test_sieve_params_are_checked_before_the_device():
         assert lib.collatz_sieve_build(ctypes.byref(p), ctypes.byref(h)) == cz.EINVAL
         assert b"k must" in lib.collatz_last_error() or b"lookahead" in lib.collatz_last_error()
     assert ctypes.sizeof(cz.SieveInfo) == 48
+
+
+def test_reference_params_are_checked_before_the_device():
+    """Reference mode refuses knobs outside the reference's domain (here a candidate window
+    longer than a bit vector, A > B) with EINVAL before any HIP call."""
+    import collatz_amd as cz
+    lib = cz.load()
+    node = cz.Node((ctypes.c_uint64 * 2)(3, 0), (ctypes.c_uint64 * 2)(9, 0), 2, 2, 1, 0)
+    res = cz.RefResult()
+    for N, A, B in ((30, 10, 8), (40, 13, 24), (30, 0, 24), (60, 10, 24)):
+        p = cz.RefParams(N, A, B, 16, 0, 100, -1, 0)
+        assert lib.collatz_reference_run(ctypes.byref(p), ctypes.byref(node), None,
+                                         ctypes.byref(res)) == cz.EINVAL, (N, A, B)
+    p = cz.RefParams(30, 10, 8, 16, 0, 100, -1, 0)
+    lib.collatz_reference_run(ctypes.byref(p), ctypes.byref(node), None, ctypes.byref(res))
+    assert b"must not exceed B" in lib.collatz_last_error()
