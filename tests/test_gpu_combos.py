@@ -46,3 +46,37 @@ def test_random_combinations(oracle, seed):
             assert_total_equal(g.total, total, tag)
         b = sieves[key].verify_batch([(lo, hi)], chunk_log2=C, limbs=limbs, cutoff_log2=cutoff)
         assert_chunks_equal(b[0].chunks, chunks, f"{tag} batch")
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_random_batches_and_server(oracle, seed):
+    """Many intervals of mixed magnitude (2^20 .. 2^127) in one batch and through the
+    server, with a random cutoff, chunk size and look-ahead: each interval's records equal
+    the oracle's."""
+    import collatz_amd as cz
+    rng = random.Random(seed)
+    k = rng.choice([16, 20, 24])
+    la = rng.choice([0, 12, 16])
+    cutoff = rng.choice([64, 80, 100, 128, 187])
+    C = rng.choice([18, 22])
+    ivs = []
+    for _ in range(40):
+        lo = rng.getrandbits(rng.choice([20, 40, 62, 65, 90, 120, 127])) | 1
+        ivs.append((lo, lo + rng.randrange(1 << 16, 1 << 23)))
+    sv = cz.Sieve(k, lookahead_bits=la)
+    o = oracle.sieve(k)
+    want = [oracle.verify_range(o, lo, hi, C, cutoff, threads=16, lookahead_bits=la)
+            for lo, hi in ivs]
+    tag = f"k={k} la={la} cutoff={cutoff} C={C}"
+    for graph in (False, True):
+        got = sv.verify_batch(ivs, chunk_log2=C, cutoff_log2=cutoff, use_graph=graph)
+        for (lo, hi), g, (chunks, total) in zip(ivs, got, want):
+            assert_chunks_equal(g.chunks, chunks, f"{tag} batch [{lo:#x},{hi:#x})")
+            assert_total_equal(g.total, total, tag)
+    srv = cz.Server(sv, chunk_log2=C, cutoff_log2=cutoff)
+    tickets = [srv.submit(lo, hi) for lo, hi in ivs]
+    for t, (lo, hi), (chunks, total) in zip(tickets, ivs, want):
+        tot, ch, _ = srv.wait(t)
+        assert_chunks_equal(ch, chunks, f"{tag} server [{lo:#x},{hi:#x})")
+    srv.close()
+    sv.close()
