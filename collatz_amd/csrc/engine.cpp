@@ -76,6 +76,8 @@ Engine::~Engine() {
   if (graph_) (void)hipGraphDestroy(graph_);
   if (e0_) (void)hipEventDestroy(e0_);
   if (e1_) (void)hipEventDestroy(e1_);
+  if (e2_) (void)hipEventDestroy(e2_);
+  if (e3_) (void)hipEventDestroy(e3_);
 }
 
 int Engine::ensure(const LaunchPlan& lp, uint64_t ovf_cap) {
@@ -98,6 +100,8 @@ int Engine::ensure(const LaunchPlan& lp, uint64_t ovf_cap) {
   CZ_HIP(h_ctl_.reserve(1));
   if (!e0_) CZ_HIP(hipEventCreate(&e0_));
   if (!e1_) CZ_HIP(hipEventCreate(&e1_));
+  if (!e2_) CZ_HIP(hipEventCreate(&e2_));
+  if (!e3_) CZ_HIP(hipEventCreate(&e3_));
   return COLLATZ_OK;
 }
 
@@ -206,8 +210,14 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   out.escalated = esc_n;
   out.esc_slots.clear();
   if (esc_n) {
+    CZ_HIP(hipEventRecord(e2_, s));
     rc = run_escalated(sv_, esc_, esc_n, cutoff, lists, out.recs, 0, s);
     if (rc) return rc;
+    CZ_HIP(hipEventRecord(e3_, s));
+    CZ_HIP(hipEventSynchronize(e3_));
+    float ems = 0;
+    (void)hipEventElapsedTime(&ems, e2_, e3_);
+    out.device_ms += ems;
     std::vector<EscEntry> h(esc_n);
     CZ_HIP(hipMemcpy(h.data(), esc_.p, esc_n * sizeof(EscEntry), hipMemcpyDeviceToHost));
     for (const EscEntry& e : h) out.esc_slots.push_back(e.slot);

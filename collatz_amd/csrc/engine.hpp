@@ -92,7 +92,7 @@ class Engine {
   HostBuf<uint64_t> h_keys_, h_part_;
   HostBuf<unsigned long long> h_cnt_;
   HostBuf<Ctl> h_ctl_;
-  hipEvent_t e0_ = nullptr, e1_ = nullptr;
+  hipEvent_t e0_ = nullptr, e1_ = nullptr, e2_ = nullptr, e3_ = nullptr;
   // Captured launch sequence and what it was captured for.
   hipGraphExec_t gexec_ = nullptr;
   hipGraph_t graph_ = nullptr;

@@ -131,14 +131,21 @@ static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeAr
   lists.ovf_cap = ovf_cap;
   lists.stall_count = dcnt.p + 2;
   const Limits lim = make_limits((int)a.width, a.cutoff);
-  hipEvent_t e0, e1;
-  CZ_HIP(hipEventCreate(&e0));
-  CZ_HIP(hipEventCreate(&e1));
-  CZ_HIP(hipEventRecord(e0, s));
+  // Device time: the verification and reduction kernels, then the three-limb pass.
+  struct Events {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Events() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } ev;
+  CZ_HIP(hipEventCreate(&ev.a));
+  CZ_HIP(hipEventCreate(&ev.b));
+  CZ_HIP(hipEventRecord(ev.a, s));
   CZ_HIP(launch_verify_plain((int)a.width, sv->arrays(), drows.p, (uint32_t)plan.rows.size(),
                              nblocks, R, sv->mod9, lim, sv->pow3.p, dblock.p, lists, s));
-  CZ_HIP(hipEventRecord(e1, s));
   CZ_HIP(launch_reduce_chunks(dblock.p, dpart.p, (uint32_t)plan.nchunks, dchunk.p, s));
+  CZ_HIP(hipEventRecord(ev.b, s));
   std::vector<DevRec> recs(plan.nchunks);
   unsigned long long cnt[3];
   CZ_HIP(hipMemcpyAsync(recs.data(), dchunk.p, recs.size() * sizeof(DevRec),
@@ -146,9 +153,7 @@ static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeAr
   CZ_HIP(hipMemcpyAsync(cnt, dcnt.p, 24, hipMemcpyDeviceToHost, s));
   CZ_HIP(hipStreamSynchronize(s));
   float ms = 0;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  (void)hipEventElapsedTime(&ms, ev.a, ev.b);
   if (a.width < 3 && cnt[0] > esc_cap) {
     // More escalations than the list holds: run the whole range one width up.
     RangeArgs b = a;
@@ -157,10 +162,14 @@ static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeAr
   }
   uint64_t ovf_n = cnt[1];
   if (cnt[0]) {
-    const uint64_t before = ovf_n;
-    (void)before;
+    CZ_HIP(hipEventRecord(ev.a, s));
     rc = run_escalated(sv, desc, cnt[0], a.cutoff, lists, recs, 0, s);
     if (rc) return rc;
+    CZ_HIP(hipEventRecord(ev.b, s));
+    CZ_HIP(hipEventSynchronize(ev.b));
+    float ems = 0;
+    (void)hipEventElapsedTime(&ems, ev.a, ev.b);
+    ms += ems;
     CZ_HIP(hipMemcpy(&cnt[1], dcnt.p + 1, 16, hipMemcpyDeviceToHost));
     ovf_n = cnt[1];
   }

@@ -108,7 +108,9 @@ typedef struct {
   uint64_t overflow_written;  /* starts reported (even if > overflow_cap) */
   uint64_t escalated;         /* candidates moved to three limbs */
   uint64_t checks;            /* checks (two odd and two even runs) executed */
-  double device_ms;           /* device time of the main verification kernel */
+  double device_ms;           /* device time of the launch: the kernels that set up, verify
+                                 and reduce (inputs already in device memory) plus the
+                                 three-limb pass of escalated candidates */
 } collatz_range_result;
 
 enum collatz_form {
