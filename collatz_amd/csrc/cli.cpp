@@ -8,6 +8,11 @@
 // --semantics opencl, collatz_openclv2.cpp:279-280).  No CPU spot checks are run, so
 // "Spot checks" is always 0.
 //
+//   collatz_run [--N 45] [--A 10] ... --cases FILE
+// runs every case line of FILE in turn (the reference's users start one process per line),
+// with one "line: launches L hash H" line each on stderr and a closing line with the XOR of
+// the complete hashes.
+//
 //   collatz_run --range LO HI [--k 32] [--chunk 24] [--cutoff 187] [--form plain|persistent]
 //               [--lookahead B [--semantics cuda|opencl]] [--out FILE]
 // verifies [LO, HI) in range mode and prints one line per chunk and a total line; with
@@ -137,6 +142,7 @@ static int usage() {
   std::fprintf(stderr,
                "usage: collatz_run [--N n] [--A a] [--B b] [--cutoff c] [--semantics cuda|opencl|rust]"
                " < case line\n"
+               "       collatz_run [same knobs] --cases FILE\n"
                "       collatz_run --range LO HI [--k k] [--chunk c] [--cutoff c] [--limbs 2|3]"
                " [--form plain|persistent] [--lookahead B] [--out FILE]\n");
   return 2;
@@ -146,6 +152,7 @@ int main(int argc, char** argv) {
   uint32_t N = 45, A = 10, B = 24, cutoff = 0, k = 32, chunk = 24, limbs = 2, form = 0;
   uint32_t lookahead = 0;
   const char* out_path = nullptr;
+  const char* cases_path = nullptr;
   uint32_t sem = COLLATZ_SEM_CUDA;
   bool range = false;
   u128 lo = 0, hi = 0;
@@ -165,6 +172,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(a, "--form")) form = !std::strcmp(next(), "persistent") ? 1 : 0;
     else if (!std::strcmp(a, "--lookahead")) lookahead = std::atoi(next());
     else if (!std::strcmp(a, "--out")) out_path = next();
+    else if (!std::strcmp(a, "--cases")) cases_path = next();
     else if (!std::strcmp(a, "--semantics")) {
       const char* v = next();
       sem = !std::strcmp(v, "rust") ? COLLATZ_SEM_RUST
@@ -227,33 +235,81 @@ int main(int argc, char** argv) {
     collatz_sieve_free(sv);
     return 0;
   }
-  std::string line;
-  std::getline(std::cin, line);
-  std::istringstream iss(line);
-  uint64_t t[6] = {0};
-  for (int i = 0; i < 6; ++i) iss >> t[i];
   collatz_ref_params p = {N, A, B, sem == COLLATZ_SEM_RUST ? 8u : 16u, sem,
                           cutoff ? cutoff : 100u, -1, 0};
-  collatz_node node;
-  std::memset(&node, 0, sizeof(node));
-  node.base[0] = t[0];
-  node.sigfig = (uint32_t)t[1];
-  node.threepower = (uint32_t)t[2];
-  node.number[0] = t[3];
-  node.doubleeven = (uint32_t)t[4];
-  node.numevens = (uint32_t)t[5];
   std::vector<uint64_t> ovf(2 * (1 << 16));
-  collatz_ref_result r;
-  std::memset(&r, 0, sizeof(r));
-  r.overflow_starts = ovf.data();
-  r.overflow_cap = 1 << 16;
-  const int rc = collatz_reference_run(&p, &node, nullptr, &r);
-  if (rc != COLLATZ_OK && rc != COLLATZ_ECAPACITY) {
-    std::fprintf(stderr, "collatz_run: %s\n", collatz_last_error());
-    return 1;
+  // One case line "base sigfig threepower number doubleeven numevens" through the whole
+  // reference program; false (with a message) if the line does not parse or the run fails.
+  auto run_line = [&](const std::string& line, collatz_ref_result& r) -> bool {
+    std::istringstream iss(line);
+    uint64_t t[6] = {0};
+    for (int i = 0; i < 6; ++i) {
+      if (!(iss >> t[i])) {
+        std::fprintf(stderr, "collatz_run: not a case line: %s\n", line.c_str());
+        return false;
+      }
+    }
+    collatz_node node;
+    std::memset(&node, 0, sizeof(node));
+    node.base[0] = t[0];
+    node.sigfig = (uint32_t)t[1];
+    node.threepower = (uint32_t)t[2];
+    node.number[0] = t[3];
+    node.doubleeven = (uint32_t)t[4];
+    node.numevens = (uint32_t)t[5];
+    std::memset(&r, 0, sizeof(r));
+    r.overflow_starts = ovf.data();
+    r.overflow_cap = 1 << 16;
+    const int rc = collatz_reference_run(&p, &node, nullptr, &r);
+    if (rc != COLLATZ_OK && rc != COLLATZ_ECAPACITY) {
+      std::fprintf(stderr, "collatz_run: %s\n", collatz_last_error());
+      return false;
+    }
+    for (uint64_t i = 0; i < std::min<uint64_t>(r.overflow_written, 1 << 16); ++i)
+      std::printf("%s\n", hex_start(ovf[2 * i], ovf[2 * i + 1]).c_str());
+    return true;
+  };
+  if (cases_path) {
+    // The case-file driver: every line of the file in turn (what the reference's users do
+    // with one process per line), one summary line each on stderr, then the XOR of the
+    // complete hashes (for lines that partition a tree, the whole tree's hash).
+    std::FILE* f = std::fopen(cases_path, "r");
+    if (!f) {
+      std::fprintf(stderr, "collatz_run: cannot open %s\n", cases_path);
+      return 1;
+    }
+    char buf[512];
+    uint32_t all = 0;
+    uint64_t lines = 0, launches = 0, cands = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (std::fgets(buf, sizeof(buf), f)) {
+      std::string line(buf);
+      while (!line.empty() && (line.back() == '\n' || line.back() == '\r')) line.pop_back();
+      if (line.find_first_not_of(" \t") == std::string::npos) continue;
+      collatz_ref_result r;
+      if (!run_line(line, r)) {
+        std::fclose(f);
+        return 1;
+      }
+      std::fprintf(stderr, "%s: launches %" PRIu64 " hash %X\n", line.c_str(), r.launches,
+                   r.complete_hash);
+      all ^= r.complete_hash;
+      ++lines;
+      launches += r.launches;
+      cands += r.candidates;
+    }
+    std::fclose(f);
+    std::fflush(stdout);
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::fprintf(stderr,
+                 "%" PRIu64 " case lines, %" PRIu64 " launches, %" PRIu64 " candidates, "
+                 "XOR of complete hashes = %X, %.3f s.\n", lines, launches, cands, all, s);
+    return 0;
   }
-  for (uint64_t i = 0; i < std::min<uint64_t>(r.overflow_written, 1 << 16); ++i)
-    std::printf("%s\n", hex_start(ovf[2 * i], ovf[2 * i + 1]).c_str());
+  std::string line;
+  std::getline(std::cin, line);
+  collatz_ref_result r;
+  if (!run_line(line, r)) return 1;
   std::fflush(stdout);
   if (sem == COLLATZ_SEM_OPENCL) {
     std::fprintf(stderr, "Number of kernel launches = %" PRIu64 " \n", r.launches);

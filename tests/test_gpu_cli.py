@@ -79,3 +79,24 @@ def test_range_result_file(tmp_path, oracle):
                                 lookahead_bits=la)
         assert open(path, "rb").read() == same
         sv.close()
+
+
+@pytest.mark.gpu
+def test_case_file_driver():
+    """--cases runs every line of a case file: the per-line hashes equal the library's for
+    each line, and their XOR (the lines of cases_tiny partition the tree) equals the
+    whole-tree run's complete hash."""
+    import collatz_amd as cz
+    path = os.path.join(ROOT, "tests", "golden", "cases_tiny.txt")
+    p = run(["--N", "40", "--A", "10", "--semantics", "cuda", "--cases", path])
+    assert p.returncode == 0, p.stderr
+    per_line = re.findall(r"^(.*): launches (\d+) hash ([0-9A-F]+)$", p.stderr, re.M)
+    lines = [l for l in open(path).read().splitlines() if l.strip()]
+    assert len(per_line) == len(lines)
+    for (line, launches, h), want in zip(per_line[:6], lines[:6]):
+        r = cz.reference_run(tuple(int(x) for x in want.split()), 40, 10, semantics=cz.SEM_CUDA)
+        assert line.strip() == want.strip() and int(h, 16) == r.complete_hash
+        assert int(launches) == r.launches
+    m = re.search(r"XOR of complete hashes = ([0-9A-F]+)", p.stderr)
+    full = cz.reference_run((3, 2, 2, 9, 1, 0), 40, 10, semantics=cz.SEM_CUDA)
+    assert m and int(m.group(1), 16) == full.complete_hash
