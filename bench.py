@@ -186,6 +186,31 @@ def served_leg(sv, reqs, rate, chunk_log2):
             "max_ms": lat[-1], "n_per_s": n / span_s, "offered_req_per_s": rate}
 
 
+def lookahead_leg(cz, ivs, total_n, steps):
+    """The batch workload with the Step 2 look-ahead of B = LOOKAHEAD_BITS steps at depth 32:
+    fewer candidates iterated (different counts and checksums by definition), the same
+    integers verified.  n/s by device time, as the headline; the per-call rate beside it."""
+    sla = cz.Sieve(K, lookahead_bits=LOOKAHEAD_BITS)
+    sla.verify_batch(ivs, chunk_log2=BATCH_CHUNK, use_graph=True)
+    ms = []
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o = sla.verify_batch(ivs, chunk_log2=BATCH_CHUNK, use_graph=True)
+        ms.append(o[0].device_ms)
+    device_sync()
+    d = (time.perf_counter() - t0) / steps
+    lah = 0
+    for x in o:
+        lah ^= int(x.total["hash"])
+    lms = sum(ms) / len(ms)
+    sla.close()
+    return {"bits": LOOKAHEAD_BITS, "semantics": "cuda", "n_per_s": total_n / (lms * 1e-3),
+            "host_inclusive_n_per_s": total_n / d, "ms_per_step": d * 1e3, "kernel_ms": lms,
+            "candidates": sum(int(x.total["count"]) for x in o),
+            "checks": sum(int(x.checks) for x in o), "total_hash": "%08X" % lah}
+
+
 def tree_leg(cz, N):
     """The reference's own program (tree mode) for all n < 2^N from the root case line
     "3 2 2 9 1 0", its default knobs (LASTBITS 10, STEPSIZEINITIAL 24, 16 bit vectors,
@@ -224,6 +249,8 @@ def main():
                     help="MAXBITS of the reference-program leg (the whole tree below 2^N)")
     ap.add_argument("--only-tree", action="store_true",
                     help="run only the reference-program leg and print its record")
+    ap.add_argument("--only-lookahead", action="store_true",
+                    help="run only the look-ahead leg (batch workload, B = 16) and print it")
     ap.add_argument("--large-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-intervals", type=int, default=128)
@@ -253,6 +280,9 @@ def main():
 
     ivs = W.batch_intervals(0, 4096, K)
     total_n = sum(hi - lo for lo, hi in ivs)
+    if args.only_lookahead:
+        print(json.dumps(lookahead_leg(cz, ivs, total_n, args.steps)), flush=True)
+        return
     t = time.time()
     sv = cz.Sieve(K)
     info = sv.info()
@@ -385,31 +415,9 @@ def main():
         del s37
 
     if not args.no_lookahead:
-        # The same batch with the Step 2 look-ahead of B = LOOKAHEAD_BITS steps at depth 32:
-        # fewer candidates iterated (different counts and checksums by definition), the
-        # same integers verified.
-        sla = cz.Sieve(K, lookahead_bits=LOOKAHEAD_BITS)
-        sla.verify_batch(ivs, chunk_log2=BATCH_CHUNK, use_graph=True)
-        ms = []
-        device_sync()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            o = sla.verify_batch(ivs, chunk_log2=BATCH_CHUNK, use_graph=True)
-            ms.append(o[0].device_ms)
-        device_sync()
-        d = (time.perf_counter() - t0) / args.steps
-        lah = 0
-        for x in o:
-            lah ^= int(x.total["hash"])
-        lms = sum(ms) / len(ms)
-        line["lookahead"] = {
-            "bits": LOOKAHEAD_BITS, "semantics": "cuda", "n_per_s": total_n / (lms * 1e-3),
-            "host_inclusive_n_per_s": total_n / d, "ms_per_step": d * 1e3, "kernel_ms": lms,
-            "candidates": sum(int(x.total["count"]) for x in o),
-            "checks": sum(int(x.checks) for x in o), "total_hash": "%08X" % lah,
-            "vs_headline": (total_n / (lms * 1e-3)) / value}
+        line["lookahead"] = lookahead_leg(cz, ivs, total_n, args.steps)
+        line["lookahead"]["vs_headline"] = line["lookahead"]["n_per_s"] / value
         log(f"look-ahead B={LOOKAHEAD_BITS}: {line['lookahead']}")
-        sla.close()
 
     if not args.no_tree:
         line["reference_tree_mode"] = tree_leg(cz, args.tree_n)
