@@ -1,7 +1,8 @@
 // Autotuner for the persistent kernel.  For each (limbs, magnitude bucket) of a sieve it
 // times a short synthetic range of that magnitude under a small grid of launch shapes
 // (waves per CU, checks per refill) and keeps the fastest; the tile size (residues per
-// wave visit) follows the size of each launch.  Results are cached in the sieve handle; COLLATZ_AUTOTUNE=0 disables tuning.
+// wave visit) follows the size of each launch.  Results are cached per sieve handle and
+// bucket; COLLATZ_AUTOTUNE=0 disables tuning.
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -58,11 +59,13 @@ Tuning choose_tuning(const collatz_sieve* sv, int limbs, u128 max_hi, uint64_t c
     if (it != g_cache.end()) {
       t = it->second;
     } else {
-      // A range of the bucket's magnitude with about 2^27 candidates, each shape timed
-      // twice (the faster counts).
+      // A range of the bucket's magnitude with about 2^27 candidates (three times as many
+      // survivors for a look-ahead sieve, which keeps about a third), each shape timed three
+      // times (the fastest counts).
       const uint32_t bits = std::max<uint32_t>(bitlen(max_hi), sv->k + 1);
       const double density = (double)sv->count / (double)((u128)1 << sv->k) * 5.0 / 9.0;
       double w = (double)(1u << 27) / (density > 0 ? density : 1e-3);
+      if (sv->la_bits) w *= 3;
       u128 width = (u128)w;
       if (width < ((u128)1 << 20)) width = (u128)1 << 20;
       const u128 lo = (u128)1 << (bits - 1);
@@ -76,13 +79,13 @@ Tuning choose_tuning(const collatz_sieve* sv, int limbs, u128 max_hi, uint64_t c
       for (uint32_t b : wavess)
         for (int ui = 0; ui < nU; ++ui) {
           const uint32_t c = Us[ui];
-          const Tuning cand{adaptive_tile(sv, 1u << 27, b), b, c};
+          const Tuning cand{adaptive_tile(sv, sv->la_bits ? 3u << 27 : 1u << 27, b), b, c};
           double ms = 1e30;
-          for (int rep = 0; rep < 2; ++rep) {
+          for (int rep = 0; rep < 3; ++rep) {
             double m = 0;
             if (verify_batch_tuned(sv, lo, hi, limbs, cand, nullptr, &m) != 0) {
               m = 1e30;
-              rep = 2;
+              rep = 3;
             }
             ms = std::min(ms, m);
           }
