@@ -79,3 +79,42 @@ def test_reference_params_are_checked_before_the_device():
     p = cz.RefParams(30, 10, 8, 16, 0, 100, -1, 0)
     lib.collatz_reference_run(ctypes.byref(p), ctypes.byref(node), None, ctypes.byref(res))
     assert b"must not exceed B" in lib.collatz_last_error()
+
+
+def test_header_compiles_as_c_and_layouts_match(tmp_path):
+    """include/collatz.h is plain C99: a C program built against it links the library,
+    calls entry points that need no GPU, and sees the struct sizes the Python binding
+    declares."""
+    import shutil
+    import subprocess
+    import collatz_amd as cz
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if not cc:
+        import pytest
+        pytest.skip("no C compiler")
+    src = tmp_path / "abi.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include "collatz.h"
+int main(void) {
+  collatz_sieve* sv = 0;
+  collatz_sieve_params bad = {7, 1, -1, 0, COLLATZ_SEM_CUDA, 0};
+  int rc = collatz_sieve_build(&bad, &sv);
+  printf("%d %d %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", collatz_version(), rc,
+         sizeof(collatz_sieve_params), sizeof(collatz_sieve_info), sizeof(collatz_chunk),
+         sizeof(collatz_range_result), sizeof(collatz_verify_opts), sizeof(collatz_batch_opts),
+         sizeof(collatz_ref_params), sizeof(collatz_ref_result), sizeof(collatz_server_opts));
+  return 0;
+}
+''')
+    exe = tmp_path / "abi"
+    lib = os.path.join(ROOT, "collatz_amd")
+    subprocess.run([cc, "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    str(src), "-o", str(exe), "-L", lib, "-lcollatz_amd",
+                    "-Wl,-rpath," + lib], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    v, rc, *sizes = (int(x) for x in out)
+    assert v == cz.load().collatz_version() and rc == cz.EINVAL
+    assert sizes == [ctypes.sizeof(t) for t in (cz.SieveParams, cz.SieveInfo, cz.Chunk,
+                                                cz.RangeResult, cz.VerifyOpts, cz.BatchOpts,
+                                                cz.RefParams, cz.RefResult, cz.ServerOpts)]
