@@ -68,3 +68,23 @@ def test_k37_sieve_and_workload_slices():
     assert int(a.chunks["count"].sum()) == int(a.total["count"])
     first = _fixture_chunks(fx["slices"][0])
     assert_chunks_equal(a.chunks[:1], first, "first chunk of the whole run")
+
+
+@pytest.mark.parametrize("k", [35, 38])
+def test_range_sieve_counts_equal_reference_mode_leaves(k):
+    """The range sieve of depth k and reference mode at N = k + 10 walk the same search1
+    tree from the root "3 2 2 9 1 0" by different routes (one bit array over all residues;
+    frontier roots expanded one after another): |S_k| equals the reference-mode leaf count.
+    At k = 35 the leaves per bit vector are the reference's own (ref_full45.json)."""
+    import json
+    import os
+    import collatz_amd as cz
+    sv = cz.Sieve(k)
+    n = int(sv.info().survivors)
+    sv.close()
+    r = cz.reference_run((3, 2, 2, 9, 1, 0), k + 10, 10, semantics=cz.SEM_CUDA)
+    assert n == r.leaves == sum(r.bin_leaves)
+    if k == 35:
+        here = os.path.dirname(os.path.abspath(__file__))
+        g = json.load(open(os.path.join(here, "golden", "ref_full45.json")))
+        assert list(r.bin_leaves)[:16] == g["bin_leaves"]
