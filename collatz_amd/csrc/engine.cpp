@@ -68,7 +68,7 @@ int plan_interval(const collatz_sieve* sv, u128 lo, u128 hi, uint32_t C, uint64_
 
 bool Engine::GraphKey::operator==(const GraphKey& o) const {
   return std::memcmp(bufs, o.bufs, sizeof(bufs)) == 0 && limbs == o.limbs &&
-         cutoff == o.cutoff && nblocks == o.nblocks && U == o.U;
+         cutoff == o.cutoff && nblocks == o.nblocks && U == o.U && max_steps == o.max_steps;
 }
 
 Engine::~Engine() {
@@ -168,6 +168,7 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   key.nblocks =
       (uint32_t)std::max<uint64_t>(1, (uint64_t)device_cu_count(sv_->device) * waves / 4);
   key.U = tn.U;
+  key.max_steps = lim.max_steps;
 
   CZ_HIP(hipEventRecord(e0_, s));
   if (graph) {

@@ -99,7 +99,7 @@ class Engine {
   struct GraphKey {
     const void* bufs[8];
     int limbs;
-    uint32_t cutoff, nblocks, U;
+    uint32_t cutoff, nblocks, U, max_steps;
     bool operator==(const GraphKey& o) const;
   } gkey_{};
   hipError_t enqueue(const GraphKey& k, const Limits& lim, DeviceLists lists, hipStream_t s);

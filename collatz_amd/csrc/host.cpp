@@ -99,6 +99,15 @@ int launch_width(int limbs, u128 max_hi, uint32_t cutoff) {
 Limits make_limits(int limbs, uint32_t cutoff) {
   Limits l;
   l.cutoff = cutoff;
+  l.max_checks = kMaxChecks;
+  l.max_steps = kMaxSteps;
+  if (const char* e = std::getenv("COLLATZ_CHECK_LIMIT")) {
+    const unsigned long v = std::strtoul(e, nullptr, 10);
+    if (v >= 1 && v < kMaxChecks) {
+      l.max_checks = (uint32_t)v;
+      l.max_steps = (uint32_t)(2 * v);
+    }
+  }
   if (limbs == 1) {
     l.fast_word = 2;
     l.fast_thr = 1ull << (kNarrowLog2 - 64);

@@ -123,6 +123,12 @@ struct Limits {
   uint32_t cutoff;       // overflow at number >= 2^cutoff
   uint32_t fast_word;    // index of the low word of the 64-bit pair checked on the fast path
   uint64_t fast_thr;     // fast path: pair >= fast_thr, or any word above the pair nonzero
+  // A candidate that has not descended after max_checks checks (the persistent kernel
+  // counts max_steps T-steps instead) is abandoned and the call fails: kMaxChecks and
+  // kMaxSteps, or less under COLLATZ_CHECK_LIMIT (a test hook that forces the failure
+  // path from the host).
+  uint32_t max_checks;
+  uint32_t max_steps;
 };
 
 // One survivor as the range kernels read it, 16 bytes in one load:

@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
         }
         h ^= v.w[0];
         done = lt_base<L>(v, pair(n[0], n[1]), pair(n[2], n[3]));
-        if (++checks >= kMaxChecks && !done) {
+        if (++checks >= lim.max_checks && !done) {
           stall += 1;
           done = true;
         }
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
           bool done = lt_base<L>(v, n_lo, 0);
           bool wide = false;
           if constexpr (L == 3) wide = v.w[2] >= (1u << (kNarrowCheckLog2 - 64));
-          if (++checks >= kMaxChecks && !done && !wide) {
+          if (++checks >= lim.max_checks && !done && !wide) {
             stall += 1;
             done = true;
           }

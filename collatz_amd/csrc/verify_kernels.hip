@@ -200,7 +200,7 @@ __device__ __forceinline__ bool run_candidate(Cand<L>& c, Acc<L>& acc, const Lim
                                               const Pow3Lds& p3) {
   SinkOne<L> sink{acc};
   if (!cand_enter<L, LOWCUT>(c, sink)) return false;
-  for (uint32_t guard = 0; guard < kMaxChecks; ++guard) {
+  for (uint32_t guard = 0; guard < lim.max_checks; ++guard) {
     bool esc, done;
     one_check<L, LOWCUT>(c, sink, lim, lists, p3, esc, done);
     acc.chk += 1;
@@ -466,7 +466,7 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
           bool esc, done;
           one_check<L, LOWCUT>(c, sink, lim, lists, p3s, esc, done);
           acc.chk += 1;
-          if (esc || done || c.steps >= kMaxSteps) {
+          if (esc || done || c.steps >= lim.max_steps) {
             if (esc) {
               // (The empty asm keeps the index sum in this rare branch: hoisted, it would
               // cost two instructions on every refill.)
