@@ -44,7 +44,8 @@ RANGE_LO, RANGE_WIDTH, RANGE_CHUNK = 1 << 40, 1 << 36, 24
 # Algorithmic integer work, in 32-bit lane operations, for L 32-bit words per value: per
 # check (two odd and two even runs, hash and compare) OPS_DR = 13 L + 7, and per candidate
 # (mod 9, start value, base) OPS_SETUP = 4 L + 8.  Two-limb work whose starts stay below
-# 2^68 runs on L = 3 words (96 bits, escalating at 2^91); otherwise L = 4.
+# 2^68 runs on L = 3 words (96 bits; a candidate whose check ends at or above 2^86 is
+# rerun on three limbs); otherwise L = 4.
 def ops_dr(L):
     return 13 * L + 7
 
@@ -63,10 +64,10 @@ RATES = (50, 100, 200, 500)
 # HBM traffic of the batched launch, measured with rocprofv3 on this workload
 # (profiles/k_verify_persistent_batch4096_k32.md): FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0,
 # the factors a 1 GiB copy kernel gives.  Reads are the 16-byte survivor record of each
-# candidate (15.79 B per candidate measured, L2 hits make up the rest); writes are the
+# candidate (15.74 B per candidate measured, L2 hits make up the rest); writes are the
 # per-tile records.
-READ_BYTES_PER_CANDIDATE = 15.79
-WRITE_BYTES_PER_LAUNCH = 0.155e9
+READ_BYTES_PER_CANDIDATE = 15.74
+WRITE_BYTES_PER_LAUNCH = 0.078e9
 
 
 def log(msg):
@@ -239,6 +240,11 @@ def main():
     ap.add_argument("--range-steps", type=int, default=20)
     ap.add_argument("--serve-seconds", type=float, default=5.0,
                     help="length of each served replay at a fixed rate")
+    ap.add_argument("--serve-rates", default=",".join(str(r) for r in RATES),
+                    help="comma-separated request rates (req/s) of the served replays")
+    ap.add_argument("--serve-all", action="store_true",
+                    help="replay the whole 20000-request stream at each rate, and all of it "
+                         "at once for the saturated figure")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-range", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -352,23 +358,31 @@ def main():
 
     if not args.no_plain:
         plain_s = 0.0
+        plain_dev_ms = 0.0
         same = True
         for p in range(args.plain_passes):
             t0 = time.perf_counter()
             pouts = [sv.verify_range(lo, hi, BATCH_CHUNK, form=cz.FORM_PLAIN) for lo, hi in ivs]
             plain_s += time.perf_counter() - t0
+            plain_dev_ms += sum(o.device_ms for o in pouts)
             if p == 0:
                 for a, b in zip(outs, pouts):
                     if not (records_equal(a.chunks, b.chunks) and
                             a.total.tobytes() == b.total.tobytes()):
                         same = False
         plain_rate = total_n / (plain_s / args.plain_passes)
+        plain_dev_rate = total_n / (plain_dev_ms / args.plain_passes * 1e-3)
         line["plain"] = {"n_per_s": plain_rate, "ms_per_pass": plain_s / args.plain_passes * 1e3,
+                         "device_n_per_s": plain_dev_rate,
+                         "device_ms_per_pass": plain_dev_ms / args.plain_passes,
                          "calls_per_pass": len(ivs)}
-        # Both forms by the wall time of their calls (4096 plain calls against one batch).
+        # Both forms by the wall time of their calls (4096 plain calls against one batch),
+        # and by device time alone (the kernels of the 4096 calls against the batch's).
         line["batched_vs_plain"] = value_host / plain_rate
+        line["batched_vs_plain_device"] = value / plain_dev_rate
         line["bit_exact_batched_vs_plain"] = same
-        log(f"plain: {plain_rate:.3e} n/s, batched/plain {value_host / plain_rate:.2f}, "
+        log(f"plain: {plain_rate:.3e} n/s ({plain_dev_rate:.3e} by device time), batched/plain "
+            f"{value_host / plain_rate:.2f} ({value / plain_dev_rate:.2f} by device time), "
             f"identical records {same}")
 
     if not args.no_range:
@@ -381,8 +395,15 @@ def main():
                 o = sv.verify_range(RANGE_LO, RANGE_LO + RANGE_WIDTH, RANGE_CHUNK, form=form)
                 ms.append(o.device_ms)
             d = (time.perf_counter() - t0) / args.range_steps
+            dms = sum(ms) / len(ms)
+            Lr = words_for(RANGE_LO + RANGE_WIDTH)
+            ach = (int(o.checks) * ops_dr(Lr) + int(o.total["count"]) * ops_setup(Lr)) / \
+                (dms * 1e-3) / 1e12
             rng[name] = {"n_per_s": RANGE_WIDTH / d, "ms_per_call": d * 1e3,
-                         "device_ms": sum(ms) / len(ms), "hash": "%08X" % int(o.total["hash"])}
+                         "device_ms": dms, "device_n_per_s": RANGE_WIDTH / (dms * 1e-3),
+                         "candidates": int(o.total["count"]), "checks": int(o.checks),
+                         "roofline_frac": ach / VALU_PEAK_TOPS,
+                         "hash": "%08X" % int(o.total["hash"])}
         line["range_2p40_w2p36"] = rng
         log(f"range: plain {rng['plain']['n_per_s']:.3e}, persistent "
             f"{rng['persistent']['n_per_s']:.3e} n/s")
@@ -426,11 +447,13 @@ def main():
     if not args.no_serve:
         reqs = W.request_stream(1, 20000)
         served = {}
-        for rate in RATES:
-            n = min(len(reqs), max(50, int(rate * args.serve_seconds)))
+        for rate in (int(x) for x in args.serve_rates.split(",") if x):
+            n = len(reqs) if args.serve_all else \
+                min(len(reqs), max(50, int(rate * args.serve_seconds)))
             served[str(rate)] = served_leg(sv, reqs[:n], rate, BATCH_CHUNK)
             log(f"served at {rate} req/s: {served[str(rate)]}")
-        served["saturated"] = served_leg(sv, reqs[:2000], None, BATCH_CHUNK)
+        served["saturated"] = served_leg(sv, reqs if args.serve_all else reqs[:2000], None,
+                                         BATCH_CHUNK)
         log(f"served saturated: {served['saturated']}")
         line["served"] = served
 

@@ -212,7 +212,7 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   out.esc_slots.clear();
   if (esc_n) {
     CZ_HIP(hipEventRecord(e2_, s));
-    rc = run_escalated(sv_, esc_, esc_n, cutoff, lists, out.recs, 0, s);
+    rc = run_escalated(sv_, esc_, esc_n, cutoff, lists, out.recs, 0, esc_recs_, s);
     if (rc) return rc;
     CZ_HIP(hipEventRecord(e3_, s));
     CZ_HIP(hipEventSynchronize(e3_));

@@ -83,7 +83,7 @@ class Engine {
   DevBuf<RowDesc> rows_;
   DevBuf<uint64_t> keys_, part_, ovf_;
   DevBuf<TileDesc> tiles_;
-  DevBuf<DevRec> tile_recs_, slot_recs_;
+  DevBuf<DevRec> tile_recs_, slot_recs_, esc_recs_;
   DevBuf<EscEntry> esc_;
   DevBuf<unsigned long long> cnt_;
   DevBuf<Ctl> ctl_;
@@ -106,9 +106,10 @@ class Engine {
 };
 
 // Escalated candidates of a launch on three limbs, folded into recs by slot.
+// `out` is the caller's scratch for the pass's records (grown as needed).
 int run_escalated(const collatz_sieve* sv, const DevBuf<EscEntry>& esc, uint64_t n,
                   uint32_t cutoff, DeviceLists lists, std::vector<DevRec>& recs,
-                  uint32_t slot_base, hipStream_t s);
+                  uint32_t slot_base, DevBuf<DevRec>& out, hipStream_t s);
 
 int device_cu_count(int device);
 

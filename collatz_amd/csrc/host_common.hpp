@@ -118,7 +118,28 @@ struct RowPlan {
 
 int plan_rows(uint32_t k, u128 lo, u128 hi, uint32_t C, uint32_t slot_base, bool mod9,
               RowPlan& plan, uint64_t max_rows);
-int resolve_rows(const struct ::collatz_sieve* sv, RowPlan& plan, hipStream_t s);
+
+// Device buffers and events of the plain form, grown on demand and kept for later calls
+// (a call then makes no device allocation unless it needs more room than any before it).
+struct PlainScratch {
+  DevBuf<uint64_t> keys, key_idx, part, ovf;
+  DevBuf<uint32_t> key_cls;
+  DevBuf<RowDesc> rows;
+  DevBuf<DevRec> block, chunk, esc_recs;
+  DevBuf<EscEntry> esc;
+  DevBuf<unsigned long long> cnt;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  PlainScratch() = default;
+  PlainScratch(const PlainScratch&) = delete;
+  PlainScratch& operator=(const PlainScratch&) = delete;
+  ~PlainScratch() {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+int resolve_rows(const struct ::collatz_sieve* sv, RowPlan& plan, PlainScratch& scr,
+                 hipStream_t s);
 
 Limits make_limits(int limbs, uint32_t cutoff);
 // Word width of a launch for a two- or three-limb request: two-limb work whose starts are
@@ -172,6 +193,10 @@ struct collatz_sieve {
   // share a sieve handle are serialised on engine_mu.
   mutable cz::Engine* engine = nullptr;
   mutable std::mutex engine_mu;
+  // Buffers of the plain form; plain calls that share a sieve handle are serialised on
+  // plain_mu.
+  mutable cz::PlainScratch plain;
+  mutable std::mutex plain_mu;
   cz::SieveArrays arrays() const {
     cz::SieveArrays a;
     a.rec = rec.p;

@@ -25,21 +25,19 @@ static int check_opts(const collatz_verify_opts* o, RangeArgs& a) {
 }
 
 // Fills i_lo / i_hi of every row from the sieve with one device lookup.
-int resolve_rows(const collatz_sieve* sv, RowPlan& plan, hipStream_t s) {
+int resolve_rows(const collatz_sieve* sv, RowPlan& plan, PlainScratch& scr, hipStream_t s) {
   std::vector<uint64_t> idx(plan.keys.size());
   if (!plan.keys.empty()) {
-    DevBuf<uint64_t> dkeys, dout;
-    DevBuf<uint32_t> dcls;
-    CZ_HIP(dkeys.alloc(plan.keys.size()));
-    CZ_HIP(dcls.alloc(plan.keys.size()));
-    CZ_HIP(dout.alloc(plan.keys.size()));
-    CZ_HIP(hipMemcpyAsync(dkeys.p, plan.keys.data(), plan.keys.size() * 8,
-                          hipMemcpyHostToDevice, s));
-    CZ_HIP(hipMemcpyAsync(dcls.p, plan.key_cls.data(), plan.key_cls.size() * 4,
-                          hipMemcpyHostToDevice, s));
-    CZ_HIP(launch_lower_bound(sv->rec.p, sv->class_start_dev.p, dkeys.p, dcls.p,
-                              plan.keys.size(), dout.p, s));
-    CZ_HIP(hipMemcpyAsync(idx.data(), dout.p, idx.size() * 8, hipMemcpyDeviceToHost, s));
+    const size_t nk = plan.keys.size();
+    CZ_HIP(scr.keys.reserve(nk));
+    CZ_HIP(scr.key_cls.reserve(nk));
+    CZ_HIP(scr.key_idx.reserve(nk));
+    CZ_HIP(hipMemcpyAsync(scr.keys.p, plan.keys.data(), nk * 8, hipMemcpyHostToDevice, s));
+    CZ_HIP(hipMemcpyAsync(scr.key_cls.p, plan.key_cls.data(), nk * 4, hipMemcpyHostToDevice,
+                          s));
+    CZ_HIP(launch_lower_bound(sv->rec.p, sv->class_start_dev.p, scr.keys.p, scr.key_cls.p, nk,
+                              scr.key_idx.p, s));
+    CZ_HIP(hipMemcpyAsync(idx.data(), scr.key_idx.p, nk * 8, hipMemcpyDeviceToHost, s));
     CZ_HIP(hipStreamSynchronize(s));
   }
   for (size_t i = 0; i < plan.rows.size(); ++i) {
@@ -54,14 +52,13 @@ int resolve_rows(const collatz_sieve* sv, RowPlan& plan, hipStream_t s) {
 // Runs the escalated candidates on three limbs and folds them into recs (by slot).
 int run_escalated(const collatz_sieve* sv, const DevBuf<EscEntry>& esc, uint64_t n,
                   uint32_t cutoff, DeviceLists lists, std::vector<DevRec>& recs,
-                  uint32_t slot_base, hipStream_t s) {
+                  uint32_t slot_base, DevBuf<DevRec>& out, hipStream_t s) {
   if (n == 0) return COLLATZ_OK;
   // The list holds at most esc_cap entries; more escalations than that were dropped by the
   // kernels, so the range must be rerun on three limbs instead.
   if (n > esc.n || n > lists.esc_cap)
     return fail(COLLATZ_ECAPACITY, "escalation list overflowed; rerun with limbs = 3");
-  DevBuf<DevRec> out;
-  CZ_HIP(out.alloc(n));
+  CZ_HIP(out.reserve(n));
   const Limits lim3 = make_limits(3, cutoff);
   CZ_HIP(launch_escalated(sv->arrays(), esc.p, n, 0, lim3, sv->pow3.p, out.p, lists, s));
   std::vector<DevRec> h(n);
@@ -77,12 +74,14 @@ int run_escalated(const collatz_sieve* sv, const DevBuf<EscEntry>& esc, uint64_t
   return COLLATZ_OK;
 }
 
+// The caller holds sv->plain_mu.
 static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs& a,
                         hipStream_t s, collatz_range_result* out) {
+  PlainScratch& scr = sv->plain;
   RowPlan plan;
   int rc = plan_rows(sv->k, lo, hi, a.C, 0, sv->mod9 != 0, plan, 1ull << 26);
   if (rc) return rc;
-  rc = resolve_rows(sv, plan, s);
+  rc = resolve_rows(sv, plan, scr, s);
   if (rc) return rc;
   // Blocks of 256 lanes; each lane takes R residues of its row in turn.
   uint64_t slots = 0;
@@ -101,26 +100,26 @@ static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeAr
     const uint64_t row = plan.chunk_first_row[c];
     part_start[c] = row < plan.rows.size() ? plan.rows[row].block_start : nblocks;
   }
-  // Rows without blocks would confuse the block -> row search; give them a sentinel start.
-  DevBuf<RowDesc> drows;
-  DevBuf<uint64_t> dpart;
-  DevBuf<DevRec> dblock, dchunk;
-  CZ_HIP(drows.alloc(plan.rows.size()));
-  CZ_HIP(dpart.alloc(part_start.size()));
-  CZ_HIP(dblock.alloc(std::max<uint64_t>(1, nblocks)));
-  CZ_HIP(dchunk.alloc(plan.nchunks));
+  DevBuf<RowDesc>& drows = scr.rows;
+  DevBuf<uint64_t>& dpart = scr.part;
+  DevBuf<DevRec>& dblock = scr.block;
+  DevBuf<DevRec>& dchunk = scr.chunk;
+  CZ_HIP(drows.reserve(plan.rows.size()));
+  CZ_HIP(dpart.reserve(part_start.size()));
+  CZ_HIP(dblock.reserve(std::max<uint64_t>(1, nblocks)));
+  CZ_HIP(dchunk.reserve(plan.nchunks));
   CZ_HIP(hipMemcpyAsync(drows.p, plan.rows.data(), plan.rows.size() * sizeof(RowDesc),
                         hipMemcpyHostToDevice, s));
   CZ_HIP(hipMemcpyAsync(dpart.p, part_start.data(), part_start.size() * 8,
                         hipMemcpyHostToDevice, s));
   const uint64_t esc_cap = 1ull << 20;
   const uint64_t ovf_cap = std::max<uint64_t>(out->overflow_cap, 1024);
-  DevBuf<EscEntry> desc;
-  DevBuf<uint64_t> dovf;
-  DevBuf<unsigned long long> dcnt;
-  CZ_HIP(desc.alloc(a.width < 3 ? esc_cap : 1));
-  CZ_HIP(dovf.alloc(4 * ovf_cap));
-  CZ_HIP(dcnt.alloc(3));
+  DevBuf<EscEntry>& desc = scr.esc;
+  DevBuf<uint64_t>& dovf = scr.ovf;
+  DevBuf<unsigned long long>& dcnt = scr.cnt;
+  CZ_HIP(desc.reserve(a.width < 3 ? esc_cap : 1));
+  CZ_HIP(dovf.reserve(4 * ovf_cap));
+  CZ_HIP(dcnt.reserve(3));
   CZ_HIP(hipMemsetAsync(dcnt.p, 0, 24, s));
   DeviceLists lists;
   lists.esc = desc.p;
@@ -132,15 +131,11 @@ static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeAr
   lists.stall_count = dcnt.p + 2;
   const Limits lim = make_limits((int)a.width, a.cutoff);
   // Device time: the verification and reduction kernels, then the three-limb pass.
-  struct Events {
-    hipEvent_t a = nullptr, b = nullptr;
-    ~Events() {
-      if (a) (void)hipEventDestroy(a);
-      if (b) (void)hipEventDestroy(b);
-    }
-  } ev;
-  CZ_HIP(hipEventCreate(&ev.a));
-  CZ_HIP(hipEventCreate(&ev.b));
+  for (hipEvent_t& e : scr.ev)
+    if (!e) CZ_HIP(hipEventCreate(&e));
+  struct {
+    hipEvent_t a, b;
+  } ev{scr.ev[0], scr.ev[1]};
   CZ_HIP(hipEventRecord(ev.a, s));
   CZ_HIP(launch_verify_plain((int)a.width, sv->arrays(), drows.p, (uint32_t)plan.rows.size(),
                              nblocks, R, sv->mod9, lim, sv->pow3.p, dblock.p, lists, s));
@@ -163,7 +158,7 @@ static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeAr
   uint64_t ovf_n = cnt[1];
   if (cnt[0]) {
     CZ_HIP(hipEventRecord(ev.a, s));
-    rc = run_escalated(sv, desc, cnt[0], a.cutoff, lists, recs, 0, s);
+    rc = run_escalated(sv, desc, cnt[0], a.cutoff, lists, recs, 0, scr.esc_recs, s);
     if (rc) return rc;
     CZ_HIP(hipEventRecord(ev.b, s));
     CZ_HIP(hipEventSynchronize(ev.b));
@@ -233,5 +228,6 @@ extern "C" int collatz_verify_range(const collatz_sieve* sv, const uint64_t lo[2
     return collatz_verify_batch(sv, 1, &iv, &bo, stream, out);
   }
   a.width = (uint32_t)launch_width((int)a.limbs, H, a.cutoff);
+  std::lock_guard<std::mutex> lk(sv->plain_mu);
   return verify_plain(sv, L, H, a, s, out);
 }

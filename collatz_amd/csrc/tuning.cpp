@@ -32,19 +32,27 @@ static uint32_t bitlen(u128 x) {
 // SIMD).  Checks between refills: 2.
 static const uint32_t kDefaultWaves = 32, kDefaultU = 2;
 
-// Survivors per tile: the largest power of two in [4096, 65536] that still gives every
-// resident wave about 16 tiles, so that work stealing evens out the tail while the per-tile
-// set-up (the q * 3^f table, the record merge) stays small.  A look-ahead keeps about a
-// third of the survivors, so tiles of such a sieve may be four times larger.  A launch too
+// Survivors per tile: the largest power of two in [4096, 131072] that still gives every
+// resident wave about 16 tiles.  Each tile ends with a tail in which lanes fall idle while
+// its last trajectories finish, so larger tiles waste less; but the tiles of a launch that
+// mixes ranges of very different sizes differ in size too (a row shorter than the tile is
+// a tile of its own), and with few tiles per wave the largest ones set the launch's end.
+// (Measured: one tile per wave ran [2^40, 2^40 + 2^36) in 2.04 ms against 2.16 ms, but
+// cut the saturated served stream from 3.7e13 to 3.2e13 n/s.)  A look-ahead keeps about a
+// third of the survivors, so tiles of such a sieve may be twice as large.  A launch too
 // small to give every resident wave a tile of 4096 takes smaller tiles (down to 256) until
 // it does: its duration is about one tile per wave, so this is its latency.
 uint32_t adaptive_tile(const collatz_sieve* sv, uint64_t candidates, uint32_t waves) {
   const uint64_t nwaves = (uint64_t)device_cu_count(sv->device) * waves;
-  uint32_t t = sv->la_bits ? 262144 : 65536;
+  uint32_t t = sv->la_bits ? 262144 : 131072;
   while (t > 4096 && candidates / t < 16 * nwaves) t >>= 1;
   while (t > 256 && candidates / t < nwaves) t >>= 1;
   return t;
 }
+
+// Tile of the tuner's timing launches: a sample of 2^27 candidates in tiles of 4096 gives
+// every wave several tiles, as the large launches the tuning is for do.
+static constexpr uint32_t kTuneTile = 4096;
 
 Tuning choose_tuning(const collatz_sieve* sv, int limbs, u128 max_hi, uint64_t candidates,
                      uint32_t tile, uint32_t waves, uint32_t U) {
@@ -79,7 +87,7 @@ Tuning choose_tuning(const collatz_sieve* sv, int limbs, u128 max_hi, uint64_t c
       for (uint32_t b : wavess)
         for (int ui = 0; ui < nU; ++ui) {
           const uint32_t c = Us[ui];
-          const Tuning cand{adaptive_tile(sv, sv->la_bits ? 3u << 27 : 1u << 27, b), b, c};
+          const Tuning cand{kTuneTile, b, c};
           double ms = 1e30;
           for (int rep = 0; rep < 3; ++rep) {
             double m = 0;
