@@ -220,10 +220,14 @@ def tree_leg(cz, N):
     t = time.perf_counter()
     r = cz.reference_run((3, 2, 2, 9, 1, 0), N, 10, semantics=cz.SEM_CUDA)
     d = time.perf_counter() - t
+    # Algorithmic integer work at L = 3 (the reference's candidates start below 2^96 and run
+    # on 96 bits) over the whole call's wall time: sieve, tables and both iteration passes.
+    ach = (int(r.checks) * ops_dr(3) + int(r.candidates) * ops_setup(3)) / d / 1e12
     tree = {"N": N, "A": 10, "B": 24, "semantics": "cuda", "seconds": d,
             "n_per_s": (1 << N) / d, "complete_hash": "%X" % r.complete_hash,
             "launches": int(r.launches), "leaves": int(r.leaves),
-            "candidates": int(r.candidates)}
+            "candidates": int(r.candidates), "checks": int(r.checks),
+            "roofline_frac": ach / VALU_PEAK_TOPS}
     if N == 50:
         tree["published"] = {"hardware": "RTX 3060", "seconds": 166.0, "n_per_s": 6.8e12,
                              "source": "paper p.3 section 1.1"}

@@ -106,7 +106,7 @@ class RefParams(C.Structure):
 
 class RefResult(C.Structure):
     _fields_ = [("complete_hash", C.c_uint32), ("pad", C.c_uint32), ("launches", C.c_uint64),
-                ("leaves", C.c_uint64), ("candidates", C.c_uint64),
+                ("leaves", C.c_uint64), ("candidates", C.c_uint64), ("checks", C.c_uint64),
                 ("bin_leaves", C.c_uint64 * 17), ("overflow_count", C.c_uint64),
                 ("overflow_starts", C.POINTER(C.c_uint64)), ("overflow_cap", C.c_uint64),
                 ("overflow_written", C.c_uint64), ("device_ms", C.c_double)]

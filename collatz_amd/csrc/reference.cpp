@@ -30,6 +30,7 @@ struct RefTotalsHost {
   unsigned int hash;
   unsigned int tile_counter;
   unsigned long long esc_count;
+  unsigned long long checks;
 };
 
 struct RefTables {
@@ -297,6 +298,7 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
   out->complete_hash = tot.hash;
   out->leaves = leaves;
   out->candidates = tot.count;
+  out->checks = tot.checks;
   out->overflow_count = tot.overflow;
   out->launches = 0;
   for (int b = 0; b < 17; ++b) out->bin_leaves[b] = bins[b];

@@ -150,6 +150,7 @@ struct RefTotals {
   unsigned int hash;
   unsigned int tile_counter;
   unsigned long long esc_count;  // candidates handed from 96 bits to the wide pass
+  unsigned long long checks;     // checks run (counted per wave from its active-lane mask)
 };
 
 // num >= 2^c, or (strict) num > 2^c.
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
     bool act = false, strict = false;
     Num<L> v;
     uint32_t n[4];
-    uint32_t checks = 0;
+    uint32_t checks = 0, wave_chk = 0;
     for (;;) {
       const uint64_t need = __ballot(!act);
       if (need && cur < end) {
@@ -222,7 +223,9 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
         }
         cur += take;
       }
-      if (__ballot(act) == 0) break;
+      const uint64_t running = __ballot(act);
+      if (running == 0) break;
+      wave_chk += (uint32_t)__popcll(running);
       if (act) {
         bool done = false;
 #pragma unroll
@@ -263,6 +266,7 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
     if (lane == 0) {
       atomicXor(&tot->hash, h);
       atomicAdd(&tot->count, (unsigned long long)cnt);
+      atomicAdd(&tot->checks, (unsigned long long)wave_chk);
       if (stall) atomicAdd(&tot->stall, (unsigned long long)stall);
       if (!ovf && ov) atomicAdd(&tot->overflow, (unsigned long long)ov);
     }
@@ -318,6 +322,7 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
     const uint32_t words = (uint32_t)(end - first) << nw_log2;
     uint32_t wcur = 0, qhead = 0, qtail = 0;
     uint32_t h = 0, cnt = 0, ov = 0, stall = 0;
+    uint32_t lane_chk = 0;  // checks of the lane's finished candidates
     // Prefetched candidate of the lane: its leaf (index, base, number, meta) and a.
     bool pf = false;
     uint32_t pf_leaf = 0, pf_a = 0, pf_meta = 0;
@@ -428,6 +433,13 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
         }
       }
       if (__ballot(act || pf) == 0) break;
+      // A candidate past the check bound is abandoned (tested once per refill: it may run
+      // up to U checks beyond the bound first).
+      if (act && __builtin_expect(checks >= lim.max_checks, 0)) {
+        stall += 1;
+        lane_chk += checks;
+        act = false;
+      }
       // (3) Up to U checks; a lane whose candidate ends before the last of them starts its
       //     prefetched candidate at once.
       for (uint32_t u = 0; u < U; ++u) {
@@ -456,11 +468,9 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
           bool done = lt_base<L>(v, n_lo, 0);
           bool wide = false;
           if constexpr (L == 3) wide = v.w[2] >= (1u << (kNarrowCheckLog2 - 64));
-          if (++checks >= lim.max_checks && !done && !wide) {
-            stall += 1;
-            done = true;
-          }
+          ++checks;
           if (done || wide) {
+            lane_chk += checks;
             if (wide) {
               const unsigned long long i = atomicAdd(&tot->esc_count, 1ull);
               if (i < esc_cap)
@@ -481,11 +491,13 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
       h ^= __shfl_xor(h, k, 64);
       cnt += __shfl_xor(cnt, k, 64);
       stall += __shfl_xor(stall, k, 64);
+      lane_chk += __shfl_xor(lane_chk, k, 64);
       if (!ovf) ov += __shfl_xor(ov, k, 64);
     }
     if (lane == 0) {
       atomicXor(&tot->hash, h);
       atomicAdd(&tot->count, (unsigned long long)cnt);
+      atomicAdd(&tot->checks, (unsigned long long)lane_chk);
       if (stall) atomicAdd(&tot->stall, (unsigned long long)stall);
       if (!ovf && ov) atomicAdd(&tot->overflow, (unsigned long long)ov);
     }

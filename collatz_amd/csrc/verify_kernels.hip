@@ -371,6 +371,9 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
     Acc<L> acc;
     acc.init();
     SinkOne<L> sink{acc};
+    // Checks of the tile, counted per wave from the mask of lanes that run one (scalar
+    // work; the lane accumulators' chk stays 0).
+    uint32_t wave_chk = 0;
     Cand<L> c;
     c.n_hi = qb_hi;  // the same for every candidate of the tile
     bool act = false;
@@ -459,31 +462,36 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
         cur += take;
       }
       if (__ballot(act || pf) == 0) break;
+      // A candidate past the step bound is abandoned (tested once per refill: it may run up
+      // to U checks beyond the bound first).
+      if (act && __builtin_expect(c.steps >= lim.max_steps, 0)) {
+        atomicAdd(lists.stall_count, 1ull);
+        act = false;
+      }
       // (3) U checks; a lane whose candidate ends before the last of them starts its
       //     prefetched candidate at once instead of idling.
       for (uint32_t u = 0; u < U; ++u) {
+        wave_chk += (uint32_t)__popcll(__ballot(act));
         if (act) {
           bool esc, done;
           one_check<L, LOWCUT>(c, sink, lim, lists, p3s, esc, done);
-          acc.chk += 1;
-          if (esc || done || c.steps >= lim.max_steps) {
-            if (esc) {
-              // (The empty asm keeps the index sum in this rare branch: hoisted, it would
-              // cost two instructions on every refill.)
-              uint32_t off = cidx;
-              asm volatile("" : "+v"(off));
-              push_escalate(lists, c.n_lo, c.n_hi, first + off, slot);
-            }
-            else if (done) finish<L>(c, acc, sv.k);
-            else atomicAdd(lists.stall_count, 1ull);
-            act = false;
+          if (__builtin_expect(esc, 0)) {
+            // (The empty asm keeps the index sum in this rare branch: hoisted, it would
+            // cost two instructions on every refill.)
+            uint32_t off = cidx;
+            asm volatile("" : "+v"(off));
+            push_escalate(lists, c.n_lo, c.n_hi, first + off, slot);
+          } else if (done) {
+            finish<L>(c, acc, sv.k);
           }
+          act = !(esc || done);
         }
         if (u + 1 < U && !act && pf) start_prefetched();
       }
     }
     DevRec r = acc_to_rec<L>(acc);
     wave_merge(r);
+    r.checks = wave_chk;
     if (lane == 0) tile_recs[t] = r;
   }
 }

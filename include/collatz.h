@@ -183,6 +183,7 @@ typedef struct {
   uint64_t launches;          /* "Number of kernel launches = %ld" */
   uint64_t leaves;            /* leaves at depth N - A */
   uint64_t candidates;        /* candidates iterated */
+  uint64_t checks;            /* checks run (a candidate rerun wider counts both runs) */
   uint64_t bin_leaves[17];    /* leaves per bigindex 0..15, and >= 16 */
   uint64_t overflow_count;
   uint64_t* overflow_starts;  /* caller-owned, two words per start, or NULL */

@@ -91,6 +91,7 @@ def test_full_tree_n45_matches_reference():
         out = sv.verify_range(1, 1 << 45, 40, cutoff_log2=100, form=form)
         assert int(out.total["hash"]) == g["complete_hash"] == ref.complete_hash, form
         assert int(out.total["count"]) == ref.candidates
+        assert out.checks == ref.checks
         assert int(out.total["n_checked"]) == (1 << 45) - 1
 
 
@@ -103,3 +104,6 @@ def test_full_tree_n40_cuda_matches_reference_mode():
     out = sv.verify_range(1, 1 << 40, 36, cutoff_log2=100, form=1)
     assert int(out.total["hash"]) == ref.complete_hash
     assert int(out.total["count"]) == ref.candidates
+    # The same candidates run the same checks, counted per lane by the range kernels and
+    # per wave by reference mode.
+    assert out.checks == ref.checks
