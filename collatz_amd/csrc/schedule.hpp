@@ -100,12 +100,15 @@ __device__ __forceinline__ bool ge_words(const uint32_t* x, const uint32_t* y) {
   }
 }
 
-// number < n for n = n_lo + 2^64 n_hi.
+// number < n for n = n_lo + 2^64 n_hi.  Three words hold starts below 2^68 only, so there
+// n_hi < 2^4 and the top compare is one 32-bit compare.
 template <int L>
 __device__ __forceinline__ bool lt_base(const Num<L>& v, uint64_t n_lo, uint64_t n_hi) {
-  uint64_t vh;
-  if constexpr (L == 3) vh = v.w[2];
-  else vh = pair(v.w[2], v.w[3]);
+  if constexpr (L == 3) {
+    const uint32_t nh = (uint32_t)n_hi;
+    return (v.w[2] < nh) | ((v.w[2] == nh) & (pair(v.w[0], v.w[1]) < n_lo));
+  }
+  const uint64_t vh = pair(v.w[2], v.w[3]);
   bool lt = (vh < n_hi) | ((vh == n_hi) & (pair(v.w[0], v.w[1]) < n_lo));
   if constexpr (L > 4) lt = lt & ((v.w[4] | v.w[5]) == 0);
   return lt;

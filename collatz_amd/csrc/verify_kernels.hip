@@ -134,14 +134,14 @@ __device__ __forceinline__ bool over_check_bound(const Num<L>& v) {
 // the cutoff test (on four words LOWCUT means a cutoff below 2^123, where the top-pair
 // compare is exact; on six words a cutoff below 2^128).
 template <int L, bool LOWCUT, class Sink>
-__device__ __forceinline__ void after_odd(Cand<L>& c, Sink& sink, const Limits& lim,
-                                          const DeviceLists& lists) {
-  sink.offer(c.v, c.n_lo, c.n_hi);
+__device__ __forceinline__ void after_odd(Cand<L>& c, uint64_t n_hi, Sink& sink,
+                                          const Limits& lim, const DeviceLists& lists) {
+  sink.offer(c.v, c.n_lo, n_hi);
   if constexpr (!Escalates<L, LOWCUT>::value) {
     if (over_fast<L, LOWCUT>(c.v, lim.fast_thr) && (L == 4 || ge_pow2<L>(c.v, lim.cutoff))) {
       // Overflow: report the start and continue from 2 (collatz_cudav7.cu:295-301).
       c.ovf += 1;
-      push_overflow(lists, c.n_lo, c.n_hi, c.slot);
+      push_overflow(lists, c.n_lo, n_hi, c.slot);
       c.v.w[0] = 2;
 #pragma unroll
       for (int i = 1; i < L; ++i) c.v.w[i] = 0;
@@ -151,22 +151,24 @@ __device__ __forceinline__ void after_odd(Cand<L>& c, Sink& sink, const Limits& 
 
 // One check.  `done`: number < n after it.  `esc`: the check ended at or above the
 // width's bound, so the candidate goes to the three-limb pass (never together with done:
-// the widths that escalate are used only for starts far below the bound).
+// the widths that escalate are used only for starts far below the bound).  n_hi is the
+// candidate's c.n_hi, passed on its own so that a caller whose candidates share it (the
+// persistent kernel: one q per tile) can keep it in scalar registers.
 template <int L, bool LOWCUT, class Sink>
-__device__ __forceinline__ void one_check(Cand<L>& c, Sink& sink, const Limits& lim,
-                                          const DeviceLists& lists, const Pow3Lds& p3,
-                                          bool& esc, bool& done) {
+__device__ __forceinline__ void one_check(Cand<L>& c, uint64_t n_hi, Sink& sink,
+                                          const Limits& lim, const DeviceLists& lists,
+                                          const Pow3Lds& p3, bool& esc, bool& done) {
   const uint32_t a1 = odd_run<L>(c.v, p3);
-  after_odd<L, LOWCUT>(c, sink, lim, lists);
+  after_odd<L, LOWCUT>(c, n_hi, sink, lim, lists);
   const uint32_t b1 = even_run<L>(c.v);
   const uint32_t a2 = odd_run<L>(c.v, p3);
-  after_odd<L, LOWCUT>(c, sink, lim, lists);
+  after_odd<L, LOWCUT>(c, n_hi, sink, lim, lists);
   const uint32_t b2 = even_run<L>(c.v);
   c.steps += a1 + b1 + a2 + b2;
   c.hash ^= c.v.w[0];
   if constexpr (Escalates<L, LOWCUT>::value) esc = over_check_bound<L>(c.v);
   else esc = false;
-  done = lt_base<L>(c.v, c.n_lo, c.n_hi);
+  done = lt_base<L>(c.v, c.n_lo, n_hi);
 }
 
 template <int L>
@@ -186,8 +188,8 @@ __device__ __forceinline__ void finish(const Cand<L>& c, Acc<L>& acc, uint32_t k
 // decrease) and multiplies by at most (3/2)^8 < 2^4.7, so values stay below 2^128 only for
 // starts below 2^123; a larger start goes to the three-limb pass.
 template <int L, bool LOWCUT, class Sink>
-__device__ __forceinline__ bool cand_enter(const Cand<L>& c, Sink& sink) {
-  sink.offer(c.v, c.n_lo, c.n_hi);
+__device__ __forceinline__ bool cand_enter(const Cand<L>& c, uint64_t n_hi, Sink& sink) {
+  sink.offer(c.v, c.n_lo, n_hi);
   if constexpr (Escalates<L, LOWCUT>::value) return !over_check_bound<L>(c.v);
   if constexpr (L == 4) return c.v.w[3] < (1u << (kWideLog2 - 96));
   return true;
@@ -199,10 +201,10 @@ __device__ __forceinline__ bool run_candidate(Cand<L>& c, Acc<L>& acc, const Lim
                                               const DeviceLists& lists, uint32_t k,
                                               const Pow3Lds& p3) {
   SinkOne<L> sink{acc};
-  if (!cand_enter<L, LOWCUT>(c, sink)) return false;
+  if (!cand_enter<L, LOWCUT>(c, c.n_hi, sink)) return false;
   for (uint32_t guard = 0; guard < lim.max_checks; ++guard) {
     bool esc, done;
-    one_check<L, LOWCUT>(c, sink, lim, lists, p3, esc, done);
+    one_check<L, LOWCUT>(c, c.n_hi, sink, lim, lists, p3, esc, done);
     acc.chk += 1;
     if (esc) return false;
     if (done) {
@@ -399,8 +401,8 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
                                         rec_r(pf_rec));
         c.slot = slot;
         cidx = pf_off;
-        if (fits && cand_enter<L, LOWCUT>(c, sink)) act = true;
-        else push_escalate(lists, c.n_lo, c.n_hi, first + cidx, slot);
+        if (fits && cand_enter<L, LOWCUT>(c, qb_hi, sink)) act = true;
+        else push_escalate(lists, c.n_lo, qb_hi, first + cidx, slot);
       };
       // (1) Lanes without a candidate start their prefetched one.
       if (!act && pf) start_prefetched();
@@ -474,13 +476,13 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
         wave_chk += (uint32_t)__popcll(__ballot(act));
         if (act) {
           bool esc, done;
-          one_check<L, LOWCUT>(c, sink, lim, lists, p3s, esc, done);
+          one_check<L, LOWCUT>(c, qb_hi, sink, lim, lists, p3s, esc, done);
           if (__builtin_expect(esc, 0)) {
             // (The empty asm keeps the index sum in this rare branch: hoisted, it would
             // cost two instructions on every refill.)
             uint32_t off = cidx;
             asm volatile("" : "+v"(off));
-            push_escalate(lists, c.n_lo, c.n_hi, first + off, slot);
+            push_escalate(lists, c.n_lo, qb_hi, first + off, slot);
           } else if (done) {
             finish<L>(c, acc, sv.k);
           }
