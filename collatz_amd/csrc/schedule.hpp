@@ -191,6 +191,11 @@ __device__ __forceinline__ void offer_max(Acc<L>& acc, const Num<L>& v, uint64_t
     if (__builtin_expect(vh >= mh, 0)) {
       if ((vh > mh) | (v.w[0] >= acc.mx[0])) offer_take<L>(acc, v, n_lo, n_hi);
     }
+  } else if constexpr (L == 4) {
+    // The same filter on the top word pair, which holds the max's magnitude on four words.
+    if (__builtin_expect(pair(v.w[2], v.w[3]) >= pair(acc.mx[2], acc.mx[3]), 0)) {
+      if (ge_words<L>(v.w, acc.mx)) offer_take<L>(acc, v, n_lo, n_hi);
+    }
   } else {
     if (ge_words<L>(v.w, acc.mx)) offer_take<L>(acc, v, n_lo, n_hi);
   }

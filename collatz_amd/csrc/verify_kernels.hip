@@ -58,8 +58,91 @@ struct Cand {
 template <int L>
 struct SinkOne {
   Acc<L>& acc;
-  __device__ __forceinline__ void offer(const Num<L>& v, uint64_t n_lo, uint64_t n_hi) {
+  __device__ __forceinline__ void offer(const Num<L>& v, uint64_t n_lo, uint64_t n_hi, bool) {
     offer_max<L>(acc, v, n_lo, n_hi);
+  }
+};
+
+// Max of a whole wave, for the persistent kernel: the wave's best (value, start) is
+// uniform (scalar registers).  The offer test runs in uniform control flow on every lane
+// (lanes without a candidate pass `live` false): a lane's value is compared with the top 64
+// bits of the wave's best, and only when some lane reaches them (rare once the best is
+// high) are the reaching lanes merged one at a time by uniform code: a larger value wins,
+// an equal one with a smaller start.  A lane-private max would start from zero in every
+// tile and take a divergent update path many times per tile; the wave's max needs the
+// merge about log(offers) times.
+template <int L>
+__device__ __forceinline__ uint64_t top_key(const uint32_t* w) {
+  if constexpr (L == 3) return pair(w[1], w[2]);
+  else if constexpr (L == 4) return pair(w[2], w[3]);
+  else return pair(w[4], w[5]);
+}
+
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, uint32_t l) {
+  return pair(__builtin_amdgcn_readlane((uint32_t)x, l),
+              __builtin_amdgcn_readlane((uint32_t)(x >> 32), l));
+}
+
+// Adjacent 32-bit reads of a value's words would be merged into a straddling 64-bit read,
+// which keeps the array out of registers (the compiler then keeps it in LDS and stores it
+// after every run); an empty asm per word keeps each read on its own.
+template <int L>
+__device__ __forceinline__ void words_apart(uint32_t* x) {
+#pragma unroll
+  for (int i = 0; i < L; ++i) asm volatile("" : "+v"(x[i]));
+}
+
+template <int L>
+struct SinkWave {
+  uint32_t bw[L];         // the wave's best value (0: none yet; every value is >= 2)
+  uint64_t bs_lo, bs_hi;  // its start
+  __device__ __forceinline__ void clear() {
+#pragma unroll
+    for (int i = 0; i < L; ++i) bw[i] = 0;
+    bs_lo = bs_hi = 0;
+  }
+  uint64_t live;          // lanes with a candidate, set by the caller before each check
+  __device__ __forceinline__ void offer(Num<L>& v, uint64_t n_lo, uint64_t n_hi, bool) {
+    // Six words: the top pair is zero below 2^128, so the test there is the whole compare.
+    // Elsewhere the key's two words pass an empty asm in place first, so that the key is
+    // built from registers and not from a straddling 64-bit read of the value's words.
+    bool reach;
+    if constexpr (L == 6) {
+      reach = ge_words<L>(v.w, bw);
+    } else {
+      constexpr int k = L - 2;
+      uint32_t lo = v.w[k], hi = v.w[k + 1];
+      asm volatile("" : "+v"(lo), "+v"(hi));
+      v.w[k] = lo;
+      v.w[k + 1] = hi;
+      reach = pair(lo, hi) >= pair(bw[k], bw[k + 1]);
+    }
+    const uint64_t pend0 = __builtin_amdgcn_ballot_w64(reach) & live;
+    if (__builtin_expect(pend0 != 0, 0)) {
+      uint32_t x[L];
+#pragma unroll
+      for (int i = 0; i < L; ++i) x[i] = v.w[i];
+      words_apart<L>(x);
+      for (uint64_t pend = pend0; pend; pend &= pend - 1) {
+        const uint32_t l = (uint32_t)__builtin_ctzll(pend);
+        uint32_t cw[L];
+#pragma unroll
+        for (int i = 0; i < L; ++i) cw[i] = __builtin_amdgcn_readlane(x[i], l);
+        const uint64_t c_lo = readlane64(n_lo, l), c_hi = readlane64(n_hi, l);
+        int cmp = 0;
+#pragma unroll
+        for (int i = L - 1; i >= 0; --i)
+          if (cmp == 0 && cw[i] != bw[i]) cmp = cw[i] > bw[i] ? 1 : -1;
+        if (cmp > 0 || (cmp == 0 && (c_hi < bs_hi || (c_hi == bs_hi && c_lo < bs_lo)))) {
+#pragma unroll
+          for (int i = 0; i < L; ++i) bw[i] = cw[i];
+          bs_lo = c_lo;
+          bs_hi = c_hi;
+        }
+      }
+    }
   }
 };
 
@@ -135,10 +218,12 @@ __device__ __forceinline__ bool over_check_bound(const Num<L>& v) {
 // compare is exact; on six words a cutoff below 2^128).
 template <int L, bool LOWCUT, class Sink>
 __device__ __forceinline__ void after_odd(Cand<L>& c, uint64_t n_hi, Sink& sink,
-                                          const Limits& lim, const DeviceLists& lists) {
-  sink.offer(c.v, c.n_lo, n_hi);
+                                          const Limits& lim, const DeviceLists& lists,
+                                          bool live) {
+  sink.offer(c.v, c.n_lo, n_hi, live);
   if constexpr (!Escalates<L, LOWCUT>::value) {
-    if (over_fast<L, LOWCUT>(c.v, lim.fast_thr) && (L == 4 || ge_pow2<L>(c.v, lim.cutoff))) {
+    if (live && over_fast<L, LOWCUT>(c.v, lim.fast_thr) &&
+        (L == 4 || ge_pow2<L>(c.v, lim.cutoff))) {
       // Overflow: report the start and continue from 2 (collatz_cudav7.cu:295-301).
       c.ovf += 1;
       push_overflow(lists, c.n_lo, n_hi, c.slot);
@@ -153,22 +238,25 @@ __device__ __forceinline__ void after_odd(Cand<L>& c, uint64_t n_hi, Sink& sink,
 // width's bound, so the candidate goes to the three-limb pass (never together with done:
 // the widths that escalate are used only for starts far below the bound).  n_hi is the
 // candidate's c.n_hi, passed on its own so that a caller whose candidates share it (the
-// persistent kernel: one q per tile) can keep it in scalar registers.
+// persistent kernel: one q per tile) can keep it in scalar registers.  A lane with `live`
+// false (no candidate) computes on whatever its state holds; its offers, overflow reports,
+// esc and done are suppressed.
 template <int L, bool LOWCUT, class Sink>
 __device__ __forceinline__ void one_check(Cand<L>& c, uint64_t n_hi, Sink& sink,
                                           const Limits& lim, const DeviceLists& lists,
-                                          const Pow3Lds& p3, bool& esc, bool& done) {
+                                          const Pow3Lds& p3, bool& esc, bool& done,
+                                          bool live = true) {
   const uint32_t a1 = odd_run<L>(c.v, p3);
-  after_odd<L, LOWCUT>(c, n_hi, sink, lim, lists);
+  after_odd<L, LOWCUT>(c, n_hi, sink, lim, lists, live);
   const uint32_t b1 = even_run<L>(c.v);
   const uint32_t a2 = odd_run<L>(c.v, p3);
-  after_odd<L, LOWCUT>(c, n_hi, sink, lim, lists);
+  after_odd<L, LOWCUT>(c, n_hi, sink, lim, lists, live);
   const uint32_t b2 = even_run<L>(c.v);
   c.steps += a1 + b1 + a2 + b2;
   c.hash ^= c.v.w[0];
-  if constexpr (Escalates<L, LOWCUT>::value) esc = over_check_bound<L>(c.v);
+  if constexpr (Escalates<L, LOWCUT>::value) esc = live && over_check_bound<L>(c.v);
   else esc = false;
-  done = lt_base<L>(c.v, c.n_lo, n_hi);
+  done = live && lt_base<L>(c.v, c.n_lo, n_hi);
 }
 
 template <int L>
@@ -181,15 +269,16 @@ __device__ __forceinline__ void finish(const Cand<L>& c, Acc<L>& acc, uint32_t k
   acc.ovf += c.ovf;
 }
 
-// Start-of-candidate test: the start value is offered to the max, and a start at or above
-// the width's per-check bound goes to the three-limb pass.  Four words with a cutoff below
-// 2^123 do not escalate per check: every odd run starts at or below max(start, 2^cutoff)
-// (an odd run that ends at or above the cutoff resets the value to 2; even runs only
-// decrease) and multiplies by at most (3/2)^8 < 2^4.7, so values stay below 2^128 only for
-// starts below 2^123; a larger start goes to the three-limb pass.
-template <int L, bool LOWCUT, class Sink>
-__device__ __forceinline__ bool cand_enter(const Cand<L>& c, uint64_t n_hi, Sink& sink) {
-  sink.offer(c.v, c.n_lo, n_hi);
+// Start-of-candidate test: a start at or above the width's per-check bound goes to the
+// three-limb pass.  Four words with a cutoff below 2^123 do not escalate per check: every
+// odd run starts at or below max(start, 2^cutoff) (an odd run that ends at or above the
+// cutoff resets the value to 2; even runs only decrease) and multiplies by at most
+// (3/2)^8 < 2^4.7, so values stay below 2^128 only for starts below 2^123; a larger start
+// goes to the three-limb pass.  The start value itself is not offered to the max: the
+// first odd run turns 2^a m into 3^a m, at least as large, for the same start, and every
+// candidate that is not escalated runs at least one check.
+template <int L, bool LOWCUT>
+__device__ __forceinline__ bool cand_enter(const Cand<L>& c) {
   if constexpr (Escalates<L, LOWCUT>::value) return !over_check_bound<L>(c.v);
   if constexpr (L == 4) return c.v.w[3] < (1u << (kWideLog2 - 96));
   return true;
@@ -201,7 +290,7 @@ __device__ __forceinline__ bool run_candidate(Cand<L>& c, Acc<L>& acc, const Lim
                                               const DeviceLists& lists, uint32_t k,
                                               const Pow3Lds& p3) {
   SinkOne<L> sink{acc};
-  if (!cand_enter<L, LOWCUT>(c, c.n_hi, sink)) return false;
+  if (!cand_enter<L, LOWCUT>(c)) return false;
   for (uint32_t guard = 0; guard < lim.max_checks; ++guard) {
     bool esc, done;
     one_check<L, LOWCUT>(c, c.n_hi, sink, lim, lists, p3, esc, done);
@@ -370,9 +459,11 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
     const uint64_t qb_lo = rw.qb_lo, qb_hi = rw.qb_hi;
     const uint32_t slot = rw.slot;
 
+    // Per-lane counts and sums; the max is the wave's (SinkWave), cleared for each tile.
     Acc<L> acc;
     acc.init();
-    SinkOne<L> sink{acc};
+    SinkWave<L> sink;
+    sink.clear();
     // Checks of the tile, counted per wave from the mask of lanes that run one (scalar
     // work; the lane accumulators' chk stays 0).
     uint32_t wave_chk = 0;
@@ -401,7 +492,7 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
                                         rec_r(pf_rec));
         c.slot = slot;
         cidx = pf_off;
-        if (fits && cand_enter<L, LOWCUT>(c, qb_hi, sink)) act = true;
+        if (fits && cand_enter<L, LOWCUT>(c)) act = true;
         else push_escalate(lists, c.n_lo, qb_hi, first + cidx, slot);
       };
       // (1) Lanes without a candidate start their prefetched one.
@@ -472,28 +563,40 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
       }
       // (3) U checks; a lane whose candidate ends before the last of them starts its
       //     prefetched candidate at once instead of idling.
+      //     Every lane runs the check (the SIMD issues it for all lanes either way); a lane
+      //     without a candidate computes on stale values and its results are dropped, so
+      //     the check's offers run in uniform control flow.
       for (uint32_t u = 0; u < U; ++u) {
-        wave_chk += (uint32_t)__popcll(__ballot(act));
-        if (act) {
-          bool esc, done;
-          one_check<L, LOWCUT>(c, qb_hi, sink, lim, lists, p3s, esc, done);
-          if (__builtin_expect(esc, 0)) {
-            // (The empty asm keeps the index sum in this rare branch: hoisted, it would
-            // cost two instructions on every refill.)
-            uint32_t off = cidx;
-            asm volatile("" : "+v"(off));
-            push_escalate(lists, c.n_lo, qb_hi, first + off, slot);
-          } else if (done) {
-            finish<L>(c, acc, sv.k);
-          }
-          act = !(esc || done);
+        sink.live = __builtin_amdgcn_ballot_w64(act);
+        wave_chk += (uint32_t)__popcll(sink.live);
+        bool esc, done;
+        one_check<L, LOWCUT>(c, qb_hi, sink, lim, lists, p3s, esc, done, act);
+        if (__builtin_expect(esc, 0)) {
+          // (The empty asm keeps the index sum in this rare branch: hoisted, it would cost
+          // two instructions on every refill.)
+          uint32_t off = cidx;
+          asm volatile("" : "+v"(off));
+          push_escalate(lists, c.n_lo, qb_hi, first + off, slot);
+        } else if (done) {
+          finish<L>(c, acc, sv.k);
         }
+        act = act && !(esc || done);
         if (u + 1 < U && !act && pf) start_prefetched();
       }
     }
     DevRec r = acc_to_rec<L>(acc);
     wave_merge(r);
     r.checks = wave_chk;
+    if (r.count) {
+      // A tile without candidates keeps max 0 (any it offered belonged to escalated
+      // candidates, whose three-limb rerun offers them again).
+      r.maxnum[0] = pair(sink.bw[0], sink.bw[1]);
+      if constexpr (L == 3) r.maxnum[1] = sink.bw[2];
+      else r.maxnum[1] = pair(sink.bw[2], sink.bw[3]);
+      if constexpr (L > 4) r.maxnum[2] = pair(sink.bw[4], sink.bw[5]);
+      r.max_start[0] = sink.bs_lo;
+      r.max_start[1] = sink.bs_hi;
+    }
     if (lane == 0) tile_recs[t] = r;
   }
 }
