@@ -50,9 +50,12 @@ uint32_t adaptive_tile(const collatz_sieve* sv, uint64_t candidates, uint32_t wa
   return t;
 }
 
-// Tile of the tuner's timing launches: a sample of 2^27 candidates in tiles of 4096 gives
-// every wave several tiles, as the large launches the tuning is for do.
+// The tuner's timing launches: 2^29 candidates in tiles of 4096, about 16 tiles per
+// resident wave, as the large launches the tuning is for have.  (With 2^27 candidates,
+// four tiles per wave, the end of the launch weighed so much that 28 waves per CU won
+// over 32, which runs the 4096-interval batch 4% faster.)
 static constexpr uint32_t kTuneTile = 4096;
+static constexpr uint32_t kTuneLog2 = 29;
 
 Tuning choose_tuning(const collatz_sieve* sv, int limbs, u128 max_hi, uint64_t candidates,
                      uint32_t tile, uint32_t waves, uint32_t U) {
@@ -67,12 +70,12 @@ Tuning choose_tuning(const collatz_sieve* sv, int limbs, u128 max_hi, uint64_t c
     if (it != g_cache.end()) {
       t = it->second;
     } else {
-      // A range of the bucket's magnitude with about 2^27 candidates (three times as many
+      // A range of the bucket's magnitude with about 2^29 candidates (three times as many
       // survivors for a look-ahead sieve, which keeps about a third), each shape timed three
       // times (the fastest counts).
       const uint32_t bits = std::max<uint32_t>(bitlen(max_hi), sv->k + 1);
       const double density = (double)sv->count / (double)((u128)1 << sv->k) * 5.0 / 9.0;
-      double w = (double)(1u << 27) / (density > 0 ? density : 1e-3);
+      double w = (double)(1u << kTuneLog2) / (density > 0 ? density : 1e-3);
       if (sv->la_bits) w *= 3;
       u128 width = (u128)w;
       if (width < ((u128)1 << 20)) width = (u128)1 << 20;
