@@ -54,7 +54,8 @@ struct Cand {
   uint32_t slot;   // output record slot (for overflow and escalation reports)
 };
 
-// Where a candidate's values are offered for the max: one lane accumulator.
+// Where a candidate's values are offered for the max: one lane accumulator.  (The last
+// argument, whether the lane holds a candidate, is always true where this sink is used.)
 template <int L>
 struct SinkOne {
   Acc<L>& acc;
@@ -62,23 +63,6 @@ struct SinkOne {
     offer_max<L>(acc, v, n_lo, n_hi);
   }
 };
-
-// Max of a whole wave, for the persistent kernel: the wave's best (value, start) is
-// uniform (scalar registers).  The offer test runs in uniform control flow on every lane
-// (lanes without a candidate pass `live` false): a lane's value is compared with the top 64
-// bits of the wave's best, and only when some lane reaches them (rare once the best is
-// high) are the reaching lanes merged one at a time by uniform code: a larger value wins,
-// an equal one with a smaller start.  A lane-private max would start from zero in every
-// tile and take a divergent update path many times per tile; the wave's max needs the
-// merge about log(offers) times.
-template <int L>
-__device__ __forceinline__ uint64_t top_key(const uint32_t* w) {
-  if constexpr (L == 3) return pair(w[1], w[2]);
-  else if constexpr (L == 4) return pair(w[2], w[3]);
-  else return pair(w[4], w[5]);
-}
-
-__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
 __device__ __forceinline__ uint64_t readlane64(uint64_t x, uint32_t l) {
   return pair(__builtin_amdgcn_readlane((uint32_t)x, l),
@@ -94,16 +78,24 @@ __device__ __forceinline__ void words_apart(uint32_t* x) {
   for (int i = 0; i < L; ++i) asm volatile("" : "+v"(x[i]));
 }
 
+// Max of a whole wave, for the persistent kernel: the wave's best (value, start) is
+// uniform (scalar registers).  The offer runs in uniform control flow on every lane: a
+// lane's value is compared with the top 64 bits of the wave's best, and only when some lane
+// of `live` (the lanes that hold a candidate, set by the caller before each check) reaches
+// them, rare once the best is high, are the reaching lanes merged one at a time by uniform
+// code: a larger value wins, an equal one with a smaller start.  A lane-private max would
+// start from zero in every tile and take a divergent update path many times per tile; the
+// wave's max needs the merge about log(offers) times.
 template <int L>
 struct SinkWave {
   uint32_t bw[L];         // the wave's best value (0: none yet; every value is >= 2)
   uint64_t bs_lo, bs_hi;  // its start
+  uint64_t live;          // lanes with a candidate
   __device__ __forceinline__ void clear() {
 #pragma unroll
     for (int i = 0; i < L; ++i) bw[i] = 0;
     bs_lo = bs_hi = 0;
   }
-  uint64_t live;          // lanes with a candidate, set by the caller before each check
   __device__ __forceinline__ void offer(Num<L>& v, uint64_t n_lo, uint64_t n_hi, bool) {
     // Six words: the top pair is zero below 2^128, so the test there is the whole compare.
     // Elsewhere the key's two words pass an empty asm in place first, so that the key is
@@ -145,9 +137,6 @@ struct SinkWave {
     }
   }
 };
-
-// Outcome of running a candidate to its end.
-enum : uint32_t { kDone = 0, kEscalate = 1 };
 
 // Set up a candidate: v = M + q3 (q3 has L words; q3_big says q * 3^f needs more), n =
 // qb | r.  Returns false if the start value does not fit L words (then the candidate must
