@@ -273,6 +273,51 @@ __device__ __forceinline__ bool cand_enter(const Cand<L>& c) {
   return true;
 }
 
+// Lane masks for the persistent kernel, each built from ballots of single compares (a
+// ballot of one compare is that compare's mask; a ballot of a combined condition would be
+// materialised in a vector register first).
+__device__ __forceinline__ uint64_t ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
+__device__ __forceinline__ bool in_mask(uint64_t m) {
+  return __builtin_amdgcn_inverse_ballot_w64(m);
+}
+
+// Lanes whose new start may run (cand_enter), where a start that did not fit its words has
+// had its top word set to all ones (a value no start of these widths reaches).
+template <int L, bool LOWCUT>
+__device__ __forceinline__ uint64_t start_ok_mask(const Num<L>& v) {
+  if constexpr (Escalates<L, LOWCUT>::value) {
+    if constexpr (L == 3) return ballot(v.w[2] < (1u << (kNarrowCheckLog2 - 64)));
+    else return ballot(pair(v.w[2], v.w[3]) < (1ull << (kWideCheckLog2 - 64)));
+  } else if constexpr (L == 4) {
+    return ballot(v.w[3] < (1u << (kWideLog2 - 96)));
+  } else {
+    return ballot(v.w[L - 1] != 0xFFFFFFFFu);
+  }
+}
+
+// Lanes whose check ended at or above the width's bound (over_check_bound).
+template <int L, bool LOWCUT>
+__device__ __forceinline__ uint64_t esc_mask(const Num<L>& v) {
+  if constexpr (!Escalates<L, LOWCUT>::value) return 0;
+  else if constexpr (L == 3) return ballot(v.w[2] >= (1u << (kNarrowCheckLog2 - 64)));
+  else return ballot(pair(v.w[2], v.w[3]) >= (1ull << (kWideCheckLog2 - 64)));
+}
+
+// Lanes with number < n (lt_base).
+template <int L>
+__device__ __forceinline__ uint64_t lt_base_mask(const Num<L>& v, uint64_t n_lo, uint64_t n_hi) {
+  const uint64_t lo = ballot(pair(v.w[0], v.w[1]) < n_lo);
+  if constexpr (L == 3) {
+    const uint32_t nh = (uint32_t)n_hi;
+    return ballot(v.w[2] < nh) | (ballot(v.w[2] == nh) & lo);
+  } else {
+    const uint64_t vh = pair(v.w[2], v.w[3]);
+    const uint64_t m = ballot(vh < n_hi) | (ballot(vh == n_hi) & lo);
+    if constexpr (L > 4) return m & ballot((v.w[4] | v.w[5]) == 0u);
+    else return m;
+  }
+}
+
 // Runs candidate c to completion.  Returns false if it was escalated.
 template <int L, bool LOWCUT>
 __device__ __forceinline__ bool run_candidate(Cand<L>& c, Acc<L>& acc, const Limits& lim,
@@ -458,10 +503,10 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
     uint32_t wave_chk = 0;
     Cand<L> c;
     c.n_hi = qb_hi;  // the same for every candidate of the tile
-    bool act = false;
+    // Lane masks (uniform): lanes with a running candidate, lanes holding a prefetched
+    // record.  Per-lane conditions are read from them with in_mask().
+    uint64_t actm = 0, pfm = 0;
     uint32_t cidx = 0;  // offset of the running candidate in the tile
-    // Prefetch slot.
-    bool pf = false;
     uint4 pf_rec = make_uint4(0, 0, 0, 0);
     uint32_t pf_off = 0;
     const uint4* tile_rec = sv.rec + first;
@@ -473,39 +518,50 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
       if (lane < n) nx = tile_rec[lane];
     }
     for (;;) {
-      // Starts the prefetched candidate of a lane without one.
-      auto start_prefetched = [&]() {
-        pf = false;
-        const uint32_t f = rec_f(pf_rec);
-        const bool fits = cand_start<L>(c, rec_M(pf_rec), T.q3[f], false, qb_lo, qb_hi,
-                                        rec_r(pf_rec));
-        c.slot = slot;
-        cidx = pf_off;
-        if (fits && cand_enter<L, LOWCUT>(c)) act = true;
-        else push_escalate(lists, c.n_lo, qb_hi, first + cidx, slot);
+      // Starts the prefetched candidates of the lanes in S (no running candidate, a
+      // prefetched record); a start that cannot run on these words is escalated.
+      auto start_prefetched = [&](uint64_t S) {
+        if (in_mask(S)) {
+          const uint32_t f = rec_f(pf_rec);
+          const bool fits = cand_start<L>(c, rec_M(pf_rec), T.q3[f], false, qb_lo, qb_hi,
+                                          rec_r(pf_rec));
+          c.slot = slot;  // (overflow reports carry it)
+          cidx = pf_off;
+          if (!fits) c.v.w[L - 1] = 0xFFFFFFFFu;
+        }
+        const uint64_t ok = start_ok_mask<L, LOWCUT>(c.v) & S;
+        const uint64_t bad = S & ~ok;
+        if (__builtin_expect(bad != 0, 0)) {
+          if (in_mask(bad)) push_escalate(lists, c.n_lo, qb_hi, first + cidx, slot);
+        }
+        actm |= ok;
+        pfm &= ~S;
       };
       // (1) Lanes without a candidate start their prefetched one.
-      if (!act && pf) start_prefetched();
+      {
+        const uint64_t S = ~actm & pfm;
+        if (S) start_prefetched(S);
+      }
       // (2) Lanes without a prefetch take the next survivors of the tile.
-      const uint64_t need = __ballot(!pf);
+      const uint64_t need = ~pfm;
       if constexpr (LA) {
         // ... the next kept ones, from the stage, refilled window by window.
         if (need) {
           LaStage& S = stages[wv];
           uint32_t want = __popcll(need);
           uint32_t rank = lane_rank(need);
+          uint64_t open = need;
           for (;;) {
             const uint32_t avail = scnt - shead;
             const uint32_t take = avail < want ? avail : want;
-            if (!pf) {
-              if (rank < take) {
-                pf_rec = S.rec[shead + rank];
-                pf_off = S.off[shead + rank];
-                pf = true;
-              } else {
-                rank -= take;
-              }
+            const uint64_t tk = ballot(rank < take) & open;
+            if (in_mask(tk)) {
+              pf_rec = S.rec[shead + rank];
+              pf_off = S.off[shead + rank];
             }
+            pfm |= tk;
+            open &= ~tk;
+            rank -= take;  // (lanes that took, or need nothing, no longer look at it)
             shead += take;
             want -= take;
             if (want == 0 || cur >= n) break;
@@ -536,19 +592,21 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
         const uint32_t avail = n - cur;
         const uint32_t take = avail < want ? avail : want;
         const uint32_t rank = lane_rank(need);
-        if (!pf && rank < take) {
+        const uint64_t tk = ballot(rank < take) & need;
+        if (in_mask(tk)) {
           pf_off = cur + rank;
           pf_rec = tile_rec[pf_off];
-          pf = true;
         }
+        pfm |= tk;
         cur += take;
       }
-      if (__ballot(act || pf) == 0) break;
+      if ((actm | pfm) == 0) break;
       // A candidate past the step bound is abandoned (tested once per refill: it may run up
       // to U checks beyond the bound first).
-      if (act && __builtin_expect(c.steps >= lim.max_steps, 0)) {
-        atomicAdd(lists.stall_count, 1ull);
-        act = false;
+      const uint64_t stalled = ballot(c.steps >= lim.max_steps) & actm;
+      if (__builtin_expect(stalled != 0, 0)) {
+        if (in_mask(stalled)) atomicAdd(lists.stall_count, 1ull);
+        actm &= ~stalled;
       }
       // (3) U checks; a lane whose candidate ends before the last of them starts its
       //     prefetched candidate at once instead of idling.
@@ -556,21 +614,30 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
       //     without a candidate computes on stale values and its results are dropped, so
       //     the check's offers run in uniform control flow.
       for (uint32_t u = 0; u < U; ++u) {
-        sink.live = __builtin_amdgcn_ballot_w64(act);
-        wave_chk += (uint32_t)__popcll(sink.live);
-        bool esc, done;
-        one_check<L, LOWCUT>(c, qb_hi, sink, lim, lists, p3s, esc, done, act);
-        if (__builtin_expect(esc, 0)) {
-          // (The empty asm keeps the index sum in this rare branch: hoisted, it would cost
-          // two instructions on every refill.)
-          uint32_t off = cidx;
-          asm volatile("" : "+v"(off));
-          push_escalate(lists, c.n_lo, qb_hi, first + off, slot);
-        } else if (done) {
-          finish<L>(c, acc, sv.k);
+        sink.live = actm;
+        wave_chk += (uint32_t)__popcll(actm);
+        bool esc_lane, done_lane;  // (unused here: the masks below are taken instead)
+        one_check<L, LOWCUT>(c, qb_hi, sink, lim, lists, p3s, esc_lane, done_lane,
+                             in_mask(actm));
+        const uint64_t escm = esc_mask<L, LOWCUT>(c.v) & actm;
+        const uint64_t donem = lt_base_mask<L>(c.v, c.n_lo, qb_hi) & actm & ~escm;
+        if (__builtin_expect(escm != 0, 0)) {
+          if (in_mask(escm)) {
+            // (The empty asm keeps the index sum in this rare branch: hoisted, it would
+            // cost two instructions on every refill.)
+            uint32_t off = cidx;
+            asm volatile("" : "+v"(off));
+            push_escalate(lists, c.n_lo, qb_hi, first + off, slot);
+          }
         }
-        act = act && !(esc || done);
-        if (u + 1 < U && !act && pf) start_prefetched();
+        if (donem) {
+          if (in_mask(donem)) finish<L>(c, acc, sv.k);
+        }
+        actm &= ~(escm | donem);
+        if (u + 1 < U) {
+          const uint64_t S = ~actm & pfm;
+          if (S) start_prefetched(S);
+        }
       }
     }
     DevRec r = acc_to_rec<L>(acc);
