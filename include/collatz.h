@@ -126,7 +126,11 @@ typedef struct {
 } collatz_verify_opts;
 
 /* Replaces launchkernel() + compute_kernel (collatz_cudav7.cu:565-649, :185-411):
- * [lo, hi) with 128-bit endpoints lo = lo[0] + 2^64 lo[1]. */
+ * [lo, hi) with 128-bit endpoints lo = lo[0] + 2^64 lo[1].  The call returns when its
+ * results are in `out`.  Calls on one sieve handle from several threads are safe: the
+ * plain form keeps its device buffers on the handle (grown on demand, reused by later
+ * calls) and runs one call at a time; the persistent form and collatz_verify_batch share
+ * the handle's launch engine the same way. */
 int collatz_verify_range(const collatz_sieve* sieve, const uint64_t lo[2], const uint64_t hi[2],
                          const collatz_verify_opts* opts, void* stream,
                          collatz_range_result* out);
