@@ -61,6 +61,7 @@ def words_for(max_hi):
 # gfx950 int32 VALU issue peak: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T lane-ops/s.
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 RATES = (50, 100, 200, 500)
+FULL_RATES = (200, 500)
 # HBM traffic of the batched launch, measured with rocprofv3 on this workload
 # (profiles/k_verify_persistent_batch4096_k32.md): FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0,
 # the factors a 1 GiB copy kernel gives.  Reads are the 16-byte survivor record of each
@@ -144,8 +145,10 @@ def cpu_baseline(intervals, threads: int, n_intervals: int, width_log2: int):
             "cpu_model": model, "host_cpus": os.cpu_count()}
 
 
-def served_leg(sv, reqs, rate, chunk_log2):
-    """Replays `reqs` (gap, lo, hi) at `rate` requests/s (rate None: all at once)."""
+def served_leg(sv, reqs, rate, chunk_log2, check_n=0):
+    """Replays `reqs` (gap, lo, hi) at `rate` requests/s (rate None: all at once).  The
+    results of the first `check_n` requests are compared with the plain kernel's records of
+    the same ranges afterwards (outside the timed replay)."""
     import collatz_amd as cz
     srv = cz.Server(sv, chunk_log2=chunk_log2)
     tickets = []
@@ -164,27 +167,41 @@ def served_leg(sv, reqs, rate, chunk_log2):
     th = threading.Thread(target=producer)
     th.start()
     lat, n, first_submit, last_done = [], 0, None, 0.0
+    kept = []
     i = 0
+    t_log = time.perf_counter()
     while True:
         if i < len(tickets):
             t, w = tickets[i]
-            _, _, (ts, ta, td) = srv.wait(t)
+            total, chunks, (ts, ta, td) = srv.wait(t)
+            if i < check_n:
+                kept.append((total, chunks))
             lat.append(td - ts)
             n += w
             first_submit = ts if first_submit is None else min(first_submit, ts)
             last_done = max(last_done, td)
             i += 1
+            if time.perf_counter() - t_log > 10:
+                t_log = time.perf_counter()
+                log(f"served at {rate or 'saturation'}: {i}/{len(reqs)} requests done")
         elif not th.is_alive() and i >= len(tickets):
             break
         else:
             time.sleep(0.0005)
     th.join()
     srv.close()
+    same = True
+    for (total, chunks), (_, lo, hi) in zip(kept, reqs):
+        p = sv.verify_range(lo, hi, chunk_log2, form=cz.FORM_PLAIN)
+        if not (records_equal(chunks, p.chunks) and total.tobytes() == p.total.tobytes()):
+            same = False
     lat.sort()
     pct = lambda p: lat[min(len(lat) - 1, int(p * len(lat)))]
     span_s = (last_done - first_submit) / 1e3
     return {"requests": len(lat), "p50_ms": pct(0.50), "p99_ms": pct(0.99),
-            "max_ms": lat[-1], "n_per_s": n / span_s, "offered_req_per_s": rate}
+            "max_ms": lat[-1], "n_per_s": n / span_s, "integers": n,
+            "offered_req_per_s": rate, "checked_vs_plain": len(kept),
+            "bit_exact_vs_plain": same}
 
 
 def lookahead_leg(cz, ivs, total_n, steps):
@@ -242,13 +259,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--plain-passes", type=int, default=1)
     ap.add_argument("--range-steps", type=int, default=20)
-    ap.add_argument("--serve-seconds", type=float, default=5.0,
-                    help="length of each served replay at a fixed rate")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="length of the served replays at the prefix rates")
     ap.add_argument("--serve-rates", default=",".join(str(r) for r in RATES),
                     help="comma-separated request rates (req/s) of the served replays")
+    ap.add_argument("--serve-full-rates", default=",".join(str(r) for r in FULL_RATES),
+                    help="rates at which the whole 20000-request stream is replayed; the "
+                         "others replay the first serve-seconds x rate requests")
+    ap.add_argument("--serve-check", type=int, default=500,
+                    help="served results compared with the plain kernel, per replay")
     ap.add_argument("--serve-all", action="store_true",
-                    help="replay the whole 20000-request stream at each rate, and all of it "
-                         "at once for the saturated figure")
+                    help="replay the whole 20000-request stream at every rate")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-range", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -450,16 +471,24 @@ def main():
 
     if not args.no_serve:
         reqs = W.request_stream(1, 20000)
+        full = {int(x) for x in args.serve_full_rates.split(",") if x}
+        # Untimed: the autotuner's first sweeps for the magnitudes the stream reaches.
+        served_leg(sv, reqs[:300], None, BATCH_CHUNK)
         served = {}
         for rate in (int(x) for x in args.serve_rates.split(",") if x):
-            n = len(reqs) if args.serve_all else \
+            n = len(reqs) if (args.serve_all or rate in full) else \
                 min(len(reqs), max(50, int(rate * args.serve_seconds)))
-            served[str(rate)] = served_leg(sv, reqs[:n], rate, BATCH_CHUNK)
+            served[str(rate)] = served_leg(sv, reqs[:n], rate, BATCH_CHUNK, args.serve_check)
             log(f"served at {rate} req/s: {served[str(rate)]}")
-        served["saturated"] = served_leg(sv, reqs if args.serve_all else reqs[:2000], None,
-                                         BATCH_CHUNK)
+        served["saturated"] = served_leg(sv, reqs, None, BATCH_CHUNK, args.serve_check)
         log(f"served saturated: {served['saturated']}")
+        served["stream"] = ("SplitMix64 seed 1, 20000 requests: widths log-U[2^16,2^36], 80% of "
+                            "starts U[2^40,2^62), 20% U[2^62,2^64); k=32 sieve resident, "
+                            "2^32-wide records; each rate replays the first `requests` of the "
+                            "stream, saturated submits all of them at once")
         line["served"] = served
+        line["bit_exact_served_vs_plain"] = all(
+            v["bit_exact_vs_plain"] for v in served.values() if isinstance(v, dict))
 
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

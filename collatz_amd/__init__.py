@@ -188,6 +188,7 @@ class RangeOutput:
     escalated: int
     device_ms: float
     checks: int = 0
+    overflow_written: int = 0  # overflow reports made (may exceed the starts listed)
 
 
 class Sieve:
@@ -258,7 +259,8 @@ class Sieve:
         total = np.frombuffer(bytearray(res.total), CHUNK_DTYPE)[0]
         nov = min(res.overflow_written, overflow_cap)
         starts = [int(ovf[2 * i]) | (int(ovf[2 * i + 1]) << 64) for i in range(nov)]
-        return RangeOutput(total, chunks, starts, res.escalated, res.device_ms, res.checks)
+        return RangeOutput(total, chunks, starts, res.escalated, res.device_ms, res.checks,
+                           res.overflow_written)
 
     def verify_batch(self, intervals, chunk_log2: int = 24, limbs: int = 2,
                      cutoff_log2: int = 187, use_graph: bool = False, tile_residues: int = 0,
@@ -299,8 +301,9 @@ class Sieve:
         else:
             starts = [[int(ovf[i, 2 * j]) | (int(ovf[i, 2 * j + 1]) << 64)
                        for j in range(min(ow[i], overflow_cap))] for i in range(m)]
-        return [RangeOutput(t, chunks[a:b], st, e, d, c)
-                for t, a, b, st, e, d, c in zip(totals, o[:-1], o[1:], starts, esc, dms, chk)]
+        return [RangeOutput(t, chunks[a:b], st, e, d, c, w)
+                for t, a, b, st, e, d, c, w in zip(totals, o[:-1], o[1:], starts, esc, dms, chk,
+                                                   ow)]
 
 
 class Server:

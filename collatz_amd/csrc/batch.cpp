@@ -7,6 +7,7 @@
 // (counter reset, row resolution, persistent kernel, per-chunk reduction) can be captured
 // in a hipGraph and is replayed for later batches.
 #include <algorithm>
+#include <optional>
 #include <vector>
 
 #include "engine.hpp"
@@ -40,7 +41,7 @@ int verify_batch_impl(const collatz_sieve* sv, uint64_t n, const collatz_interva
   const int width = a.width ? (int)a.width : launch_width((int)a.limbs, max_hi, a.cutoff);
   const Tuning tn = choose_tuning(sv, width, max_hi, est, a.tile, a.waves, a.U);
   Span sp_all("batch total");
-  Span* sp_plan = new Span("batch plan");
+  std::optional<Span> sp_plan(std::in_place, "batch plan");
   LaunchPlan lp;
   std::vector<uint64_t> nchk, first_chunk(n), slot_base(n + 1, 0);
   for (uint64_t i = 0; i < n; ++i) {
@@ -52,10 +53,14 @@ int verify_batch_impl(const collatz_sieve* sv, uint64_t n, const collatz_interva
   }
   lp.part.push_back(lp.ntiles);
   if (lp.part.size() != lp.nslots + 1) return fail(COLLATZ_EDEVICE, "internal: slot without rows");
-  uint64_t ovf_cap = 1024;
-  for (uint64_t i = 0; i < n; ++i) ovf_cap = std::max<uint64_t>(ovf_cap, outs[i].overflow_cap);
+  // The launch's overflow list holds the starts of every interval that asks for them: room
+  // for all their buffers together (up to 2^24 reports), and at least 1024.
+  uint64_t ovf_cap = 0;
+  for (uint64_t i = 0; i < n; ++i)
+    if (outs[i].overflow_starts) ovf_cap = std::min<uint64_t>(ovf_cap + outs[i].overflow_cap, 1ull << 24);
+  ovf_cap = std::max<uint64_t>(ovf_cap, 1024);
 
-  delete sp_plan;
+  sp_plan.reset();
   LaunchResult res;
   int rc;
   {
@@ -72,10 +77,20 @@ int verify_batch_impl(const collatz_sieve* sv, uint64_t n, const collatz_interva
     return verify_batch_impl(sv, n, iv, b, s, outs, true);
   }
   if (rc) return rc;
+  if (res.stalled) {
+    // A trajectory that does not descend is a finding, not a result: the call fails and
+    // names the first interval and chunk where it happened.
+    const uint64_t sl = res.stall_slots.empty() ? 0 : res.stall_slots[0];
+    const uint64_t i =
+        std::upper_bound(slot_base.begin(), slot_base.end(), sl) - slot_base.begin() - 1;
+    return fail(COLLATZ_EDEVICE,
+                "a trajectory did not descend within the check bound (interval " +
+                    std::to_string(i) + ", chunk " +
+                    std::to_string(first_chunk[std::min<uint64_t>(i, n - 1)] + sl -
+                                   slot_base[std::min<uint64_t>(i, n - 1)]) + ")");
+  }
 
   bool short_buf = false;
-  for (uint64_t i = 0; i < n; ++i)
-    if (outs[i].overflow_starts && res.ovf_total > ovf_cap) short_buf = true;
   for (uint64_t i = 0; i < n; ++i) {
     collatz_range_result* out = &outs[i];
     const uint64_t nc = slot_base[i + 1] - slot_base[i];
@@ -93,16 +108,23 @@ int verify_batch_impl(const collatz_sieve* sv, uint64_t n, const collatz_interva
     out->chunks_written = nc;
     out->escalated = 0;
     out->device_ms = res.device_ms;
-    out->overflow_written = 0;
+    // The exact number of overflow reports is the interval's overflow count (every report
+    // is one overflow event of a candidate that finished); the device list, which holds at
+    // most ovf_cap reports of the whole launch, only supplies the starts.
+    out->overflow_written = out->total.overflow_count;
+    uint64_t listed = 0;
     for (size_t j = 0; j < res.ovf.size() / 4; ++j) {
       const uint64_t sl = res.ovf[4 * j + 2];
       if (sl < slot_base[i] || sl >= slot_base[i + 1]) continue;
-      if (out->overflow_starts && out->overflow_written < out->overflow_cap) {
-        out->overflow_starts[2 * out->overflow_written] = res.ovf[4 * j];
-        out->overflow_starts[2 * out->overflow_written + 1] = res.ovf[4 * j + 1];
+      if (out->overflow_starts && listed < out->overflow_cap) {
+        out->overflow_starts[2 * listed] = res.ovf[4 * j];
+        out->overflow_starts[2 * listed + 1] = res.ovf[4 * j + 1];
       }
-      out->overflow_written++;
+      listed++;
     }
+    // The list of the launch was cut short, so this interval's starts may be incomplete.
+    if (out->overflow_starts && listed < std::min(out->overflow_written, out->overflow_cap))
+      short_buf = true;
     if ((out->chunks && nc > out->chunks_cap) ||
         (out->overflow_starts && out->overflow_written > out->overflow_cap))
       short_buf = true;

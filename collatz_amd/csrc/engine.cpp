@@ -94,6 +94,7 @@ int Engine::ensure(const LaunchPlan& lp, uint64_t ovf_cap) {
   CZ_HIP(esc_.reserve(esc_cap));
   CZ_HIP(ovf_.reserve(4 * ovf_cap));
   CZ_HIP(cnt_.reserve(3));
+  CZ_HIP(stall_.reserve(kStallCap));
   CZ_HIP(ctl_.reserve(1));
   CZ_HIP(h_recs_.reserve(std::max<uint64_t>(1, lp.nslots)));
   CZ_HIP(h_cnt_.reserve(3));
@@ -150,6 +151,8 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   lists.ovf_count = cnt_.p + 1;
   lists.ovf_cap = ovf_cap;
   lists.stall_count = cnt_.p + 2;
+  lists.stall_slots = stall_.p;
+  lists.stall_cap = kStallCap;
   const Limits lim = make_limits(limbs, cutoff);
   GraphKey key;
   std::memset(&key, 0, sizeof(key));
@@ -205,7 +208,7 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   (void)hipEventElapsedTime(&ms, e0_, e1_);
   out.device_ms = ms;
   out.recs.assign(h_recs_.p, h_recs_.p + lp.nslots);
-  const uint64_t esc_n = h_cnt_.p[0], stall = h_cnt_.p[2];
+  const uint64_t esc_n = h_cnt_.p[0];
   if (limbs < 3 && esc_n > esc_.n)
     return fail(COLLATZ_ECAPACITY, "escalation list overflowed; rerun one width up");
   out.escalated = esc_n;
@@ -224,8 +227,13 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
     for (const EscEntry& e : h) out.esc_slots.push_back(e.slot);
     CZ_HIP(hipMemcpy(h_cnt_.p, cnt_.p, 24, hipMemcpyDeviceToHost));
   }
-  if (stall || h_cnt_.p[2])
-    return fail(COLLATZ_EDEVICE, "a trajectory did not descend within the check bound");
+  out.stalled = h_cnt_.p[2];
+  out.stall_slots.clear();
+  if (out.stalled) {
+    out.stall_slots.resize(std::min<uint64_t>(out.stalled, kStallCap));
+    CZ_HIP(hipMemcpy(out.stall_slots.data(), stall_.p, out.stall_slots.size() * 4,
+                     hipMemcpyDeviceToHost));
+  }
   out.ovf_total = h_cnt_.p[1];
   const uint64_t nov = std::min<uint64_t>(out.ovf_total, ovf_cap);
   out.ovf.resize(4 * nov);

@@ -60,6 +60,8 @@ struct LaunchResult {
   uint64_t ovf_total = 0;          // reports made (may exceed the stored ones)
   uint64_t escalated = 0;
   std::vector<uint32_t> esc_slots; // slot of each escalated candidate
+  uint64_t stalled = 0;            // candidates abandoned at the check bound
+  std::vector<uint32_t> stall_slots;  // their slots (the first kStallCap of them)
   double device_ms = 0;
 };
 
@@ -72,7 +74,9 @@ class Engine {
  public:
   explicit Engine(const collatz_sieve* sv) : sv_(sv) {}
   ~Engine();
-  // Runs one persistent launch over the plan (rows/keys/tiles/part) on stream s.
+  // Runs one persistent launch over the plan (rows/keys/tiles/part) on stream s.  Candidates
+  // abandoned at the check bound do not fail the launch: they are reported in out.stalled
+  // and out.stall_slots, and the records of the other slots stay valid.
   int run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cutoff, bool graph,
           uint64_t ovf_cap, hipStream_t s, LaunchResult& out);
   std::mutex mu;
@@ -86,6 +90,7 @@ class Engine {
   DevBuf<DevRec> tile_recs_, slot_recs_, esc_recs_;
   DevBuf<EscEntry> esc_;
   DevBuf<unsigned long long> cnt_;
+  DevBuf<uint32_t> stall_;
   DevBuf<Ctl> ctl_;
   HostBuf<DevRec> h_recs_;
   HostBuf<RowDesc> h_rows_;
@@ -110,6 +115,10 @@ class Engine {
 int run_escalated(const collatz_sieve* sv, const DevBuf<EscEntry>& esc, uint64_t n,
                   uint32_t cutoff, DeviceLists lists, std::vector<DevRec>& recs,
                   uint32_t slot_base, DevBuf<DevRec>& out, hipStream_t s);
+// The same with the pass's records in out[0, n) (the caller provides room for n).
+int run_escalated_into(const collatz_sieve* sv, const EscEntry* esc, uint64_t n,
+                       uint32_t cutoff, DeviceLists lists, std::vector<DevRec>& recs,
+                       uint32_t slot_base, DevRec* out, hipStream_t s);
 
 int device_cu_count(int device);
 

@@ -3,8 +3,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -128,6 +130,7 @@ struct PlainScratch {
   DevBuf<DevRec> block, chunk, esc_recs;
   DevBuf<EscEntry> esc;
   DevBuf<unsigned long long> cnt;
+  DevBuf<uint32_t> stall;
   hipEvent_t ev[2] = {nullptr, nullptr};
   PlainScratch() = default;
   PlainScratch(const PlainScratch&) = delete;
@@ -138,8 +141,52 @@ struct PlainScratch {
   }
 };
 
-int resolve_rows(const struct ::collatz_sieve* sv, RowPlan& plan, PlainScratch& scr,
-                 hipStream_t s);
+// Options of one plain-form call (collatz_verify_opts with defaults applied).
+struct RangeArgs {
+  uint32_t C = 24, limbs = 2, cutoff = 187;
+  uint32_t width = 2;  // word width of the kernels (launch_width)
+};
+int range_args(const collatz_verify_opts* o, RangeArgs& a);
+
+// Device memory of one plain-form call, as handed out by a PlainBuffers provider.
+struct PlainView {
+  uint64_t* keys = nullptr;      // row bounds to look up
+  uint32_t* key_cls = nullptr;
+  uint64_t* key_idx = nullptr;   // their indices in the sieve
+  RowDesc* rows = nullptr;
+  uint64_t* part = nullptr;      // first block of each chunk (nchunks + 1)
+  DevRec* block = nullptr;       // one record per block
+  DevRec* chunk = nullptr;       // one record per chunk
+  EscEntry* esc = nullptr;       // escalation list
+  uint64_t esc_cap = 0;
+  DevRec* esc_recs = nullptr;    // records of the three-limb pass
+  uint64_t* ovf = nullptr;       // overflow reports, 4 words each
+  uint64_t ovf_cap = 0;
+  unsigned long long* cnt = nullptr;  // escalations, overflow reports, stalls
+  uint32_t* stall = nullptr;     // slots of abandoned candidates (kStallCap)
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+// Where a plain-form call gets its device memory: the sieve's grow-on-demand buffers
+// (collatz_verify_range) or a fixed static array (collatz_verify_range_noalloc).
+class PlainBuffers {
+ public:
+  virtual ~PlainBuffers() = default;
+  virtual uint64_t max_rows() const = 0;
+  virtual uint64_t max_chunks() const = 0;
+  // Blocks the launch aims at (residues per lane follow from it).
+  virtual uint64_t target_blocks() const = 0;
+  virtual int keys(uint64_t n, PlainView& v) = 0;
+  // Everything a launch of these sizes needs; ovf_cap is the overflow reports wanted.
+  virtual int launch(uint64_t nrows, uint64_t nblocks, uint64_t nchunks, uint64_t ovf_cap,
+                     bool esc, PlainView& v) = 0;
+  virtual int esc_recs(uint64_t n, PlainView& v) = 0;
+};
+
+// The plain form over [lo, hi) with the given buffers (the caller serialises calls that
+// share them).
+int verify_plain_with(const struct ::collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs& a,
+                      hipStream_t s, collatz_range_result* out, PlainBuffers& B);
 
 Limits make_limits(int limbs, uint32_t cutoff);
 // Word width of a launch for a two- or three-limb request: two-limb work whose starts are
@@ -197,6 +244,11 @@ struct collatz_sieve {
   // plain_mu.
   mutable cz::PlainScratch plain;
   mutable std::mutex plain_mu;
+  // Launch shapes the autotuner chose for this sieve, per (word width, magnitude bucket):
+  // {waves per CU, checks per refill}.  The sieve's depth, mod 9 and look-ahead are fixed
+  // for the handle, so they need no place in the key.  Tuning of one sieve holds tune_mu.
+  mutable std::map<uint32_t, std::array<uint32_t, 2>> tuned;
+  mutable std::mutex tune_mu;
   cz::SieveArrays arrays() const {
     cz::SieveArrays a;
     a.rec = rec.p;

@@ -95,7 +95,12 @@ struct DeviceLists {
   unsigned long long* ovf_count;
   uint64_t ovf_cap;
   unsigned long long* stall_count;  // candidates abandoned after kMaxChecks checks
+  uint32_t* stall_slots;            // output slot of each abandoned candidate (the first
+  uint64_t stall_cap;               // stall_cap of them), so a caller can name the range
 };
+
+// Abandoned candidates reported per launch.
+constexpr uint64_t kStallCap = 256;
 
 // Safety bound on checks per candidate.  Stopping times of all n < 2^128 that have been
 // computed are a few thousand T-steps; a candidate still running after this many checks

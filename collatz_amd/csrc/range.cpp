@@ -7,12 +7,7 @@
 
 namespace cz {
 
-struct RangeArgs {
-  uint32_t C = 24, limbs = 2, cutoff = 187;
-  uint32_t width = 2;  // word width of the kernels (launch_width)
-};
-
-static int check_opts(const collatz_verify_opts* o, RangeArgs& a) {
+int range_args(const collatz_verify_opts* o, RangeArgs& a) {
   if (o) {
     if (o->chunk_log2) a.C = o->chunk_log2;
     if (o->limbs) a.limbs = o->limbs;
@@ -25,19 +20,18 @@ static int check_opts(const collatz_verify_opts* o, RangeArgs& a) {
 }
 
 // Fills i_lo / i_hi of every row from the sieve with one device lookup.
-int resolve_rows(const collatz_sieve* sv, RowPlan& plan, PlainScratch& scr, hipStream_t s) {
+static int resolve_rows(const collatz_sieve* sv, RowPlan& plan, PlainBuffers& B,
+                        PlainView& v, hipStream_t s) {
   std::vector<uint64_t> idx(plan.keys.size());
   if (!plan.keys.empty()) {
     const size_t nk = plan.keys.size();
-    CZ_HIP(scr.keys.reserve(nk));
-    CZ_HIP(scr.key_cls.reserve(nk));
-    CZ_HIP(scr.key_idx.reserve(nk));
-    CZ_HIP(hipMemcpyAsync(scr.keys.p, plan.keys.data(), nk * 8, hipMemcpyHostToDevice, s));
-    CZ_HIP(hipMemcpyAsync(scr.key_cls.p, plan.key_cls.data(), nk * 4, hipMemcpyHostToDevice,
-                          s));
-    CZ_HIP(launch_lower_bound(sv->rec.p, sv->class_start_dev.p, scr.keys.p, scr.key_cls.p, nk,
-                              scr.key_idx.p, s));
-    CZ_HIP(hipMemcpyAsync(idx.data(), scr.key_idx.p, nk * 8, hipMemcpyDeviceToHost, s));
+    int rc = B.keys(nk, v);
+    if (rc) return rc;
+    CZ_HIP(hipMemcpyAsync(v.keys, plan.keys.data(), nk * 8, hipMemcpyHostToDevice, s));
+    CZ_HIP(hipMemcpyAsync(v.key_cls, plan.key_cls.data(), nk * 4, hipMemcpyHostToDevice, s));
+    CZ_HIP(launch_lower_bound(sv->rec.p, sv->class_start_dev.p, v.keys, v.key_cls, nk,
+                              v.key_idx, s));
+    CZ_HIP(hipMemcpyAsync(idx.data(), v.key_idx, nk * 8, hipMemcpyDeviceToHost, s));
     CZ_HIP(hipStreamSynchronize(s));
   }
   for (size_t i = 0; i < plan.rows.size(); ++i) {
@@ -50,19 +44,18 @@ int resolve_rows(const collatz_sieve* sv, RowPlan& plan, PlainScratch& scr, hipS
 }
 
 // Runs the escalated candidates on three limbs and folds them into recs (by slot).
-int run_escalated(const collatz_sieve* sv, const DevBuf<EscEntry>& esc, uint64_t n,
-                  uint32_t cutoff, DeviceLists lists, std::vector<DevRec>& recs,
-                  uint32_t slot_base, DevBuf<DevRec>& out, hipStream_t s) {
+int run_escalated_into(const collatz_sieve* sv, const EscEntry* esc, uint64_t n,
+                       uint32_t cutoff, DeviceLists lists, std::vector<DevRec>& recs,
+                       uint32_t slot_base, DevRec* out, hipStream_t s) {
   if (n == 0) return COLLATZ_OK;
   // The list holds at most esc_cap entries; more escalations than that were dropped by the
   // kernels, so the range must be rerun on three limbs instead.
-  if (n > esc.n || n > lists.esc_cap)
+  if (n > lists.esc_cap)
     return fail(COLLATZ_ECAPACITY, "escalation list overflowed; rerun with limbs = 3");
-  CZ_HIP(out.reserve(n));
   const Limits lim3 = make_limits(3, cutoff);
-  CZ_HIP(launch_escalated(sv->arrays(), esc.p, n, 0, lim3, sv->pow3.p, out.p, lists, s));
+  CZ_HIP(launch_escalated(sv->arrays(), esc, n, 0, lim3, sv->pow3.p, out, lists, s));
   std::vector<DevRec> h(n);
-  CZ_HIP(hipMemcpyAsync(h.data(), out.p, n * sizeof(DevRec), hipMemcpyDeviceToHost, s));
+  CZ_HIP(hipMemcpyAsync(h.data(), out, n * sizeof(DevRec), hipMemcpyDeviceToHost, s));
   CZ_HIP(hipStreamSynchronize(s));
   for (const DevRec& r : h) {
     const uint32_t slot = r.pad - slot_base;
@@ -74,19 +67,29 @@ int run_escalated(const collatz_sieve* sv, const DevBuf<EscEntry>& esc, uint64_t
   return COLLATZ_OK;
 }
 
-// The caller holds sv->plain_mu.
-static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs& a,
-                        hipStream_t s, collatz_range_result* out) {
-  PlainScratch& scr = sv->plain;
+int run_escalated(const collatz_sieve* sv, const DevBuf<EscEntry>& esc, uint64_t n,
+                  uint32_t cutoff, DeviceLists lists, std::vector<DevRec>& recs,
+                  uint32_t slot_base, DevBuf<DevRec>& out, hipStream_t s) {
+  if (n == 0) return COLLATZ_OK;
+  if (n > esc.n || n > lists.esc_cap)
+    return fail(COLLATZ_ECAPACITY, "escalation list overflowed; rerun with limbs = 3");
+  CZ_HIP(out.reserve(n));
+  return run_escalated_into(sv, esc.p, n, cutoff, lists, recs, slot_base, out.p, s);
+}
+
+int verify_plain_with(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs& a,
+                      hipStream_t s, collatz_range_result* out, PlainBuffers& B) {
+  PlainView v;
   RowPlan plan;
-  int rc = plan_rows(sv->k, lo, hi, a.C, 0, sv->mod9 != 0, plan, 1ull << 26);
+  int rc = plan_rows(sv->k, lo, hi, a.C, 0, sv->mod9 != 0, plan, B.max_rows());
   if (rc) return rc;
-  rc = resolve_rows(sv, plan, scr, s);
+  if (plan.nchunks > B.max_chunks()) return fail(COLLATZ_EINVAL, "range has too many chunks");
+  rc = resolve_rows(sv, plan, B, v, s);
   if (rc) return rc;
   // Blocks of 256 lanes; each lane takes R residues of its row in turn.
   uint64_t slots = 0;
   for (const RowDesc& r : plan.rows) slots += r.i_hi - r.i_lo;
-  const uint64_t target_blocks = 1ull << 18;
+  const uint64_t target_blocks = B.target_blocks();
   const uint32_t R = (uint32_t)std::min<uint64_t>(
       256, std::max<uint64_t>(1, (slots + 256 * target_blocks - 1) / (256 * target_blocks)));
   const uint64_t span = 256ull * R;
@@ -100,76 +103,66 @@ static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeAr
     const uint64_t row = plan.chunk_first_row[c];
     part_start[c] = row < plan.rows.size() ? plan.rows[row].block_start : nblocks;
   }
-  DevBuf<RowDesc>& drows = scr.rows;
-  DevBuf<uint64_t>& dpart = scr.part;
-  DevBuf<DevRec>& dblock = scr.block;
-  DevBuf<DevRec>& dchunk = scr.chunk;
-  CZ_HIP(drows.reserve(plan.rows.size()));
-  CZ_HIP(dpart.reserve(part_start.size()));
-  CZ_HIP(dblock.reserve(std::max<uint64_t>(1, nblocks)));
-  CZ_HIP(dchunk.reserve(plan.nchunks));
-  CZ_HIP(hipMemcpyAsync(drows.p, plan.rows.data(), plan.rows.size() * sizeof(RowDesc),
-                        hipMemcpyHostToDevice, s));
-  CZ_HIP(hipMemcpyAsync(dpart.p, part_start.data(), part_start.size() * 8,
-                        hipMemcpyHostToDevice, s));
-  const uint64_t esc_cap = 1ull << 20;
   const uint64_t ovf_cap = std::max<uint64_t>(out->overflow_cap, 1024);
-  DevBuf<EscEntry>& desc = scr.esc;
-  DevBuf<uint64_t>& dovf = scr.ovf;
-  DevBuf<unsigned long long>& dcnt = scr.cnt;
-  CZ_HIP(desc.reserve(a.width < 3 ? esc_cap : 1));
-  CZ_HIP(dovf.reserve(4 * ovf_cap));
-  CZ_HIP(dcnt.reserve(3));
-  CZ_HIP(hipMemsetAsync(dcnt.p, 0, 24, s));
+  rc = B.launch(plan.rows.size(), nblocks, plan.nchunks, ovf_cap, a.width < 3, v);
+  if (rc) return rc;
+  CZ_HIP(hipMemcpyAsync(v.rows, plan.rows.data(), plan.rows.size() * sizeof(RowDesc),
+                        hipMemcpyHostToDevice, s));
+  CZ_HIP(hipMemcpyAsync(v.part, part_start.data(), part_start.size() * 8,
+                        hipMemcpyHostToDevice, s));
+  CZ_HIP(hipMemsetAsync(v.cnt, 0, 24, s));
   DeviceLists lists;
-  lists.esc = desc.p;
-  lists.esc_count = dcnt.p;
-  lists.esc_cap = a.width < 3 ? esc_cap : 0;
-  lists.ovf = dovf.p;
-  lists.ovf_count = dcnt.p + 1;
-  lists.ovf_cap = ovf_cap;
-  lists.stall_count = dcnt.p + 2;
+  lists.esc = v.esc;
+  lists.esc_count = v.cnt;
+  lists.esc_cap = a.width < 3 ? v.esc_cap : 0;
+  lists.ovf = v.ovf;
+  lists.ovf_count = v.cnt + 1;
+  lists.ovf_cap = v.ovf_cap;
+  lists.stall_count = v.cnt + 2;
+  lists.stall_slots = v.stall;
+  lists.stall_cap = kStallCap;
   const Limits lim = make_limits((int)a.width, a.cutoff);
   // Device time: the verification and reduction kernels, then the three-limb pass.
-  for (hipEvent_t& e : scr.ev)
-    if (!e) CZ_HIP(hipEventCreate(&e));
-  struct {
-    hipEvent_t a, b;
-  } ev{scr.ev[0], scr.ev[1]};
-  CZ_HIP(hipEventRecord(ev.a, s));
-  CZ_HIP(launch_verify_plain((int)a.width, sv->arrays(), drows.p, (uint32_t)plan.rows.size(),
-                             nblocks, R, sv->mod9, lim, sv->pow3.p, dblock.p, lists, s));
-  CZ_HIP(launch_reduce_chunks(dblock.p, dpart.p, (uint32_t)plan.nchunks, dchunk.p, s));
-  CZ_HIP(hipEventRecord(ev.b, s));
+  CZ_HIP(hipEventRecord(v.ev[0], s));
+  CZ_HIP(launch_verify_plain((int)a.width, sv->arrays(), v.rows, (uint32_t)plan.rows.size(),
+                             nblocks, R, sv->mod9, lim, sv->pow3.p, v.block, lists, s));
+  CZ_HIP(launch_reduce_chunks(v.block, v.part, (uint32_t)plan.nchunks, v.chunk, s));
+  CZ_HIP(hipEventRecord(v.ev[1], s));
   std::vector<DevRec> recs(plan.nchunks);
   unsigned long long cnt[3];
-  CZ_HIP(hipMemcpyAsync(recs.data(), dchunk.p, recs.size() * sizeof(DevRec),
+  CZ_HIP(hipMemcpyAsync(recs.data(), v.chunk, recs.size() * sizeof(DevRec),
                         hipMemcpyDeviceToHost, s));
-  CZ_HIP(hipMemcpyAsync(cnt, dcnt.p, 24, hipMemcpyDeviceToHost, s));
+  CZ_HIP(hipMemcpyAsync(cnt, v.cnt, 24, hipMemcpyDeviceToHost, s));
   CZ_HIP(hipStreamSynchronize(s));
   float ms = 0;
-  (void)hipEventElapsedTime(&ms, ev.a, ev.b);
-  if (a.width < 3 && cnt[0] > esc_cap) {
+  (void)hipEventElapsedTime(&ms, v.ev[0], v.ev[1]);
+  if (a.width < 3 && cnt[0] > lists.esc_cap) {
     // More escalations than the list holds: run the whole range one width up.
     RangeArgs b = a;
     b.width = a.width + 1;
-    return verify_plain(sv, lo, hi, b, s, out);
+    return verify_plain_with(sv, lo, hi, b, s, out, B);
   }
   uint64_t ovf_n = cnt[1];
   if (cnt[0]) {
-    CZ_HIP(hipEventRecord(ev.a, s));
-    rc = run_escalated(sv, desc, cnt[0], a.cutoff, lists, recs, 0, scr.esc_recs, s);
+    rc = B.esc_recs(cnt[0], v);
     if (rc) return rc;
-    CZ_HIP(hipEventRecord(ev.b, s));
-    CZ_HIP(hipEventSynchronize(ev.b));
+    CZ_HIP(hipEventRecord(v.ev[0], s));
+    rc = run_escalated_into(sv, v.esc, cnt[0], a.cutoff, lists, recs, 0, v.esc_recs, s);
+    if (rc) return rc;
+    CZ_HIP(hipEventRecord(v.ev[1], s));
+    CZ_HIP(hipEventSynchronize(v.ev[1]));
     float ems = 0;
-    (void)hipEventElapsedTime(&ems, ev.a, ev.b);
+    (void)hipEventElapsedTime(&ems, v.ev[0], v.ev[1]);
     ms += ems;
-    CZ_HIP(hipMemcpy(&cnt[1], dcnt.p + 1, 16, hipMemcpyDeviceToHost));
+    CZ_HIP(hipMemcpy(&cnt[1], v.cnt + 1, 16, hipMemcpyDeviceToHost));
     ovf_n = cnt[1];
   }
-  if (cnt[2])
-    return fail(COLLATZ_EDEVICE, "a trajectory did not descend within the check bound");
+  if (cnt[2]) {
+    uint32_t slot = 0;
+    CZ_HIP(hipMemcpy(&slot, v.stall, 4, hipMemcpyDeviceToHost));
+    return fail(COLLATZ_EDEVICE, "a trajectory did not descend within the check bound (chunk " +
+                                     std::to_string(plan.first_chunk + slot) + ")");
+  }
   out->escalated = cnt[0];
   out->device_ms = ms;
   out->checks = 0;
@@ -184,20 +177,75 @@ static int verify_plain(const collatz_sieve* sv, u128 lo, u128 hi, const RangeAr
   }
   out->total.chunk_index = plan.first_chunk;
   out->overflow_written = ovf_n;
+  uint64_t listed = 0;
   if (out->overflow_starts && out->overflow_cap) {
-    const uint64_t n = std::min<uint64_t>(std::min(ovf_n, ovf_cap), out->overflow_cap);
-    std::vector<uint64_t> h(4 * n);
-    if (n) CZ_HIP(hipMemcpy(h.data(), dovf.p, n * 32, hipMemcpyDeviceToHost));
-    for (uint64_t i = 0; i < n; ++i) {
+    listed = std::min<uint64_t>(std::min<uint64_t>(ovf_n, v.ovf_cap), out->overflow_cap);
+    std::vector<uint64_t> h(4 * listed);
+    if (listed) CZ_HIP(hipMemcpy(h.data(), v.ovf, listed * 32, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < listed; ++i) {
       out->overflow_starts[2 * i] = h[4 * i];
       out->overflow_starts[2 * i + 1] = h[4 * i + 1];
     }
   }
   if ((out->chunks && plan.nchunks > out->chunks_cap) ||
-      (out->overflow_starts && ovf_n > out->overflow_cap))
+      (out->overflow_starts && (ovf_n > out->overflow_cap || listed < ovf_n)))
     return fail(COLLATZ_ECAPACITY, "output buffer smaller than the result");
   return COLLATZ_OK;
 }
+
+// The sieve handle's buffers: grown on demand and kept for later calls, so a call makes no
+// device allocation unless it needs more room than any call before it.
+class ScratchBuffers final : public PlainBuffers {
+ public:
+  explicit ScratchBuffers(PlainScratch& s) : scr_(s) {}
+  uint64_t max_rows() const override { return 1ull << 26; }
+  uint64_t max_chunks() const override { return 0x7FFFFFFFull; }
+  uint64_t target_blocks() const override { return 1ull << 18; }
+  int keys(uint64_t n, PlainView& v) override {
+    CZ_HIP(scr_.keys.reserve(n));
+    CZ_HIP(scr_.key_cls.reserve(n));
+    CZ_HIP(scr_.key_idx.reserve(n));
+    v.keys = scr_.keys.p;
+    v.key_cls = scr_.key_cls.p;
+    v.key_idx = scr_.key_idx.p;
+    return COLLATZ_OK;
+  }
+  int launch(uint64_t nrows, uint64_t nblocks, uint64_t nchunks, uint64_t ovf_cap, bool esc,
+             PlainView& v) override {
+    const uint64_t esc_cap = 1ull << 20;
+    CZ_HIP(scr_.rows.reserve(std::max<uint64_t>(1, nrows)));
+    CZ_HIP(scr_.part.reserve(nchunks + 1));
+    CZ_HIP(scr_.block.reserve(std::max<uint64_t>(1, nblocks)));
+    CZ_HIP(scr_.chunk.reserve(std::max<uint64_t>(1, nchunks)));
+    CZ_HIP(scr_.esc.reserve(esc ? esc_cap : 1));
+    CZ_HIP(scr_.ovf.reserve(4 * ovf_cap));
+    CZ_HIP(scr_.cnt.reserve(3));
+    CZ_HIP(scr_.stall.reserve(kStallCap));
+    for (hipEvent_t& e : scr_.ev)
+      if (!e) CZ_HIP(hipEventCreate(&e));
+    v.rows = scr_.rows.p;
+    v.part = scr_.part.p;
+    v.block = scr_.block.p;
+    v.chunk = scr_.chunk.p;
+    v.esc = scr_.esc.p;
+    v.esc_cap = esc ? esc_cap : 0;
+    v.ovf = scr_.ovf.p;
+    v.ovf_cap = ovf_cap;
+    v.cnt = scr_.cnt.p;
+    v.stall = scr_.stall.p;
+    v.ev[0] = scr_.ev[0];
+    v.ev[1] = scr_.ev[1];
+    return COLLATZ_OK;
+  }
+  int esc_recs(uint64_t n, PlainView& v) override {
+    CZ_HIP(scr_.esc_recs.reserve(n));
+    v.esc_recs = scr_.esc_recs.p;
+    return COLLATZ_OK;
+  }
+
+ private:
+  PlainScratch& scr_;
+};
 
 }  // namespace cz
 
@@ -208,7 +256,7 @@ extern "C" int collatz_verify_range(const collatz_sieve* sv, const uint64_t lo[2
                                     void* stream, collatz_range_result* out) {
   if (!sv || !lo || !hi || !out) return fail(COLLATZ_EINVAL, "null argument");
   RangeArgs a;
-  int rc = check_opts(opts, a);
+  int rc = range_args(opts, a);
   if (rc) return rc;
   const u128 L = mk128(lo[0], lo[1]), H = mk128(hi[0], hi[1]);
   if (L >= H) return fail(COLLATZ_EINVAL, "empty range (lo >= hi)");
@@ -229,5 +277,6 @@ extern "C" int collatz_verify_range(const collatz_sieve* sv, const uint64_t lo[2
   }
   a.width = (uint32_t)launch_width((int)a.limbs, H, a.cutoff);
   std::lock_guard<std::mutex> lk(sv->plain_mu);
-  return verify_plain(sv, L, H, a, s, out);
+  ScratchBuffers B(sv->plain);
+  return verify_plain_with(sv, L, H, a, s, out, B);
 }

@@ -35,9 +35,22 @@ struct Request {
   uint64_t escalated = 0;
   uint64_t footprint = 0;
   bool admitted = false, done = false;
+  // Rows [launch_row0, next_row) belong to the launch numbered launch_id; a launch that fails
+  // as a whole gives them back (next_row = launch_row0), at most kMaxRetries times.
+  uint64_t launch_id = 0;
+  size_t launch_row0 = 0;
+  uint64_t launch_est0 = 0;
+  uint32_t retries = 0;
+  bool requeued = false;  // its rows of the last launch are to run again
+  // Waiters: a result is handed out once; the last waiter to leave frees the request.
+  uint32_t waiters = 0;
+  bool collected = false;
   int status = COLLATZ_OK;
   std::string error;
 };
+
+// Launches that fail as a whole (a HIP error) are retried this many times per request.
+constexpr uint32_t kMaxRetries = 2;
 
 }  // namespace cz
 
@@ -53,10 +66,12 @@ struct collatz_server {
   std::vector<cz::Request*> active;
   std::map<int64_t, cz::Request*> all;
   int64_t next_ticket = 0;
-  bool stop = false;
+  bool stop = false;   // destroy called: no more submissions, the worker drains and exits
+  bool dead = false;   // the worker has exited: waiters of unfinished requests give up
+  uint32_t nwaiters = 0;
+  uint64_t launches = 0;
   double t0 = 0;
   uint64_t bytes_in_use = 0;
-  cz::Tuning tn{0, 32, 2};  // waves per CU and checks per refill; tiles follow each launch
 };
 
 namespace cz {
@@ -71,8 +86,10 @@ static void server_loop(collatz_server* S) {
       uint64_t chunk;
     };
     std::vector<SlotRef> slots;
+    std::vector<Request*> in_launch;   // requests with rows in this launch
     std::vector<uint64_t> row_est;     // expected survivors of each row of the launch
     std::vector<uint8_t> slot_first;   // 1 where a row starts a new slot
+    uint64_t launch_id = 0;
     {
       std::unique_lock<std::mutex> lk(S->mu);
       S->cv.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->active.empty(); });
@@ -97,8 +114,15 @@ static void server_loop(collatz_server* S) {
       const uint64_t budget = std::min<uint64_t>(
           std::max<uint64_t>(S->opts.slice_candidates, backlog / 4), 16 * S->opts.slice_candidates);
       uint64_t taken = 0;
+      launch_id = ++S->launches;
       auto take = [&](Request* r, uint64_t want, bool one_row_always) {
         uint64_t mine = 0, last_chunk = UINT64_MAX;
+        if (r->next_row < r->plan.rows.size() && r->launch_id != launch_id) {
+          r->launch_id = launch_id;
+          r->launch_row0 = r->next_row;
+          r->launch_est0 = r->est_left;
+          in_launch.push_back(r);
+        }
         while (r->next_row < r->plan.rows.size() &&
                (mine == 0 && one_row_always ? true : (mine < want && taken < budget))) {
           const size_t i = r->next_row;
@@ -141,14 +165,18 @@ static void server_loop(collatz_server* S) {
         take(r, budget - taken, false);
       }
     }
-    // Tiles for this launch's size (a small launch spreads over all resident waves), then
-    // each row's pieces and each slot's first tile.
+    // The launch shape: the autotuned waves per CU and checks per refill of the sieve for
+    // this word width and magnitude (tuned on first use), and tiles for this launch's size
+    // (a small launch spreads over all resident waves); then each row's pieces and each
+    // slot's first tile.
     uint64_t est_total = 0;
     for (const uint64_t e : row_est) est_total += e;
-    const uint32_t tile = adaptive_tile(S->sv, est_total, S->tn.waves);
+    int width = launch_width(2, max_hi, S->opts.cutoff_log2);
+    const Tuning tn = lp.rows.empty() ? Tuning{4096, 32, 2}
+                                      : choose_tuning(S->sv, width, max_hi, est_total, 0, 0, 0);
     for (size_t i = 0; i < lp.rows.size(); ++i) {
       if (slot_first[i]) lp.part.push_back(lp.ntiles);
-      uint64_t pieces = (uint64_t)((double)row_est[i] / tile + 0.5);
+      uint64_t pieces = (uint64_t)((double)row_est[i] / tn.tile + 0.5);
       pieces = std::min<uint64_t>(std::max<uint64_t>(pieces, 1), 1u << 20);
       lp.rows[i].block_start = lp.ntiles;
       lp.ntiles += pieces;
@@ -157,24 +185,67 @@ static void server_loop(collatz_server* S) {
     lp.part.push_back(lp.ntiles);
     LaunchResult res;
     int rc = COLLATZ_OK;
+    std::string launch_error;
     if (!lp.rows.empty()) {
-      int width = launch_width(2, max_hi, S->opts.cutoff_log2);
-      rc = S->engine->run(lp, S->tn, width, S->opts.cutoff_log2, true, 0, S->stream, res);
+      rc = S->engine->run(lp, tn, width, S->opts.cutoff_log2, true, 0, S->stream, res);
       // More escalations than the list holds: the same launch one width up.
       while (rc == COLLATZ_ECAPACITY && width < 3)
-        rc = S->engine->run(lp, S->tn, ++width, S->opts.cutoff_log2, true, 0, S->stream, res);
+        rc = S->engine->run(lp, tn, ++width, S->opts.cutoff_log2, true, 0, S->stream, res);
+      if (rc != COLLATZ_OK) launch_error = collatz_last_error();
     }
     {
       std::lock_guard<std::mutex> lk(S->mu);
       if (rc != COLLATZ_OK) {
-        // Fail every request that had rows in this launch.
-        for (const SlotRef& sr : slots) {
-          sr.r->status = rc;
-          sr.r->error = collatz_last_error();
-          sr.r->next_row = sr.r->plan.rows.size();
+        // The launch failed as a whole, so none of its records can be used: each request
+        // with rows in it takes them again in a later launch, and fails after kMaxRetries.
+        for (Request* r : in_launch) {
+          if (++r->retries > kMaxRetries) {
+            r->status = rc;
+            r->error = launch_error;
+            r->next_row = r->plan.rows.size();
+          } else {
+            r->next_row = r->launch_row0;
+            r->est_left = r->launch_est0;
+          }
         }
       } else {
-        for (size_t i = 0; i < slots.size(); ++i) rec_merge_host(slots[i].r->recs[slots[i].chunk], res.recs[i]);
+        // A candidate abandoned at the check bound fails only the request that owns it; the
+        // other slots of the launch hold complete records and are folded.  If more were
+        // abandoned than the device listed, the requests none of the listed ones name run
+        // their rows of this launch again instead (their candidates may be among the
+        // unlisted ones).
+        std::vector<uint8_t> bad(slots.size(), 0);
+        for (uint32_t sl : res.stall_slots)
+          if (sl < slots.size()) bad[sl] = 1;
+        for (size_t i = 0; i < slots.size(); ++i) {
+          Request* r = slots[i].r;
+          if (bad[i] && r->status == COLLATZ_OK) {
+            r->status = COLLATZ_EDEVICE;
+            r->error = "a trajectory did not descend within the check bound (chunk " +
+                       std::to_string(r->plan.first_chunk + slots[i].chunk) + ")";
+            r->next_row = r->plan.rows.size();
+          }
+        }
+        const bool unlisted = res.stalled > res.stall_slots.size();
+        for (Request* r : in_launch) {
+          r->requeued = false;
+          if (unlisted && r->status == COLLATZ_OK) {
+            if (++r->retries > kMaxRetries) {
+              r->status = COLLATZ_EDEVICE;
+              r->error = "a trajectory did not descend within the check bound";
+              r->next_row = r->plan.rows.size();
+            } else {
+              r->next_row = r->launch_row0;
+              r->est_left = r->launch_est0;
+              r->requeued = true;
+            }
+          }
+        }
+        for (size_t i = 0; i < slots.size(); ++i) {
+          Request* r = slots[i].r;
+          if (r->status == COLLATZ_OK && !r->requeued)
+            rec_merge_host(r->recs[slots[i].chunk], res.recs[i]);
+        }
         for (uint32_t sl : res.esc_slots)
           if (sl < slots.size()) slots[sl].r->escalated++;
       }
@@ -266,6 +337,10 @@ int64_t collatz_server_submit(collatz_server* S, const collatz_interval* range) 
   for (DevRec& d : r->recs) std::memset(&d, 0, sizeof(d));
   r->footprint = r->plan.rows.size() * (sizeof(RowDesc) + 16) + r->plan.nchunks * sizeof(DevRec);
   std::lock_guard<std::mutex> lk(S->mu);
+  if (S->stop) {
+    delete r;
+    return fail(COLLATZ_EINVAL, "server is shutting down");
+  }
   r->ticket = S->next_ticket++;
   S->all[r->ticket] = r;
   S->pending.push_back(r);
@@ -280,29 +355,48 @@ int collatz_server_wait(collatz_server* S, int64_t ticket, double timeout_ms,
   auto it = S->all.find(ticket);
   if (it == S->all.end()) return fail(COLLATZ_EINVAL, "unknown ticket");
   Request* r = it->second;
-  auto ready = [&] { return r->done; };
+  if (r->collected) return fail(COLLATZ_EINVAL, "ticket already collected");
+  // Several threads may wait on one ticket: the first to wake takes the result, the others
+  // get EINVAL, and the last one to leave frees the request.  collatz_server_destroy waits
+  // for every waiter to leave before it frees anything.
+  r->waiters++;
+  S->nwaiters++;
+  auto ready = [&] { return r->done || r->collected || S->dead; };
+  bool woke = true;
   if (timeout_ms < 0) S->done_cv.wait(lk, ready);
-  else if (!S->done_cv.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready))
-    return fail(COLLATZ_ENOTREADY, "request not finished");
-  std::memset(out, 0, sizeof(*out));
-  out->chunks = r->plan.nchunks;
-  out->submit_ms = r->t_submit;
-  out->admit_ms = r->t_admit;
-  out->done_ms = r->t_done;
-  for (uint64_t c = 0; c < r->plan.nchunks; ++c) {
-    collatz_chunk ch;
-    chunk_from_rec(r->recs[c], r->plan.first_chunk + c, r->plan.n_checked[c], &ch);
-    if (chunks && c < cap) chunks[c] = ch;
-    chunk_merge(&out->total, ch);
+  else woke = S->done_cv.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready);
+  r->waiters--;
+  S->nwaiters--;
+  if (S->nwaiters == 0) S->done_cv.notify_all();  // (destroy may be waiting for this)
+  int rc = COLLATZ_OK;
+  if (r->collected) {
+    rc = fail(COLLATZ_EINVAL, "ticket already collected");
+  } else if (!r->done) {
+    rc = woke ? fail(COLLATZ_EINVAL, "server destroyed before the request finished")
+              : fail(COLLATZ_ENOTREADY, "request not finished");
+  } else {
+    std::memset(out, 0, sizeof(*out));
+    out->chunks = r->plan.nchunks;
+    out->submit_ms = r->t_submit;
+    out->admit_ms = r->t_admit;
+    out->done_ms = r->t_done;
+    for (uint64_t c = 0; c < r->plan.nchunks; ++c) {
+      collatz_chunk ch;
+      chunk_from_rec(r->recs[c], r->plan.first_chunk + c, r->plan.n_checked[c], &ch);
+      if (chunks && c < cap) chunks[c] = ch;
+      chunk_merge(&out->total, ch);
+    }
+    out->total.chunk_index = r->plan.first_chunk;
+    r->collected = true;
+    if (r->status != COLLATZ_OK) rc = fail(r->status, r->error);
+    else if (chunks && out->chunks > cap) rc = fail(COLLATZ_ECAPACITY, "chunk buffer too small");
+    S->done_cv.notify_all();  // other waiters of this ticket see it collected
   }
-  out->total.chunk_index = r->plan.first_chunk;
-  const int status = r->status;
-  const std::string err = r->error;
-  S->all.erase(it);
-  delete r;
-  if (status != COLLATZ_OK) return fail(status, err);
-  if (chunks && out->chunks > cap) return fail(COLLATZ_ECAPACITY, "chunk buffer too small");
-  return COLLATZ_OK;
+  if (r->collected && r->waiters == 0) {
+    S->all.erase(r->ticket);
+    delete r;
+  }
+  return rc;
 }
 
 int collatz_server_destroy(collatz_server* S) {
@@ -312,7 +406,14 @@ int collatz_server_destroy(collatz_server* S) {
     S->stop = true;
   }
   S->cv.notify_all();
+  // The worker finishes the admitted and pending requests, then exits.
   if (S->worker.joinable()) S->worker.join();
+  {
+    std::unique_lock<std::mutex> lk(S->mu);
+    S->dead = true;
+    S->done_cv.notify_all();
+    S->done_cv.wait(lk, [&] { return S->nwaiters == 0; });
+  }
   HipDeviceGuard guard(S->sv->device);
   delete S->engine;
   if (S->stream) (void)hipStreamDestroy(S->stream);

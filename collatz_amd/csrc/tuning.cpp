@@ -1,10 +1,9 @@
 // Autotuner for the persistent kernel.  For each (limbs, magnitude bucket) of a sieve it
 // times a short synthetic range of that magnitude under a small grid of launch shapes
 // (waves per CU, checks per refill) and keeps the fastest; the tile size (residues per
-// wave visit) follows the size of each launch.  Results are cached per sieve handle and
-// bucket; COLLATZ_AUTOTUNE=0 disables tuning.
+// wave visit) follows the size of each launch.  Results are kept on the sieve handle (and
+// go with it when it is freed), per bucket; COLLATZ_AUTOTUNE=0 disables tuning.
 #include <cstdlib>
-#include <map>
 #include <mutex>
 
 #include "engine.hpp"
@@ -15,9 +14,6 @@ namespace cz {
 struct BatchArgs;
 int verify_batch_tuned(const collatz_sieve* sv, u128 lo, u128 hi, int limbs, Tuning t,
                        hipStream_t s, double* ms);
-
-static std::mutex g_mu;
-static std::map<std::pair<const void*, uint32_t>, Tuning> g_cache;
 
 static uint32_t bitlen(u128 x) {
   uint32_t b = 0;
@@ -62,13 +58,14 @@ Tuning choose_tuning(const collatz_sieve* sv, int limbs, u128 max_hi, uint64_t c
   Tuning t{0, kDefaultWaves, kDefaultU};
   const char* env = std::getenv("COLLATZ_AUTOTUNE");
   const bool tune = !(env && env[0] == '0') && candidates > (1u << 22) && !(waves && U);
-  const uint32_t bucket = (sv->k << 16) + (uint32_t)limbs * 256 + bitlen(max_hi) / 8;
-  const auto key = std::make_pair((const void*)sv, bucket);
+  const uint32_t bucket = (uint32_t)limbs * 256 + bitlen(max_hi) / 8;
   if (tune) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    auto it = g_cache.find(key);
-    if (it != g_cache.end()) {
-      t = it->second;
+    // Callers on other sieves are not held up; callers on this one wait for its sweep.
+    std::lock_guard<std::mutex> lk(sv->tune_mu);
+    auto it = sv->tuned.find(bucket);
+    if (it != sv->tuned.end()) {
+      t.waves = it->second[0];
+      t.U = it->second[1];
     } else {
       // A range of the bucket's magnitude with about 2^29 candidates (three times as many
       // survivors for a look-ahead sieve, which keeps about a third), each shape timed three
@@ -105,7 +102,7 @@ Tuning choose_tuning(const collatz_sieve* sv, int limbs, u128 max_hi, uint64_t c
             t = Tuning{0, b, c};
           }
         }
-      g_cache[key] = t;
+      sv->tuned[bucket] = {t.waves, t.U};
     }
   }
   if (waves) t.waves = waves;

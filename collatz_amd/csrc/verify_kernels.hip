@@ -171,6 +171,12 @@ __device__ __forceinline__ void push_overflow(const DeviceLists& lists, uint64_t
   }
 }
 
+// A candidate abandoned at the check bound: counted, and its slot listed.
+__device__ __forceinline__ void push_stall(const DeviceLists& lists, uint32_t slot) {
+  const unsigned long long i = atomicAdd(lists.stall_count, 1ull);
+  if (i < lists.stall_cap) lists.stall_slots[i] = slot;
+}
+
 __device__ __forceinline__ void push_escalate(const DeviceLists& lists, uint64_t n_lo,
                                               uint64_t n_hi, uint64_t res_index, uint32_t slot) {
   const unsigned long long i = atomicAdd(lists.esc_count, 1ull);
@@ -335,7 +341,7 @@ __device__ __forceinline__ bool run_candidate(Cand<L>& c, Acc<L>& acc, const Lim
       return true;
     }
   }
-  atomicAdd(lists.stall_count, 1ull);
+  push_stall(lists, c.slot);
   return true;
 }
 
@@ -605,7 +611,7 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
       // to U checks beyond the bound first).
       const uint64_t stalled = ballot(c.steps >= lim.max_steps) & actm;
       if (__builtin_expect(stalled != 0, 0)) {
-        if (in_mask(stalled)) atomicAdd(lists.stall_count, 1ull);
+        if (in_mask(stalled)) push_stall(lists, slot);
         actm &= ~stalled;
       }
       // (3) U checks; a lane whose candidate ends before the last of them starts its

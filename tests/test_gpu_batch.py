@@ -72,3 +72,36 @@ def test_overlapping_and_duplicate_intervals(s20, o20, oracle):
         assert all(lo <= n < hi for n in r.overflow_starts)
         single = s20.verify_range(lo, hi, 21, cutoff_log2=70)
         assert sorted(single.overflow_starts) == sorted(r.overflow_starts)
+
+
+def test_overflow_count_exact_beyond_the_list(s20):
+    """More overflow reports than the device list keeps (cutoff 2^64 just above starts near
+    2^62): every form reports the exact count in overflow_written, with no start buffer and
+    with one that is too short (then the call fails with ECAPACITY and still reports it)."""
+    import ctypes as C
+
+    import collatz_amd as cz
+    lo, hi = 1 << 62, (1 << 62) + (1 << 24)
+    plain = s20.verify_range(lo, hi, 22, cutoff_log2=64, overflow_cap=1 << 20)
+    n = int(plain.total["overflow_count"])
+    assert n > 4096 and plain.overflow_written == n == len(plain.overflow_starts)
+    assert all(lo <= x < hi for x in plain.overflow_starts)
+    pers = s20.verify_range(lo, hi, 22, cutoff_log2=64, form=cz.FORM_PERSISTENT,
+                            overflow_cap=1 << 20)
+    assert pers.overflow_written == n
+    assert sorted(pers.overflow_starts) == sorted(plain.overflow_starts)
+    # Batched, no start buffers (the list of the launch keeps 1024 reports).
+    other = (lo + (1 << 30), lo + (1 << 30) + (1 << 20))
+    b = s20.verify_batch([(lo, hi), other], chunk_log2=22, cutoff_log2=64)
+    assert b[0].overflow_written == n
+    assert b[1].overflow_written == int(b[1].total["overflow_count"]) > 0
+    for form in (cz.FORM_PLAIN, cz.FORM_PERSISTENT):
+        res = cz.RangeResult()
+        buf = (C.c_uint64 * 64)()
+        res.overflow_starts = C.cast(buf, C.POINTER(C.c_uint64))
+        res.overflow_cap = 32
+        opts = cz.VerifyOpts(22, 2, 64, form)
+        rc = cz.load().collatz_verify_range(s20.handle, cz.u128_pair(lo), cz.u128_pair(hi),
+                                            C.byref(opts), None, C.byref(res))
+        assert rc == cz.ECAPACITY and res.overflow_written == n, (form, rc)
+        assert all(lo <= (buf[2 * i] | (buf[2 * i + 1] << 64)) < hi for i in range(32))
