@@ -24,7 +24,8 @@ FORM_PLAIN, FORM_PERSISTENT = 0, 1
 # Entry points declared in include/collatz.h.
 EXPORTS = [
     "collatz_sieve_build", "collatz_sieve_query", "collatz_sieve_copy", "collatz_sieve_free",
-    "collatz_verify_range", "collatz_verify_batch", "collatz_reference_run",
+    "collatz_verify_range", "collatz_verify_range_noalloc", "collatz_verify_batch",
+    "collatz_reference_run",
     "collatz_reference_tables",
     "collatz_server_create", "collatz_server_submit", "collatz_server_wait",
     "collatz_server_destroy", "collatz_version", "collatz_last_error",
@@ -150,6 +151,7 @@ def load(path: str = LIB_PATH):
     L.collatz_sieve_free.argtypes = [C.c_void_p]
     L.collatz_verify_range.argtypes = [C.c_void_p, P(C.c_uint64), P(C.c_uint64), P(VerifyOpts),
                                        C.c_void_p, P(RangeResult)]
+    L.collatz_verify_range_noalloc.argtypes = L.collatz_verify_range.argtypes
     L.collatz_verify_batch.argtypes = [C.c_void_p, C.c_uint64, P(Interval), P(BatchOpts),
                                        C.c_void_p, P(RangeResult)]
     L.collatz_reference_run.argtypes = [P(RefParams), P(Node), C.c_void_p, P(RefResult)]
@@ -244,7 +246,9 @@ class Sieve:
 
     def verify_range(self, lo: int, hi: int, chunk_log2: int = 24, limbs: int = 2,
                      cutoff_log2: int = 187, form: int = FORM_PLAIN, overflow_cap: int = 4096,
-                     stream=None) -> RangeOutput:
+                     stream=None, noalloc: bool = False) -> RangeOutput:
+        """collatz_verify_range over [lo, hi); with noalloc, collatz_verify_range_noalloc
+        (the plain form on the library's fixed device array)."""
         n = ((hi - 1) >> chunk_log2) - (lo >> chunk_log2) + 1
         chunks = np.zeros(n, CHUNK_DTYPE)
         ovf = np.zeros(2 * max(overflow_cap, 1), np.uint64)
@@ -254,8 +258,8 @@ class Sieve:
         res.overflow_starts = ovf.ctypes.data_as(C.POINTER(C.c_uint64))
         res.overflow_cap = overflow_cap
         opts = VerifyOpts(chunk_log2, limbs, cutoff_log2, form)
-        _check(load().collatz_verify_range(self._h, u128_pair(lo), u128_pair(hi), C.byref(opts),
-                                           stream, C.byref(res)))
+        fn = load().collatz_verify_range_noalloc if noalloc else load().collatz_verify_range
+        _check(fn(self._h, u128_pair(lo), u128_pair(hi), C.byref(opts), stream, C.byref(res)))
         total = np.frombuffer(bytearray(res.total), CHUNK_DTYPE)[0]
         nov = min(res.overflow_written, overflow_cap)
         starts = [int(ovf[2 * i]) | (int(ovf[2 * i + 1]) << 64) for i in range(nov)]
