@@ -58,8 +58,16 @@ def words_for(max_hi):
     return 3 if max_hi <= (1 << 68) else 4
 
 
-# gfx950 int32 VALU issue peak: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T lane-ops/s.
-VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+# 32-bit VALU peak in lane-ops/s: MI355X's published FP32 vector rate, 157.3 TFLOP/s, is
+# 78.6 T fused multiply-adds (two flops each) per second, which is 256 CUs x 2.4 GHz x two
+# wave64 instructions per CU per cycle.  The probe collatz_amd/valu_peak measures what the
+# integer instructions reach (profiles/valu_peak_gfx950.md): VOP2 add, xor, or and shifts
+# 1.6-1.7 wave instructions per CU per cycle (55-62 T), VOP3 ones (multiplies, funnel
+# shifts, bit-field extract, three-input adds) 0.95 (37-38 T), and the schedule's own check
+# loop on 96-bit values in registers 40.5 T algorithmic lane-ops/s.  bench.py runs the
+# probe beside the workload and reports the achieved rate against both.
+VALU_PEAK_TOPS = 157.3 / 2
+VALU_PROBE = os.path.join(ROOT, "collatz_amd", "valu_peak")
 RATES = (50, 100, 200, 500)
 FULL_RATES = (200, 500)
 # HBM traffic of the batched launch, measured with rocprofv3 on this workload
@@ -229,6 +237,26 @@ def lookahead_leg(cz, ivs, total_n, steps):
             "checks": sum(int(x.checks) for x in o), "total_hash": "%08X" % lah}
 
 
+def valu_probe():
+    """The int32 VALU ceiling measured on this GPU by collatz_amd/valu_peak (a child process,
+    about a second): per-instruction lane-op rates at full occupancy and the rate of the
+    schedule's check loop in registers.  None if the probe is not built."""
+    import subprocess
+    if not os.path.exists(VALU_PROBE):
+        return None
+    out = subprocess.run([VALU_PROBE], capture_output=True, text=True, timeout=120, check=True)
+    d = json.loads(out.stdout)
+    ops = d["ops"]
+    vop2 = ("v_add_u32", "v_xor_b32", "v_or_b32", "v_lshrrev_b32")
+    vop3 = ("v_mul_lo_u32", "v_alignbit_b32", "v_mad_u32_u24", "v_bfe_u32", "v_add3_u32")
+    return {"vop2_simple_tops": max(ops[k]["tops"] for k in vop2 if k in ops),
+            "vop3_tops": max(ops[k]["tops"] for k in vop3 if k in ops),
+            "checks_loop_tops": d["checks_loop"]["algorithmic_tops"],
+            "checks_loop_checks_per_s": d["checks_loop"]["checks_per_s"],
+            "per_instruction_tops": {k: round(v["tops"], 2) for k, v in ops.items()},
+            "source": "collatz_amd/valu_peak (profiles/valu_peak_gfx950.md)"}
+
+
 def tree_leg(cz, N):
     """The reference's own program (tree mode) for all n < 2^N from the root case line
     "3 2 2 9 1 0", its default knobs (LASTBITS 10, STEPSIZEINITIAL 24, 16 bit vectors,
@@ -377,9 +405,17 @@ def main():
                      "hbm_GBps": (count * READ_BYTES_PER_CANDIDATE + WRITE_BYTES_PER_LAUNCH)
                                  / (kern_ms * 1e-3) / 1e9,
                      "kernel": f"k_verify_persistent<{Lw}>",
-                     "ops": f"checks x (13L+7) + candidates x (4L+8), L={Lw}"},
+                     "ops": f"checks x (13L+7) + candidates x (4L+8), L={Lw}",
+                     "peak_source": "published FP32 vector 157.3 TFLOP/s as lane-ops (FMA = 2 "
+                                    "flop): two wave64 instructions per CU per cycle"},
         "cpu_baseline": None,
     }
+    probe = valu_probe()
+    if probe:
+        line["roofline"]["measured"] = probe
+        line["roofline"]["frac_of_checks_loop"] = achieved / probe["checks_loop_tops"]
+        log(f"valu probe: {probe['vop2_simple_tops']:.1f} T (VOP2), {probe['vop3_tops']:.1f} T "
+            f"(VOP3), check loop {probe['checks_loop_tops']:.1f} T")
 
     if not args.no_plain:
         plain_s = 0.0
@@ -472,9 +508,18 @@ def main():
     if not args.no_serve:
         reqs = W.request_stream(1, 20000)
         full = {int(x) for x in args.serve_full_rates.split(",") if x}
-        # Untimed: the autotuner's first sweeps for the magnitudes the stream reaches.
+        # Untimed: the autotuner's first sweeps for the (word width, magnitude) buckets the
+        # launches of the stream fall in, once per sieve: the first 300 requests one at a
+        # time (a launch per request, as at low rates) and then all at once (launches that
+        # mix many requests, as at high rates).
+        t_warm = time.perf_counter()
+        srv = cz.Server(sv, chunk_log2=BATCH_CHUNK)
+        for _, lo, hi in reqs[:300]:
+            srv.wait(srv.submit(lo, hi))
+        srv.close()
         served_leg(sv, reqs[:300], None, BATCH_CHUNK)
-        served = {}
+        served = {"warmup_s": time.perf_counter() - t_warm}
+        log(f"served warm-up (autotuner sweeps) {served['warmup_s']:.1f} s")
         for rate in (int(x) for x in args.serve_rates.split(",") if x):
             n = len(reqs) if (args.serve_all or rate in full) else \
                 min(len(reqs), max(50, int(rate * args.serve_seconds)))

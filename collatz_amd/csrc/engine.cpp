@@ -68,7 +68,8 @@ int plan_interval(const collatz_sieve* sv, u128 lo, u128 hi, uint32_t C, uint64_
 
 bool Engine::GraphKey::operator==(const GraphKey& o) const {
   return std::memcmp(bufs, o.bufs, sizeof(bufs)) == 0 && limbs == o.limbs &&
-         cutoff == o.cutoff && nblocks == o.nblocks && U == o.U && max_steps == o.max_steps;
+         cutoff == o.cutoff && nblocks == o.nblocks && U == o.U && max_steps == o.max_steps &&
+         max_loop == o.max_loop;
 }
 
 Engine::~Engine() {
@@ -93,11 +94,11 @@ int Engine::ensure(const LaunchPlan& lp, uint64_t ovf_cap) {
   CZ_HIP(slot_recs_.reserve(std::max<uint64_t>(1, lp.nslots)));
   CZ_HIP(esc_.reserve(esc_cap));
   CZ_HIP(ovf_.reserve(4 * ovf_cap));
-  CZ_HIP(cnt_.reserve(3));
+  CZ_HIP(cnt_.reserve(4));
   CZ_HIP(stall_.reserve(kStallCap));
   CZ_HIP(ctl_.reserve(1));
   CZ_HIP(h_recs_.reserve(std::max<uint64_t>(1, lp.nslots)));
-  CZ_HIP(h_cnt_.reserve(3));
+  CZ_HIP(h_cnt_.reserve(4));
   CZ_HIP(h_ctl_.reserve(1));
   if (!e0_) CZ_HIP(hipEventCreate(&e0_));
   if (!e1_) CZ_HIP(hipEventCreate(&e1_));
@@ -108,7 +109,7 @@ int Engine::ensure(const LaunchPlan& lp, uint64_t ovf_cap) {
 
 hipError_t Engine::enqueue(const GraphKey& k, const Limits& lim, DeviceLists lists,
                            hipStream_t s) {
-  hipError_t e = launch_engine_reset(ctl_.p, cnt_.p, 3, s);
+  hipError_t e = launch_engine_reset(ctl_.p, cnt_.p, 4, s);
   if (e != hipSuccess) return e;
   e = launch_rows_resolve_ctl(rows_.p, ctl_.p, sv_->rec.p, sv_->class_start_dev.p, keys_.p, s);
   if (e != hipSuccess) return e;
@@ -153,6 +154,7 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   lists.stall_count = cnt_.p + 2;
   lists.stall_slots = stall_.p;
   lists.stall_cap = kStallCap;
+  lists.loop_aborts = cnt_.p + 3;
   const Limits lim = make_limits(limbs, cutoff);
   GraphKey key;
   std::memset(&key, 0, sizeof(key));
@@ -172,6 +174,7 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
       (uint32_t)std::max<uint64_t>(1, (uint64_t)device_cu_count(sv_->device) * waves / 4);
   key.U = tn.U;
   key.max_steps = lim.max_steps;
+  key.max_loop = lim.max_loop;
 
   CZ_HIP(hipEventRecord(e0_, s));
   if (graph) {
@@ -199,7 +202,7 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   if (lp.nslots)
     CZ_HIP(hipMemcpyAsync(h_recs_.p, slot_recs_.p, lp.nslots * sizeof(DevRec),
                           hipMemcpyDeviceToHost, s));
-  CZ_HIP(hipMemcpyAsync(h_cnt_.p, cnt_.p, 24, hipMemcpyDeviceToHost, s));
+  CZ_HIP(hipMemcpyAsync(h_cnt_.p, cnt_.p, 32, hipMemcpyDeviceToHost, s));
   {
     Span sp("engine wait");
     CZ_HIP(hipStreamSynchronize(s));
@@ -225,9 +228,13 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
     std::vector<EscEntry> h(esc_n);
     CZ_HIP(hipMemcpy(h.data(), esc_.p, esc_n * sizeof(EscEntry), hipMemcpyDeviceToHost));
     for (const EscEntry& e : h) out.esc_slots.push_back(e.slot);
-    CZ_HIP(hipMemcpy(h_cnt_.p, cnt_.p, 24, hipMemcpyDeviceToHost));
+    CZ_HIP(hipMemcpy(h_cnt_.p, cnt_.p, 32, hipMemcpyDeviceToHost));
   }
   out.stalled = h_cnt_.p[2];
+  out.loop_aborts = h_cnt_.p[3];
+  if (out.loop_aborts)
+    return fail(COLLATZ_EDEVICE, "a persistent work loop reached its iteration bound (" +
+                                     std::to_string(out.loop_aborts) + " tiles abandoned)");
   out.stall_slots.clear();
   if (out.stalled) {
     out.stall_slots.resize(std::min<uint64_t>(out.stalled, kStallCap));

@@ -61,6 +61,7 @@ struct LaunchResult {
   uint64_t escalated = 0;
   std::vector<uint32_t> esc_slots; // slot of each escalated candidate
   uint64_t stalled = 0;            // candidates abandoned at the check bound
+  uint64_t loop_aborts = 0;        // tiles abandoned at the work-loop bound (records incomplete)
   std::vector<uint32_t> stall_slots;  // their slots (the first kStallCap of them)
   double device_ms = 0;
 };
@@ -104,7 +105,7 @@ class Engine {
   struct GraphKey {
     const void* bufs[8];
     int limbs;
-    uint32_t cutoff, nblocks, U, max_steps;
+    uint32_t cutoff, nblocks, U, max_steps, max_loop;
     bool operator==(const GraphKey& o) const;
   } gkey_{};
   hipError_t enqueue(const GraphKey& k, const Limits& lim, DeviceLists lists, hipStream_t s);

@@ -101,6 +101,11 @@ Limits make_limits(int limbs, uint32_t cutoff) {
   l.cutoff = cutoff;
   l.max_checks = kMaxChecks;
   l.max_steps = kMaxSteps;
+  l.max_loop = kMaxLoop;
+  if (const char* e = std::getenv("COLLATZ_LOOP_LIMIT")) {
+    const unsigned long v = std::strtoul(e, nullptr, 10);
+    if (v >= 1 && v < kMaxLoop) l.max_loop = (uint32_t)v;
+  }
   if (const char* e = std::getenv("COLLATZ_CHECK_LIMIT")) {
     const unsigned long v = std::strtoul(e, nullptr, 10);
     if (v >= 1 && v < kMaxChecks) {

@@ -97,6 +97,7 @@ struct DeviceLists {
   unsigned long long* stall_count;  // candidates abandoned after kMaxChecks checks
   uint32_t* stall_slots;            // output slot of each abandoned candidate (the first
   uint64_t stall_cap;               // stall_cap of them), so a caller can name the range
+  unsigned long long* loop_aborts;  // tiles abandoned at the work-loop bound (Limits)
 };
 
 // Abandoned candidates reported per launch.
@@ -122,6 +123,7 @@ constexpr uint32_t kMaxChecks = 1u << 16;
 // The same bound in T-steps for the persistent kernel: every half-check (odd run, even run)
 // takes at least one step, so 2^18 steps mean at most 2^17 checks.
 constexpr uint32_t kMaxSteps = 1u << 18;
+constexpr uint32_t kMaxLoop = 1u << 31;
 
 // Limits of the schedule for one launch (cutoff = CUTOFFPOWER, collatz_cudav7.cu:44-46).
 struct Limits {
@@ -134,6 +136,11 @@ struct Limits {
   // path from the host).
   uint32_t max_checks;
   uint32_t max_steps;
+  // Iterations of a persistent kernel's work loop per tile (and of its take loop per
+  // refill): kMaxLoop, or less under COLLATZ_LOOP_LIMIT (a test hook).  Every such loop makes
+  // progress on each iteration, so real work never comes near the bound; a wave that reaches
+  // it abandons its tile, counts the loop abort, and exits normally, and the call fails.
+  uint32_t max_loop;
 };
 
 // One survivor as the range kernels read it, 16 bytes in one load:

@@ -121,6 +121,7 @@ int verify_plain_with(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs
   lists.stall_count = v.cnt + 2;
   lists.stall_slots = v.stall;
   lists.stall_cap = kStallCap;
+  lists.loop_aborts = nullptr;  // (the plain kernel's loops have fixed trip counts)
   const Limits lim = make_limits((int)a.width, a.cutoff);
   // Device time: the verification and reduction kernels, then the three-limb pass.
   CZ_HIP(hipEventRecord(v.ev[0], s));

@@ -31,6 +31,7 @@ struct RefTotalsHost {
   unsigned int tile_counter;
   unsigned long long esc_count;
   unsigned long long checks;
+  unsigned long long aborts;
 };
 
 struct RefTables {
@@ -294,6 +295,9 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
   CZ_HIP(hipMemcpyAsync(&tot, dtot.p, sizeof(tot), hipMemcpyDeviceToHost, s));
   CZ_HIP(hipMemcpyAsync(bins, dbins.p, sizeof(bins), hipMemcpyDeviceToHost, s));
   CZ_HIP(hipStreamSynchronize(s));
+  if (tot.aborts)
+    return fail(COLLATZ_EDEVICE, "a persistent work loop reached its iteration bound (" +
+                                     std::to_string(tot.aborts) + " tiles abandoned)");
   if (tot.stall) return fail(COLLATZ_EDEVICE, "a trajectory did not descend within the check bound");
   out->complete_hash = tot.hash;
   out->leaves = leaves;

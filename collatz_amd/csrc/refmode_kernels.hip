@@ -151,6 +151,7 @@ struct RefTotals {
   unsigned int tile_counter;
   unsigned long long esc_count;  // candidates handed from 96 bits to the wide pass
   unsigned long long checks;     // checks run (counted per wave from its active-lane mask)
+  unsigned long long aborts;     // tiles abandoned at the work-loop bound (Limits::max_loop)
 };
 
 // num >= 2^c, or (strict) num > 2^c.
@@ -191,7 +192,13 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
     Num<L> v;
     uint32_t n[4];
     uint32_t checks = 0, wave_chk = 0;
+    uint32_t iters = 0;
+    bool aborted = false;
     for (;;) {
+      if (__builtin_expect(++iters > lim.max_loop, 0)) {
+        aborted = true;
+        break;
+      }
       const uint64_t need = __ballot(!act);
       if (need && cur < end) {
         const uint32_t want = __popcll(need);
@@ -269,6 +276,7 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
       atomicAdd(&tot->checks, (unsigned long long)wave_chk);
       if (stall) atomicAdd(&tot->stall, (unsigned long long)stall);
       if (!ovf && ov) atomicAdd(&tot->overflow, (unsigned long long)ov);
+      if (aborted) atomicAdd(&tot->aborts, 1ull);
     }
   }
 }
@@ -365,7 +373,15 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
         }
       }
     };
+    // Iterations of the tile's loop (and of the take loop per refill), uniform: a wave
+    // that reaches the bound abandons the tile; the abort is counted and the call fails.
+    uint32_t iters = 0;
+    bool aborted = false;
     for (;;) {
+      if (__builtin_expect(++iters > lim.max_loop, 0)) {
+        aborted = true;
+        break;
+      }
       // (1) Lanes without a candidate start their prefetched one.
       if (!act && pf) start_prefetched();
       // (2) Lanes without a prefetch take the next queue entries; an empty queue is
@@ -375,7 +391,12 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
         uint32_t want = __popcll(need);
         uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32),
                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+        uint32_t takes = 0;
         for (;;) {
+          if (__builtin_expect(++takes > lim.max_loop, 0)) {
+            aborted = true;
+            break;
+          }
           const uint32_t avail = qtail - qhead;
           const uint32_t take = avail < want ? avail : want;
           if (!pf) {
@@ -500,6 +521,7 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
       atomicAdd(&tot->checks, (unsigned long long)lane_chk);
       if (stall) atomicAdd(&tot->stall, (unsigned long long)stall);
       if (!ovf && ov) atomicAdd(&tot->overflow, (unsigned long long)ov);
+      if (aborted) atomicAdd(&tot->aborts, 1ull);
     }
   }
 }

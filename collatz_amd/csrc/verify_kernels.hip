@@ -523,7 +523,15 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
     if constexpr (LA) {
       if (lane < n) nx = tile_rec[lane];
     }
+    // Iterations of the tile's loop (and of the take loop per refill), uniform: a wave
+    // that reaches the bound abandons the tile; the abort is counted and the call fails.
+    uint32_t iters = 0;
+    bool aborted = false;
     for (;;) {
+      if (__builtin_expect(++iters > lim.max_loop, 0)) {
+        aborted = true;
+        break;
+      }
       // Starts the prefetched candidates of the lanes in S (no running candidate, a
       // prefetched record); a start that cannot run on these words is escalated.
       auto start_prefetched = [&](uint64_t S) {
@@ -557,7 +565,12 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
           uint32_t want = __popcll(need);
           uint32_t rank = lane_rank(need);
           uint64_t open = need;
+          uint32_t takes = 0;
           for (;;) {
+            if (__builtin_expect(++takes > lim.max_loop, 0)) {
+              aborted = true;
+              break;
+            }
             const uint32_t avail = scnt - shead;
             const uint32_t take = avail < want ? avail : want;
             const uint64_t tk = ballot(rank < take) & open;
@@ -645,6 +658,9 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
           if (S) start_prefetched(S);
         }
       }
+    }
+    if (__builtin_expect(aborted, 0)) {
+      if (lane == 0) atomicAdd(lists.loop_aborts, 1ull);
     }
     DevRec r = acc_to_rec<L>(acc);
     wave_merge(r);

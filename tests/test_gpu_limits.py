@@ -1,6 +1,9 @@
-"""The failure path of the kernels' own check bound, forced from the host: with
-COLLATZ_CHECK_LIMIT set to a few checks, candidates that need more are abandoned (every
-kernel still exits normally) and each form reports the failure instead of a result."""
+"""The failure paths of the kernels' own bounds, forced from the host: with
+COLLATZ_CHECK_LIMIT set to a few checks, candidates that need more are abandoned, and with
+COLLATZ_LOOP_LIMIT set to one iteration the persistent kernels abandon their tiles; every
+kernel still exits normally and each form reports the failure instead of a result.  Also the
+served form's isolation of such a failure to the request it belongs to, and the lifetime
+rules of its tickets."""
 import os
 
 import pytest
@@ -34,9 +37,42 @@ def test_check_bound_failure_is_reported(tiny_check_limit):
     srv.close()
 
 
+def test_loop_bound_failure_is_reported():
+    """The work-loop bound of the persistent kernels (iterations per tile, and of the take
+    loop per refill), forced to one iteration with COLLATZ_LOOP_LIMIT: waves abandon their
+    tiles and exit normally, and the persistent, batched, served and reference-mode forms
+    fail with the bound named; the plain kernel has no such loop and still succeeds."""
+    import collatz_amd as cz
+    os.environ["COLLATZ_LOOP_LIMIT"] = "1"
+    try:
+        sv = cz.Sieve(20)
+        sla = cz.Sieve(20, lookahead_bits=16)
+        lo, hi = (1 << 40) + 1, (1 << 40) + (1 << 24)
+        for s in (sv, sla):
+            with pytest.raises(cz.CollatzError) as e:
+                s.verify_range(lo, hi, 20, form=cz.FORM_PERSISTENT)
+            assert e.value.code == cz.EDEVICE and "iteration bound" in str(e.value)
+            with pytest.raises(cz.CollatzError) as e:
+                s.verify_batch([(lo, hi), (lo << 8, (lo << 8) + (1 << 22))], chunk_log2=20,
+                               use_graph=True)
+            assert "iteration bound" in str(e.value)
+        with pytest.raises(cz.CollatzError) as e:
+            cz.reference_run((3, 2, 2, 9, 1, 0), 36, 10)
+        assert "iteration bound" in str(e.value)
+        srv = cz.Server(sv, chunk_log2=20)
+        t = srv.submit(lo, hi)
+        with pytest.raises(cz.CollatzError) as e:
+            srv.wait(t, 60000)
+        assert "iteration bound" in str(e.value)
+        srv.close()
+        assert int(sv.verify_range(lo, hi, 20, form=cz.FORM_PLAIN).total["count"]) > 0
+    finally:
+        del os.environ["COLLATZ_LOOP_LIMIT"]
+
+
 def test_default_bound_after_the_hook():
-    """Without the variable the same calls succeed (the launch sequence captured under the
-    lowered bound is not replayed)."""
+    """Without the variables the same calls succeed (the launch sequences captured under the
+    lowered bounds are not replayed)."""
     import collatz_amd as cz
     sv = cz.Sieve(20)
     lo, hi = (1 << 40) + 1, (1 << 40) + (1 << 22)
