@@ -237,6 +237,37 @@ def lookahead_leg(cz, ivs, total_n, steps):
             "checks": sum(int(x.checks) for x in o), "total_hash": "%08X" % lah}
 
 
+# main.rs's configuration (LASTBITS 6, eight bit vectors, its own search3 in 16-step blocks):
+# the paper's one-core Rust timings and counts of integers iterated (p.3 section 1.1, p.10
+# Fig 5.1).
+RUST_PUBLISHED = {40: {"seconds": 21.0, "candidates": 756583624},
+                  50: {"seconds": 176 * 60.0, "candidates": 434413363553}}
+
+
+def rust_leg(cz, Ns):
+    """The whole tree below 2^N under main.rs semantics from the root "3 2 2 9 1 0": Step 1
+    with main.rs's prunes, its Step 2 filters and its search3 blocks; numcases is what
+    main.rs prints."""
+    out = {}
+    for N in Ns:
+        t = time.perf_counter()
+        r = cz.reference_run((3, 2, 2, 9, 1, 0), N, 6, num_bv=8, semantics=cz.SEM_RUST,
+                             cutoff_log2=120)
+        d = time.perf_counter() - t
+        rec = {"seconds": d, "n_per_s": (1 << N) / d, "candidates": int(r.candidates),
+               "numcases": int(r.numcases), "leaves": int(r.leaves),
+               "overflow_count": int(r.overflow_count)}
+        pub = RUST_PUBLISHED.get(N)
+        if pub:
+            rec["published"] = dict(pub, hardware="1 AMD Zen 3 core at 4.6 GHz (main.rs)",
+                                    source="paper p.3 section 1.1, p.10 Fig 5.1")
+            rec["candidates_match_published"] = rec["candidates"] == pub["candidates"]
+            rec["speedup_vs_published"] = pub["seconds"] / d
+        out[str(N)] = rec
+        log(f"main.rs configuration, all n < 2^{N}: {rec}")
+    return out
+
+
 def valu_probe():
     """The int32 VALU ceiling measured on this GPU by collatz_amd/valu_peak (a child process,
     about a second): per-instruction lane-op rates at full occupancy and the rate of the
@@ -308,6 +339,11 @@ def main():
                     help="MAXBITS of the reference-program leg (the whole tree below 2^N)")
     ap.add_argument("--only-tree", action="store_true",
                     help="run only the reference-program leg and print its record")
+    ap.add_argument("--no-rust", action="store_true")
+    ap.add_argument("--rust-n", default="40,50",
+                    help="MAXBITS of the main.rs-configuration legs (comma-separated)")
+    ap.add_argument("--only-rust", action="store_true",
+                    help="run only the main.rs-configuration legs and print them")
     ap.add_argument("--only-lookahead", action="store_true",
                     help="run only the look-ahead leg (batch workload, B = 16) and print it")
     ap.add_argument("--large-steps", type=int, default=3)
@@ -335,6 +371,10 @@ def main():
 
     if args.only_tree:
         print(json.dumps(tree_leg(cz, args.tree_n)), flush=True)
+        return
+    if args.only_rust:
+        print(json.dumps(rust_leg(cz, [int(x) for x in args.rust_n.split(",") if x])),
+              flush=True)
         return
 
     ivs = W.batch_intervals(0, 4096, K)
@@ -504,6 +544,9 @@ def main():
     if not args.no_tree:
         line["reference_tree_mode"] = tree_leg(cz, args.tree_n)
         log(f"reference program, all n < 2^{args.tree_n}: {line['reference_tree_mode']}")
+
+    if not args.no_rust:
+        line["rust_mode"] = rust_leg(cz, [int(x) for x in args.rust_n.split(",") if x])
 
     if not args.no_serve:
         reqs = W.request_stream(1, 20000)

@@ -110,7 +110,8 @@ class RefResult(C.Structure):
                 ("leaves", C.c_uint64), ("candidates", C.c_uint64), ("checks", C.c_uint64),
                 ("bin_leaves", C.c_uint64 * 17), ("overflow_count", C.c_uint64),
                 ("overflow_starts", C.POINTER(C.c_uint64)), ("overflow_cap", C.c_uint64),
-                ("overflow_written", C.c_uint64), ("device_ms", C.c_double)]
+                ("overflow_written", C.c_uint64), ("device_ms", C.c_double),
+                ("numcases", C.c_uint64)]
 
 
 class ServerOpts(C.Structure):

@@ -8,6 +8,12 @@
 // --semantics opencl, collatz_openclv2.cpp:279-280).  No CPU spot checks are run, so
 // "Spot checks" is always 0.
 //
+//   echo "3 2 2 9" | collatz_run --semantics rust [--N 40] [--A 6] [--cutoff 120]
+// is main.rs's protocol (main.rs:85-98): the first four numbers on stdin (doubleeven and
+// numevens are 0 there, main.rs:97), "Done with precompute" and "numcases = X" on stderr, and
+// in decimal on stdout every start whose trajectory reached 2^cutoff after a 16-step block
+// (main.rs:336-338).
+//
 //   collatz_run [--N 45] [--A 10] ... --cases FILE
 // runs every case line of FILE in turn (the reference's users start one process per line),
 // with one "line: launches L hash H" line each on stderr and a closing line with the XOR of
@@ -151,6 +157,7 @@ static int usage() {
 int main(int argc, char** argv) {
   uint32_t N = 45, A = 10, B = 24, cutoff = 0, k = 32, chunk = 24, limbs = 2, form = 0;
   uint32_t lookahead = 0;
+  bool a_given = false;
   const char* out_path = nullptr;
   const char* cases_path = nullptr;
   uint32_t sem = COLLATZ_SEM_CUDA;
@@ -163,7 +170,7 @@ int main(int argc, char** argv) {
       return argv[++i];
     };
     if (!std::strcmp(a, "--N")) N = std::atoi(next());
-    else if (!std::strcmp(a, "--A")) A = std::atoi(next());
+    else if (!std::strcmp(a, "--A")) A = std::atoi(next()), a_given = true;
     else if (!std::strcmp(a, "--B")) B = std::atoi(next());
     else if (!std::strcmp(a, "--cutoff")) cutoff = std::atoi(next());
     else if (!std::strcmp(a, "--k")) k = std::atoi(next());
@@ -235,20 +242,27 @@ int main(int argc, char** argv) {
     collatz_sieve_free(sv);
     return 0;
   }
-  collatz_ref_params p = {N, A, B, sem == COLLATZ_SEM_RUST ? 8u : 16u, sem,
-                          cutoff ? cutoff : 100u, -1, 0};
+  const bool rust = sem == COLLATZ_SEM_RUST;
+  // main.rs's knobs: LASTBITS 6, eight bit vectors and its overflow test at 2^120
+  // (main.rs:20, :31, :336); MAXBITS stays --N (main.rs's 72 is beyond one run here).
+  if (rust && !a_given) A = 6;
+  collatz_ref_params p = {N, A, B, rust ? 8u : 16u, sem, cutoff ? cutoff : (rust ? 120u : 100u),
+                          -1, 0};
   std::vector<uint64_t> ovf(2 * (1 << 16));
   // One case line "base sigfig threepower number doubleeven numevens" through the whole
   // reference program; false (with a message) if the line does not parse or the run fails.
+  // Under main.rs semantics the line is main.rs's input: its first four numbers, with
+  // doubleeven and numevens 0 whatever follows (main.rs:88-97).
   auto run_line = [&](const std::string& line, collatz_ref_result& r) -> bool {
     std::istringstream iss(line);
     uint64_t t[6] = {0};
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < (rust ? 4 : 6); ++i) {
       if (!(iss >> t[i])) {
         std::fprintf(stderr, "collatz_run: not a case line: %s\n", line.c_str());
         return false;
       }
     }
+    if (rust) t[4] = t[5] = 0;
     collatz_node node;
     std::memset(&node, 0, sizeof(node));
     node.base[0] = t[0];
@@ -265,8 +279,12 @@ int main(int argc, char** argv) {
       std::fprintf(stderr, "collatz_run: %s\n", collatz_last_error());
       return false;
     }
-    for (uint64_t i = 0; i < std::min<uint64_t>(r.overflow_written, 1 << 16); ++i)
-      std::printf("%s\n", hex_start(ovf[2 * i], ovf[2 * i + 1]).c_str());
+    // Overflow starts: hex as the CUDA and OpenCL programs print them (collatz_cudav7.cu:298),
+    // decimal as main.rs does (main.rs:337).
+    for (uint64_t i = 0; i < std::min<uint64_t>(r.overflow_written, 1 << 16); ++i) {
+      if (rust) std::printf("%s\n", to_dec(((u128)ovf[2 * i + 1] << 64) | ovf[2 * i]).c_str());
+      else std::printf("%s\n", hex_start(ovf[2 * i], ovf[2 * i + 1]).c_str());
+    }
     return true;
   };
   if (cases_path) {
@@ -280,7 +298,7 @@ int main(int argc, char** argv) {
     }
     char buf[512];
     uint32_t all = 0;
-    uint64_t lines = 0, launches = 0, cands = 0;
+    uint64_t lines = 0, launches = 0, cands = 0, numcases = 0;
     const auto t0 = std::chrono::steady_clock::now();
     while (std::fgets(buf, sizeof(buf), f)) {
       std::string line(buf);
@@ -291,27 +309,48 @@ int main(int argc, char** argv) {
         std::fclose(f);
         return 1;
       }
-      std::fprintf(stderr, "%s: launches %" PRIu64 " hash %X\n", line.c_str(), r.launches,
-                   r.complete_hash);
+      if (rust)
+        std::fprintf(stderr, "%s: numcases %" PRIu64 "\n", line.c_str(), r.numcases);
+      else
+        std::fprintf(stderr, "%s: launches %" PRIu64 " hash %X\n", line.c_str(), r.launches,
+                     r.complete_hash);
       all ^= r.complete_hash;
       ++lines;
       launches += r.launches;
       cands += r.candidates;
+      numcases += r.numcases;
     }
     std::fclose(f);
     std::fflush(stdout);
     const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    std::fprintf(stderr,
-                 "%" PRIu64 " case lines, %" PRIu64 " launches, %" PRIu64 " candidates, "
-                 "XOR of complete hashes = %X, %.3f s.\n", lines, launches, cands, all, s);
+    if (rust)
+      std::fprintf(stderr,
+                   "%" PRIu64 " case lines, %" PRIu64 " candidates, numcases = %" PRIu64
+                   ", %.3f s.\n", lines, cands, numcases, s);
+    else
+      std::fprintf(stderr,
+                   "%" PRIu64 " case lines, %" PRIu64 " launches, %" PRIu64 " candidates, "
+                   "XOR of complete hashes = %X, %.3f s.\n", lines, launches, cands, all, s);
     return 0;
   }
   std::string line;
-  std::getline(std::cin, line);
+  if (rust) {
+    // main.rs reads the whole of stdin and splits it on whitespace (main.rs:88-94).
+    std::ostringstream in;
+    in << std::cin.rdbuf();
+    line = in.str();
+    // main.rs prints this once its tables are built (main.rs:85-86); here the run itself
+    // builds them on the device.
+    std::fprintf(stderr, "Done with precompute\n");
+  } else {
+    std::getline(std::cin, line);
+  }
   collatz_ref_result r;
   if (!run_line(line, r)) return 1;
   std::fflush(stdout);
-  if (sem == COLLATZ_SEM_OPENCL) {
+  if (rust) {
+    std::fprintf(stderr, "numcases = %" PRIu64 "\n", r.numcases);
+  } else if (sem == COLLATZ_SEM_OPENCL) {
     std::fprintf(stderr, "Number of kernel launches = %" PRIu64 " \n", r.launches);
     std::fprintf(stderr, "Spot checks = %u. Complete hash = %X.\n", 0u, r.complete_hash);
   } else {

@@ -117,7 +117,9 @@ Limits make_limits(int limbs, uint32_t cutoff) {
     l.fast_word = 2;
     l.fast_thr = 1ull << (kNarrowLog2 - 64);
   } else if (limbs == 2) {
-    const uint32_t c = std::min<uint32_t>(cutoff, kWideLog2);
+    // (A cutoff below 2^64 happens only under main.rs semantics, whose test does not use
+    // the fast threshold.)
+    const uint32_t c = std::max<uint32_t>(std::min<uint32_t>(cutoff, kWideLog2), 64);
     l.fast_word = 2;
     l.fast_thr = 1ull << (c - 64);
   } else if (cutoff >= 128) {
@@ -125,7 +127,7 @@ Limits make_limits(int limbs, uint32_t cutoff) {
     l.fast_thr = 1ull << (cutoff - 128);
   } else {
     l.fast_word = 2;
-    l.fast_thr = 1ull << (cutoff - 64);
+    l.fast_thr = 1ull << (std::max<uint32_t>(cutoff, 64) - 64);
   }
   return l;
 }
@@ -190,7 +192,7 @@ int plan_rows(uint32_t k, u128 lo, u128 hi, uint32_t C, uint32_t slot_base, bool
 }
 
 static bool root_pruned(const SieveRoot& r) {
-  if (r.rust && r.number <= r.base) return true;
+  if (r.rust && mk128(r.number, r.number_hi) <= r.base) return true;
   if (kDenominator * r.threepower <= kNumerator * r.sigfig) return true;
   if (r.numevens >= 2 && r.numevens % 2 == 0 &&
       kNumerator * (r.sigfig - 1) >= kDenominator * (r.threepower - 1))
@@ -201,25 +203,29 @@ static bool root_pruned(const SieveRoot& r) {
 static uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
 
 int build_leaves(const SieveRoot& root, hipStream_t s, LeafSet& out) {
-  if (root.depth < root.sigfig || root.depth - root.sigfig > 36 || root.depth > 40)
+  if (root.depth < root.sigfig || root.depth - root.sigfig > 36 || root.depth > kWalkMaxDepth)
     return fail(COLLATZ_EINVAL, "sieve depth out of range");
+  if (root.depth <= kWalk64MaxDepth && root.number_hi)
+    return fail(COLLATZ_EINVAL, "root number too large for its depth");
   out.count = 0;
+  out.wide = root.depth > kWalk64MaxDepth;
   const uint32_t e = root.depth - root.sigfig;
   if (root_pruned(root) || e < 7) {
     // Tiny trees (and pruned roots) are not worth a device pass; the bit array is empty
     // and the leaves, if any, are produced by walking on the host.
-    std::vector<uint64_t> r, n;
+    std::vector<uint64_t> r, n, nh;
     std::vector<uint32_t> m;
     if (!root_pruned(root)) {
       for (uint64_t h = 0; h < (1ull << e); ++h) {
-        uint64_t base = root.base, number = root.number, p3 = 1;
+        uint64_t base = root.base;
+        u128 number = mk128(root.number, root.number_hi), p3 = 1;
         for (uint32_t i = 0; i < root.threepower; ++i) p3 *= 3;
         uint32_t f = root.threepower, de = root.doubleeven, ne = root.numevens;
         bool alive = true;
         for (uint32_t j = 0; j < e && alive; ++j) {
           const uint32_t sd = root.sigfig + j;
           const uint32_t bit = (h >> j) & 1;
-          const uint64_t nb = number + (bit ? p3 : 0);
+          const u128 nb = number + (bit ? p3 : 0);
           if (bit) base |= 1ull << sd;
           const uint32_t inplay = kNumerator * (sd + 1) >= kDenominator * f;
           if ((nb & 1) == 0) {
@@ -238,7 +244,8 @@ int build_leaves(const SieveRoot& root, hipStream_t s, LeafSet& out) {
         }
         if (alive) {
           r.push_back(base);
-          n.push_back(number);
+          n.push_back(lo64(number));
+          nh.push_back(hi64(number));
           m.push_back(f | ((uint32_t)(base % 9) << 8) | (de << 12) | ((ne & 0xFF) << 16));
         }
       }
@@ -246,10 +253,13 @@ int build_leaves(const SieveRoot& root, hipStream_t s, LeafSet& out) {
     out.count = r.size();
     CZ_HIP(out.res.alloc(std::max<size_t>(1, r.size())));
     CZ_HIP(out.num.alloc(std::max<size_t>(1, r.size())));
+    CZ_HIP(out.num_hi.alloc(out.wide ? std::max<size_t>(1, r.size()) : 1));
     CZ_HIP(out.meta.alloc(std::max<size_t>(1, r.size())));
     if (!r.empty()) {
       CZ_HIP(hipMemcpyAsync(out.res.p, r.data(), r.size() * 8, hipMemcpyHostToDevice, s));
       CZ_HIP(hipMemcpyAsync(out.num.p, n.data(), n.size() * 8, hipMemcpyHostToDevice, s));
+      if (out.wide)
+        CZ_HIP(hipMemcpyAsync(out.num_hi.p, nh.data(), nh.size() * 8, hipMemcpyHostToDevice, s));
       CZ_HIP(hipMemcpyAsync(out.meta.p, m.data(), m.size() * 4, hipMemcpyHostToDevice, s));
       CZ_HIP(hipStreamSynchronize(s));
     }
@@ -258,15 +268,16 @@ int build_leaves(const SieveRoot& root, hipStream_t s, LeafSet& out) {
   }
   const uint32_t p = std::min<uint32_t>(e, 20);
   const uint64_t np = 1ull << p;
-  DevBuf<uint64_t> pnum;
+  DevBuf<uint64_t> pnum, pnum_hi;
   DevBuf<uint32_t> pmeta;
   DevBuf<uint64_t> alive;
   CZ_HIP(pnum.alloc(np));
+  CZ_HIP(pnum_hi.alloc(out.wide ? np : 1));
   CZ_HIP(pmeta.alloc(np));
   const uint64_t alive_words64 = round_up(std::max<uint64_t>(1, np / 64), 512);
   CZ_HIP(alive.alloc(alive_words64));
   CZ_HIP(hipMemsetAsync(alive.p, 0, alive_words64 * 8, s));
-  CZ_HIP(launch_sieve_prefix(root, p, pnum.p, pmeta.p, alive.p, s));
+  CZ_HIP(launch_sieve_prefix(root, p, pnum.p, pnum_hi.p, pmeta.p, alive.p, s));
   const uint32_t* bits32;
   uint64_t nwords32;
   if (e > p) {
@@ -275,7 +286,7 @@ int build_leaves(const SieveRoot& root, hipStream_t s, LeafSet& out) {
     CZ_HIP(out.bits.alloc(nwords32));
     if (nwords32 > words)
       CZ_HIP(hipMemsetAsync(out.bits.p + words, 0, (nwords32 - words) * 4, s));
-    CZ_HIP(launch_sieve_expand(root, p, pnum.p, pmeta.p, alive.p, out.bits.p, s));
+    CZ_HIP(launch_sieve_expand(root, p, pnum.p, pnum_hi.p, pmeta.p, alive.p, out.bits.p, s));
     bits32 = out.bits.p;
   } else {
     nwords32 = alive_words64 * 2;
@@ -297,10 +308,11 @@ int build_leaves(const SieveRoot& root, hipStream_t s, LeafSet& out) {
   out.count = total;
   CZ_HIP(out.res.alloc(std::max<uint64_t>(1, total)));
   CZ_HIP(out.num.alloc(std::max<uint64_t>(1, total)));
+  CZ_HIP(out.num_hi.alloc(out.wide ? std::max<uint64_t>(1, total) : 1));
   CZ_HIP(out.meta.alloc(std::max<uint64_t>(1, total)));
   CZ_HIP(launch_compact_write(bits32, nwords32, offs.p, out.res.p, s));
-  CZ_HIP(launch_sieve_fill(root, p, pnum.p, pmeta.p, out.res.p, out.num.p, out.meta.p, total,
-                           s));
+  CZ_HIP(launch_sieve_fill(root, p, pnum.p, pnum_hi.p, pmeta.p, out.res.p, out.num.p,
+                           out.num_hi.p, out.meta.p, total, s));
   CZ_HIP(hipStreamSynchronize(s));
   return COLLATZ_OK;
 }
@@ -339,6 +351,7 @@ int collatz_sieve_build(const collatz_sieve_params* params, collatz_sieve** out)
   SieveRoot root;
   root.base = 3;
   root.number = 9;
+  root.number_hi = 0;
   root.sigfig = 2;
   root.threepower = 2;
   root.doubleeven = 1;

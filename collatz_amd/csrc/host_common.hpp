@@ -199,6 +199,8 @@ int launch_width(int limbs, u128 max_hi, uint32_t cutoff);
 // sieve (sieve_kernels.hip) behind collatz_sieve_build and reference mode.
 struct LeafSet {
   DevBuf<uint64_t> res, num;
+  DevBuf<uint64_t> num_hi;  // high words of num (a one-entry placeholder unless wide)
+  bool wide = false;        // depth above 40: numbers take 128 bits
   DevBuf<uint32_t> meta;
   DevBuf<uint32_t> bits;   // dense bit array over the extensions h (padded)
   uint64_t bits_bits = 0;

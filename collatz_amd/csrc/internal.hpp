@@ -17,7 +17,8 @@ constexpr int kPow3Max = 64;            // 3^f tables cover f < 64
 // collatz_cudav7.cu:497-505): n = base + h * 2^sigfig for the extension h.
 struct SieveRoot {
   uint64_t base;        // n mod 2^sigfig
-  uint64_t number;      // T^sigfig(base) + 1
+  uint64_t number;      // T^sigfig(base) + 1, low 64 bits
+  uint64_t number_hi;   // its high 64 bits (0 unless the walk is 128-bit wide)
   uint32_t sigfig;      // depth of the root
   uint32_t threepower;  // f at the root
   uint32_t doubleeven;  // odd-even-even state
@@ -168,6 +169,15 @@ struct SieveArrays {
   uint32_t la1_mask;
 };
 
+// Leaves of a reference-mode run on the device: base, number = T^(N-A)(base) + 1 (its high
+// words in num_hi when the walk was 128-bit wide, else null) and the meta word.
+struct RefLeaves {
+  const uint64_t* base;
+  const uint64_t* num;
+  const uint64_t* num_hi;
+  const uint32_t* meta;
+};
+
 // Look-ahead tables of reference mode on the device.
 struct RefTablesDev {
   const uint32_t* bv;      // num_bv vectors of words_per_bv words (2^B bits + mirrored tail)
@@ -183,18 +193,24 @@ struct RefTablesDev {
 };
 
 // ---- launchers (defined in the .hip files) ------------------------------------------
+// Walks below depth kWalk64MaxDepth keep numbers in 64 bits (T^40(n) + 1 < 2^64); deeper
+// ones (reference mode up to N - A = 62) in 128, with the high words in the _hi arrays.
+constexpr uint32_t kWalk64MaxDepth = 40;
+constexpr uint32_t kWalkMaxDepth = 62;
 hipError_t launch_sieve_prefix(const SieveRoot& root, uint32_t p, uint64_t* pnum,
-                               uint32_t* pmeta, uint64_t* alive_bits, hipStream_t s);
+                               uint64_t* pnum_hi, uint32_t* pmeta, uint64_t* alive_bits,
+                               hipStream_t s);
 hipError_t launch_sieve_expand(const SieveRoot& root, uint32_t p, const uint64_t* pnum,
-                               const uint32_t* pmeta, const uint64_t* alive_bits,
-                               uint32_t* bits, hipStream_t s);
+                               const uint64_t* pnum_hi, const uint32_t* pmeta,
+                               const uint64_t* alive_bits, uint32_t* bits, hipStream_t s);
 hipError_t launch_compact_count(const uint32_t* bits, uint64_t nwords, uint32_t* block_counts,
                                 uint64_t nblocks, hipStream_t s);
 hipError_t launch_scan(const uint32_t* in, uint64_t n, uint64_t* out_excl, hipStream_t s);
 hipError_t launch_compact_write(const uint32_t* bits, uint64_t nwords,
                                 const uint64_t* block_offsets, uint64_t* out_h, hipStream_t s);
 hipError_t launch_sieve_fill(const SieveRoot& root, uint32_t p, const uint64_t* pnum,
-                             const uint32_t* pmeta, uint64_t* res_inout, uint64_t* num,
+                             const uint64_t* pnum_hi, const uint32_t* pmeta,
+                             uint64_t* res_inout, uint64_t* num, uint64_t* num_hi,
                              uint32_t* meta, uint64_t count, hipStream_t s);
 hipError_t launch_lower_bound(const uint4* sorted, const uint64_t* class_start,
                               const uint64_t* keys, const uint32_t* kcls, uint64_t nkeys,
@@ -236,16 +252,17 @@ hipError_t launch_la_build(uint32_t B, uint32_t k, uint32_t sem, uint32_t* table
                            hipStream_t s);
 hipError_t launch_leaf_bins(const uint32_t* meta, uint64_t n, uint32_t tpo,
                             unsigned long long* bins, hipStream_t s);
-hipError_t launch_ref_fused(int limbs, const RefTablesDev& T, const uint64_t* base,
-                            const uint64_t* num, const uint32_t* meta, uint64_t nleaves,
-                            uint32_t mbi, const Limits& lim, const uint64_t* pow3, uint32_t tile,
-                            uint32_t U, uint32_t nblocks, void* tot, uint64_t* ovf,
-                            uint64_t ovf_cap, uint64_t* esc, uint64_t esc_cap, hipStream_t s);
-hipError_t launch_ref_iterate(int limbs, const uint64_t* cands, uint64_t ncand,
-                              const uint64_t* base, const uint64_t* num, const uint32_t* meta,
-                              uint32_t mbi, const Limits& lim, const uint64_t* pow3, uint32_t tile,
-                              uint32_t nblocks, void* tot, uint64_t* ovf, uint64_t ovf_cap,
-                              hipStream_t s);
+// Reference mode: Steps 2 and 3 (rust: Step 3 is main.rs's search3), and the wide rerun of
+// escalated candidates.  pow3w holds 3^f for f < 64 as (low, high) word pairs.
+hipError_t launch_ref_fused(int limbs, bool rust, const RefTablesDev& T, const RefLeaves& lv,
+                            uint64_t nleaves, uint32_t mbi, const Limits& lim,
+                            const uint64_t* pow3w, uint32_t tile, uint32_t U, uint32_t nblocks,
+                            void* tot, uint64_t* ovf, uint64_t ovf_cap, uint64_t* esc,
+                            uint64_t esc_cap, hipStream_t s);
+hipError_t launch_ref_iterate(int limbs, bool rust, const uint64_t* cands, uint64_t ncand,
+                              const RefLeaves& lv, uint32_t mbi, const Limits& lim,
+                              const uint64_t* pow3w, uint32_t tile, uint32_t nblocks, void* tot,
+                              uint64_t* ovf, uint64_t ovf_cap, hipStream_t s);
 hipError_t launch_reduce_chunks(const DevRec* parts, const uint64_t* part_start,
                                 uint32_t nchunks, DevRec* out, hipStream_t s);
 hipError_t launch_escalated(const SieveArrays& sv, const EscEntry* esc, uint64_t n,

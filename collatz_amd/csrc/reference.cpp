@@ -32,11 +32,12 @@ struct RefTotalsHost {
   unsigned long long esc_count;
   unsigned long long checks;
   unsigned long long aborts;
+  unsigned long long numcases;
 };
 
 struct RefTables {
   DevBuf<uint32_t> bv, mod9, magic;
-  DevBuf<uint64_t> pow3;
+  DevBuf<uint64_t> pow3w;  // 3^f for f < 64 as (low, high) word pairs
   RefTablesDev dev;
 };
 
@@ -46,12 +47,22 @@ static std::map<std::tuple<int, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t
     g_ref_cache;
 
 // Tables alone can be built for deeper runs (N - A up to 120, e.g. main.rs's N = 72); a run
-// needs N - A <= 40 (leaves with 64-bit base and number: T^40(n) + 1 < 3^40 + 1 < 2^64).
-static int check_params(const collatz_ref_params* p, uint32_t max_mbi = 40) {
+// needs N - A <= 62 and N <= 64 (every start n = base + a 2^(N - A) in one u64; leaf numbers
+// take 128 bits beyond depth 40).
+static int check_params(const collatz_ref_params* p, bool run) {
   if (!p) return fail(COLLATZ_EINVAL, "null params");
   if (p->A < 1 || p->A > 12) return fail(COLLATZ_EINVAL, "A (LASTBITS) must be in [1, 12]");
-  if (p->N < p->A + 4 || p->N - p->A > max_mbi)
-    return fail(COLLATZ_EINVAL, "N - A out of range (4..40 for a run, 4..120 for tables)");
+  const uint32_t max_mbi = run ? kWalkMaxDepth : 120;
+  if (p->N < p->A + 4 || p->N - p->A > max_mbi || (run && p->N > 64))
+    return fail(COLLATZ_EINVAL,
+                "N - A out of range (4..62 with N <= 64 for a run, 4..120 for tables)");
+  // main.rs's blocks multiply by up to 2^9.4 between tests, which six words hold below 2^182;
+  // its test is a plain compare after each block, so it also takes cutoffs below 2^64.
+  const bool rust = p->semantics == COLLATZ_SEM_RUST;
+  if (rust ? (p->cutoff_log2 < 32 || p->cutoff_log2 > 180)
+           : (p->cutoff_log2 < 64 || p->cutoff_log2 > 187))
+    return fail(COLLATZ_EINVAL, rust ? "cutoff_log2 must be in [32, 180] under main.rs semantics"
+                                     : "cutoff_log2 must be in [64, 187]");
   if (p->B < 8 || p->B > 28) return fail(COLLATZ_EINVAL, "B must be in [8, 28]");
   // A leaf's 2^A-bit candidate window is read from its bit vector with the vector's first
   // words mirrored after its end (collatz_cudav7.cu:475-477), which wraps a window correctly
@@ -59,8 +70,6 @@ static int check_params(const collatz_ref_params* p, uint32_t max_mbi = 40) {
   if (p->A > p->B) return fail(COLLATZ_EINVAL, "A (LASTBITS) must not exceed B");
   if (p->num_bv < 1 || p->num_bv > 32) return fail(COLLATZ_EINVAL, "num_bv must be in [1, 32]");
   if (p->semantics > 2) return fail(COLLATZ_EINVAL, "unknown semantics");
-  if (p->cutoff_log2 < 64 || p->cutoff_log2 > 187)
-    return fail(COLLATZ_EINVAL, "cutoff_log2 must be in [64, 187]");
   return COLLATZ_OK;
 }
 
@@ -98,20 +107,21 @@ static int get_tables(const collatz_ref_params* p, int device, RefTables*& out) 
       if (red >= 9) red -= 9;
     }
   }
-  uint64_t p3[kPow3Max] = {0};
-  uint64_t x = 1;
-  for (int i = 0; i <= 40; ++i) {
-    p3[i] = x;
+  uint64_t p3w[2 * kPow3Max];
+  u128 x = 1;
+  for (int i = 0; i < kPow3Max; ++i) {
+    p3w[2 * i] = lo64(x);
+    p3w[2 * i + 1] = hi64(x);
     x *= 3;
   }
   CZ_HIP(t->bv.alloc(p->num_bv * wpb));
   CZ_HIP(t->mod9.alloc(mod9.size()));
   CZ_HIP(t->magic.alloc(p->num_bv));
-  CZ_HIP(t->pow3.alloc(kPow3Max));
+  CZ_HIP(t->pow3w.alloc(2 * kPow3Max));
   CZ_HIP(hipMemset(t->bv.p, 0, t->bv.bytes()));
   CZ_HIP(hipMemcpy(t->mod9.p, mod9.data(), mod9.size() * 4, hipMemcpyHostToDevice));
   CZ_HIP(hipMemcpy(t->magic.p, magic.data(), magic.size() * 4, hipMemcpyHostToDevice));
-  CZ_HIP(hipMemcpy(t->pow3.p, p3, sizeof(p3), hipMemcpyHostToDevice));
+  CZ_HIP(hipMemcpy(t->pow3w.p, p3w, sizeof(p3w), hipMemcpyHostToDevice));
   CZ_HIP(launch_bv_build(p->B, p->num_bv, first, p->semantics, t->magic.p, t->bv.p, wpb, nw, nullptr));
   CZ_HIP(hipDeviceSynchronize());
   t->dev.bv = t->bv.p;
@@ -133,14 +143,15 @@ static int get_tables(const collatz_ref_params* p, int device, RefTables*& out) 
 // host (search1's prunes, collatz_cudav7.cu:506-561).
 static void frontier(const SieveRoot& root, uint32_t d, std::vector<SieveRoot>& out) {
   struct St {
-    uint64_t base, number, p3;
+    uint64_t base;
+    u128 number, p3;
     uint32_t s, f, de, ne;
   };
   std::vector<St> stack;
-  uint64_t p3 = 1;
+  u128 p3 = 1;
   for (uint32_t i = 0; i < root.threepower; ++i) p3 *= 3;
-  stack.push_back({root.base, root.number, p3, root.sigfig, root.threepower, root.doubleeven,
-                   root.numevens});
+  stack.push_back({root.base, mk128(root.number, root.number_hi), p3, root.sigfig,
+                   root.threepower, root.doubleeven, root.numevens});
   const uint32_t target = root.sigfig + d;
   while (!stack.empty()) {
     St x = stack.back();
@@ -148,7 +159,8 @@ static void frontier(const SieveRoot& root, uint32_t d, std::vector<SieveRoot>& 
     if (x.s == target) {
       SieveRoot r = root;
       r.base = x.base;
-      r.number = x.number;
+      r.number = lo64(x.number);
+      r.number_hi = hi64(x.number);
       r.sigfig = x.s;
       r.threepower = x.f;
       r.doubleeven = x.de;
@@ -158,7 +170,7 @@ static void frontier(const SieveRoot& root, uint32_t d, std::vector<SieveRoot>& 
     }
     for (uint32_t bit = 0; bit < 2; ++bit) {
       St y = x;
-      const uint64_t nb = x.number + (bit ? x.p3 : 0);
+      const u128 nb = x.number + (bit ? x.p3 : 0);
       if (bit) y.base |= 1ull << x.s;
       const uint32_t inplay = kNumerator * (x.s + 1) >= kDenominator * x.f;
       if ((nb & 1) == 0) {
@@ -188,7 +200,7 @@ using namespace cz;
 
 extern "C" int collatz_reference_tables(const collatz_ref_params* params, uint32_t* bv,
                                         uint32_t* mod9) {
-  int rc = check_params(params, 120);
+  int rc = check_params(params, false);
   if (rc) return rc;
   HipDeviceGuard guard(params->device);
   int dev = 0;
@@ -207,12 +219,14 @@ extern "C" int collatz_reference_tables(const collatz_ref_params* params, uint32
 
 extern "C" int collatz_reference_run(const collatz_ref_params* params, const collatz_node* root,
                                      void* stream, collatz_ref_result* out) {
-  int rc = check_params(params);
+  int rc = check_params(params, true);
   if (rc) return rc;
   if (!root || !out) return fail(COLLATZ_EINVAL, "null argument");
   const uint32_t mbi = params->N - params->A;
-  if (root->base[1] || root->number[1] || root->sigfig > mbi || root->threepower > 40)
-    return fail(COLLATZ_EINVAL, "case line out of range (base and number must fit 64 bits)");
+  // The root's base is below 2^sigfig and its number below 3^threepower + 1.
+  if (root->base[1] || root->sigfig > mbi || root->threepower >= kPow3Max ||
+      (root->sigfig <= kWalk64MaxDepth && root->number[1]))
+    return fail(COLLATZ_EINVAL, "case line out of range");
   HipDeviceGuard guard(params->device);
   hipStream_t s = (hipStream_t)stream;
   int dev = 0;
@@ -225,6 +239,7 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
   SieveRoot r;
   r.base = root->base[0];
   r.number = root->number[0];
+  r.number_hi = root->number[1];
   r.sigfig = root->sigfig;
   r.threepower = root->threepower;
   r.doubleeven = root->doubleeven;
@@ -236,7 +251,12 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
   if (e > 30) frontier(r, e - 30, roots);
   else roots.push_back(r);
 
-  const int limbs = params->cutoff_log2 <= kWideLog2 ? 2 : 3;
+  // Word widths: the fused pass runs on 96 bits where it can and the candidates that leave
+  // them are rerun on two limbs or three.  The reference schedule tests its cutoff after
+  // each odd run (which multiplies by less than 2^4.7), main.rs's after each 16-step block
+  // (less than 2^9.4), so two limbs hold cutoffs up to 2^123 and 2^118 respectively.
+  const bool rust = params->semantics == COLLATZ_SEM_RUST;
+  const int limbs = params->cutoff_log2 <= (rust ? 118u : kWideLog2) ? 2 : 3;
   const Limits lim = make_limits(limbs, params->cutoff_log2);
   DevBuf<unsigned long long> dbins;
   DevBuf<RefTotalsHost> dtot;
@@ -268,9 +288,14 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
       // candidate counts, so small tiles only shorten the launch's tail) and five checks
       // between refills (candidates that pass the look-ahead filters run long); both
       // measured best at N = 48.
-      CZ_HIP(launch_ref_fused(width, T->dev, ls.res.p, ls.num.p, ls.meta.p, ls.count, mbi, lim,
-                              T->pow3.p, 256, 5, nblocks, dtot.p,
-                              ovf_cap ? dovf.p : nullptr, ovf_cap, desc.p, esc_cap, s));
+      RefLeaves lv;
+      lv.base = ls.res.p;
+      lv.num = ls.num.p;
+      lv.num_hi = ls.wide ? ls.num_hi.p : nullptr;
+      lv.meta = ls.meta.p;
+      CZ_HIP(launch_ref_fused(width, rust, T->dev, lv, ls.count, mbi, lim, T->pow3w.p, 256, 5,
+                              nblocks, dtot.p, ovf_cap ? dovf.p : nullptr, ovf_cap, desc.p,
+                              esc_cap, s));
       unsigned long long nesc = 0;
       CZ_HIP(hipMemcpyAsync(&nesc, &dtot.p->esc_count, 8, hipMemcpyDeviceToHost, s));
       CZ_HIP(hipStreamSynchronize(s));
@@ -278,15 +303,16 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
       if (nesc) {
         // Candidates that left 96 bits: rerun from their start on two or three limbs.
         CZ_HIP(hipMemsetAsync(&dtot.p->tile_counter, 0, 4, s));
-        CZ_HIP(launch_ref_iterate(limbs, desc.p, nesc, ls.res.p, ls.num.p, ls.meta.p, mbi, lim,
-                                  T->pow3.p, 4096, nblocks, dtot.p, ovf_cap ? dovf.p : nullptr,
-                                  ovf_cap, s));
+        CZ_HIP(launch_ref_iterate(limbs, rust, desc.p, nesc, lv, mbi, lim, T->pow3w.p, 4096,
+                                  nblocks, dtot.p, ovf_cap ? dovf.p : nullptr, ovf_cap, s));
         CZ_HIP(hipStreamSynchronize(s));
       }
     }
     return COLLATZ_OK;
   };
-  const bool narrow = params->cutoff_log2 >= 96;
+  // 96 bits for the reference schedule needs a cutoff of at least 2^96 (it has no overflow
+  // test there); main.rs's blocks test a cutoff below 2^86 on 96 bits and escalate above.
+  const bool narrow = rust || params->cutoff_log2 >= 96;
   rc = run_all(narrow ? 1 : limbs);
   if (rc == COLLATZ_ECAPACITY && narrow) rc = run_all(limbs);
   if (rc) return rc;
@@ -300,6 +326,7 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
                                      std::to_string(tot.aborts) + " tiles abandoned)");
   if (tot.stall) return fail(COLLATZ_EDEVICE, "a trajectory did not descend within the check bound");
   out->complete_hash = tot.hash;
+  out->numcases = tot.numcases;
   out->leaves = leaves;
   out->candidates = tot.count;
   out->checks = tot.checks;

@@ -152,6 +152,7 @@ struct RefTotals {
   unsigned long long esc_count;  // candidates handed from 96 bits to the wide pass
   unsigned long long checks;     // checks run (counted per wave from its active-lane mask)
   unsigned long long aborts;     // tiles abandoned at the work-loop bound (Limits::max_loop)
+  unsigned long long numcases;   // main.rs semantics: 16-step blocks of finished candidates
 };
 
 // num >= 2^c, or (strict) num > 2^c.
@@ -166,15 +167,79 @@ __device__ __forceinline__ bool reaches(const Num<L>& v, uint32_t c, bool strict
   return !exact;
 }
 
-template <int L, bool LOWCUT>
+// main.rs's search3 (main.rs:321-341) takes 16 steps of the shifted map per table lookup
+// and tests the value after each such block.  Here a block is two 8-step steps from a
+// 256-entry table in LDS: for r = number mod 2^8, entry r holds 3^t in bits 0-12 and
+// S^8(r) above (t the odd steps among the 8, S the shifted map), and
+// S^8(2^8 h + r) = 3^t h + S^8(r) gives number = (number >> 8) * 3^t + S^8(r).
+struct Step8Lds {
+  uint32_t e[256];
+};
+
+__device__ __forceinline__ void stage_step8(Step8Lds& t) {
+  for (uint32_t r = threadIdx.x; r < 256; r += blockDim.x) {
+    uint32_t n = r, m = 1;
+    for (int i = 0; i < 8; ++i) {
+      if ((n & 1u) == 0) {
+        n = n + (n >> 1);
+        m *= 3;
+      } else {
+        n = (n >> 1) + 1;
+      }
+    }
+    t.e[r] = m | (n << 13);
+  }
+}
+
+template <int L>
+__device__ __forceinline__ void step8(Num<L>& v, const Step8Lds& T) {
+  const uint32_t e = T.e[v.w[0] & 255u];
+  const uint32_t m = e & 0x1FFFu, add = e >> 13;
+  uint64_t p[L];
+#pragma unroll
+  for (int i = 0; i < L - 1; ++i) p[i] = (uint64_t)alignr(v.w[i + 1], v.w[i], 8) * m;
+  p[L - 1] = (uint64_t)(v.w[L - 1] >> 8) * m;
+  uint32_t c = 0;
+  v.w[0] = addc((uint32_t)p[0], add, 0u, c);
+#pragma unroll
+  for (int i = 1; i < L; ++i) v.w[i] = addc((uint32_t)p[i], (uint32_t)(p[i - 1] >> 32), c, c);
+}
+
+// Start value number + a * 3^f of candidate a of a leaf (Lemma 2.3; collatz_cudav7.cu:262-278,
+// main.rs:314), with 3^f from a table of (low, high) words: up to 2^111 at N - A = 62.
+__device__ __forceinline__ unsigned __int128 ref_start(uint64_t num_lo, uint64_t num_hi,
+                                                       uint64_t a, const uint64_t* p3w,
+                                                       uint32_t f) {
+  const unsigned __int128 p3 =
+      (unsigned __int128)p3w[2 * f] | ((unsigned __int128)p3w[2 * f + 1] << 64);
+  return (unsigned __int128)a * p3 + ((unsigned __int128)num_hi << 64 | num_lo);
+}
+
+template <int L>
+__device__ __forceinline__ void set_words(Num<L>& v, unsigned __int128 x) {
+  v.w[0] = (uint32_t)x;
+  v.w[1] = (uint32_t)(x >> 32);
+  v.w[2] = (uint32_t)(x >> 64);
+  if constexpr (L > 3) v.w[3] = (uint32_t)(x >> 96);
+#pragma unroll
+  for (int i = 4; i < L; ++i) v.w[i] = 0;
+}
+
+// Step 3 over a list of (leaf, a) candidates on L words: the two- or three-limb rerun of the
+// candidates that left 96 bits in k_ref_fused.  RUST selects main.rs's blocks (a check is
+// one 16-step block; a value at or above 2^cutoff after a block stops the candidate and
+// reports its start) instead of the reference schedule (a check is two odd and two even
+// runs; a value at 2^cutoff is reported and replaced by 2).
+template <int L, bool LOWCUT, bool RUST>
 __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint64_t ncand,
-                                                     const uint64_t* base, const uint64_t* num,
-                                                     const uint32_t* meta, uint32_t mbi,
-                                                     Limits lim, const uint64_t* pow3,
-                                                     uint32_t tile, RefTotals* tot,
-                                                     uint64_t* ovf, uint64_t ovf_cap) {
+                                                     RefLeaves lv, uint32_t mbi, Limits lim,
+                                                     const uint64_t* pow3w, uint32_t tile,
+                                                     RefTotals* tot, uint64_t* ovf,
+                                                     uint64_t ovf_cap) {
   __shared__ Pow3Lds p3s;
+  __shared__ Step8Lds st8;
   stage_pow3(p3s);
+  if constexpr (RUST) stage_step8(st8);
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t ntiles = (ncand + tile - 1) / tile;
@@ -187,10 +252,10 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
     if (t >= ntiles) break;
     uint64_t cur = (uint64_t)t * tile;
     const uint64_t end = cur + tile < ncand ? cur + tile : ncand;
-    uint32_t h = 0, cnt = 0, ov = 0, stall = 0;
+    uint32_t h = 0, cnt = 0, ov = 0, stall = 0, nc = 0;
     bool act = false, strict = false;
     Num<L> v;
-    uint32_t n[4];
+    uint64_t n_lo = 0;
     uint32_t checks = 0, wave_chk = 0;
     uint32_t iters = 0;
     bool aborted = false;
@@ -211,19 +276,9 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
           const uint32_t leaf = (uint32_t)c;
           const uint64_t a = (c >> 32) & 0xFFFFu;
           strict = (c >> 48) & 1u;
-          const uint64_t nn = base[leaf] + (a << mbi);
-          const uint32_t f = meta_f(meta[leaf]);
-          // number + a * 3^f (Lemma 2.3; collatz_cudav7.cu:262-278)
-          const unsigned __int128 x = (unsigned __int128)a * pow3[f] + num[leaf];
-          v.w[0] = (uint32_t)x;
-          v.w[1] = (uint32_t)(x >> 32);
-          v.w[2] = (uint32_t)(x >> 64);
-          v.w[3] = (uint32_t)(x >> 96);
-#pragma unroll
-          for (int i = 4; i < L; ++i) v.w[i] = 0;
-          n[0] = (uint32_t)nn;
-          n[1] = (uint32_t)(nn >> 32);
-          n[2] = n[3] = 0;
+          n_lo = lv.base[leaf] + (a << mbi);
+          set_words<L>(v, ref_start(lv.num[leaf], lv.num_hi ? lv.num_hi[leaf] : 0, a, pow3w,
+                                    meta_f(lv.meta[leaf])));
           checks = 0;
           act = true;
           cnt += 1;
@@ -235,27 +290,48 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
       wave_chk += (uint32_t)__popcll(running);
       if (act) {
         bool done = false;
-#pragma unroll
-        for (int half = 0; half < 2; ++half) {
-          odd_run<L>(v, p3s);
-          if (over_fast<L, LOWCUT>(v, thr) && reaches<L>(v, cutoff, strict && half == 1)) {
+        if constexpr (RUST) {
+          step8<L>(v, st8);
+          step8<L>(v, st8);
+          ++checks;
+          done = lt_base<L>(v, n_lo, 0);
+          if (!done && ge_pow2<L>(v, cutoff)) {
+            // main.rs:336-338: the start is printed and the candidate stops.
             ov += 1;
             if (ovf) {
               const unsigned long long i = atomicAdd(&tot->overflow, 1ull);
               if (i < ovf_cap) {
-                ovf[2 * i] = pair(n[0], n[1]);
-                ovf[2 * i + 1] = pair(n[2], n[3]);
+                ovf[2 * i] = n_lo;
+                ovf[2 * i + 1] = 0;
               }
             }
-            v.w[0] = 2;
-#pragma unroll
-            for (int i = 1; i < L; ++i) v.w[i] = 0;
+            done = true;
           }
-          even_run<L>(v);
+          if (done) nc += checks;
+        } else {
+#pragma unroll
+          for (int half = 0; half < 2; ++half) {
+            odd_run<L>(v, p3s);
+            if (over_fast<L, LOWCUT>(v, thr) && reaches<L>(v, cutoff, strict && half == 1)) {
+              ov += 1;
+              if (ovf) {
+                const unsigned long long i = atomicAdd(&tot->overflow, 1ull);
+                if (i < ovf_cap) {
+                  ovf[2 * i] = n_lo;
+                  ovf[2 * i + 1] = 0;
+                }
+              }
+              v.w[0] = 2;
+#pragma unroll
+              for (int i = 1; i < L; ++i) v.w[i] = 0;
+            }
+            even_run<L>(v);
+          }
+          h ^= v.w[0];
+          done = lt_base<L>(v, n_lo, 0);
+          ++checks;
         }
-        h ^= v.w[0];
-        done = lt_base<L>(v, pair(n[0], n[1]), pair(n[2], n[3]));
-        if (++checks >= lim.max_checks && !done) {
+        if (checks >= lim.max_checks && !done) {
           stall += 1;
           done = true;
         }
@@ -268,12 +344,14 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
       h ^= __shfl_xor(h, m, 64);
       cnt += __shfl_xor(cnt, m, 64);
       stall += __shfl_xor(stall, m, 64);
+      nc += __shfl_xor(nc, m, 64);
       if (!ovf) ov += __shfl_xor(ov, m, 64);
     }
     if (lane == 0) {
       atomicXor(&tot->hash, h);
       atomicAdd(&tot->count, (unsigned long long)cnt);
       atomicAdd(&tot->checks, (unsigned long long)wave_chk);
+      if (RUST) atomicAdd(&tot->numcases, (unsigned long long)nc);
       if (stall) atomicAdd(&tot->stall, (unsigned long long)stall);
       if (!ovf && ov) atomicAdd(&tot->overflow, (unsigned long long)ov);
       if (aborted) atomicAdd(&tot->aborts, 1ull);
@@ -289,29 +367,31 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
 // number and f one refill ahead.  A candidate starts as number + a * 3^f with
 // n = base + a * 2^MBI (Lemma 2.3) and runs up to U checks between refills; a lane whose
 // candidate ends inside the U checks starts its prefetched one at once.  No candidate list
-// is written to memory.  Hash, count and overflow count are folded with one atomic per
-// wave and tile.
+// is written to memory.  Hash (reference schedule) or block tally (main.rs's, RUST), count
+// and overflow count are folded with one atomic per wave and tile.
 //
-// L = 3 (96 bits, cutoff at or above 2^96) escalates instead of testing for overflow: a
-// check multiplies by less than 2^9.4, so a candidate that starts every check below 2^86
-// never reaches 2^96 (nor the cutoff) inside one; a candidate that ends a check above 2^86
-// is listed as (leaf, a, CPU-leaf flag), its partial hash dropped, and rerun from its start
-// by k_ref_iterate on two or three limbs.
+// L = 3 (96 bits) escalates instead of testing every value: a check multiplies by less than
+// 2^9.4, so a candidate that starts every check below 2^86 never leaves 96 bits inside one;
+// a start or a check end at or above 2^86 is listed as (leaf, a, CPU-leaf flag), its partial
+// results dropped, and rerun from its start by k_ref_iterate on two or three limbs.  The
+// reference schedule runs on 96 bits only with a cutoff at or above 2^96 (no overflow test
+// below it); main.rs's blocks test their cutoff on 96 bits when it is below 2^86.
 constexpr uint32_t kRefQueue = 512;  // queue entries per wave: (leaf, a)
 
-template <int L, bool LOWCUT>
-__global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_t* base,
-                                                   const uint64_t* num, const uint32_t* meta,
+template <int L, bool LOWCUT, bool RUST>
+__global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, RefLeaves lv,
                                                    uint64_t nleaves, uint32_t mbi, Limits lim,
-                                                   const uint64_t* pow3, uint32_t tile,
+                                                   const uint64_t* pow3w, uint32_t tile,
                                                    uint32_t U, RefTotals* tot, uint64_t* ovf,
                                                    uint64_t ovf_cap, uint64_t* esc,
                                                    uint64_t esc_cap) {
   __shared__ Pow3Lds p3s;
-  __shared__ uint64_t p3f[kPow3Max];
+  __shared__ Step8Lds st8;
+  __shared__ uint64_t p3w[2 * kPow3Max];
   __shared__ uint2 queues[4][kRefQueue];
   stage_pow3(p3s);
-  if (threadIdx.x < kPow3Max) p3f[threadIdx.x] = pow3[threadIdx.x];
+  if constexpr (RUST) stage_step8(st8);
+  if (threadIdx.x < 2 * kPow3Max) p3w[threadIdx.x] = pow3w[threadIdx.x];
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63;
   uint2* Q = queues[threadIdx.x >> 6];
@@ -319,6 +399,8 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
   const uint32_t cutoff = lim.cutoff;
   const uint64_t thr = lim.fast_thr;
   const uint32_t nw_log2 = __builtin_ctz(T.nw);
+  // main.rs's cutoff is tested on 96 bits only below the escalation bound.
+  const bool rust_cut96 = RUST && L == 3 && cutoff < kNarrowCheckLog2;
   for (;;) {
     uint32_t t = 0;
     if (lane == 0) t = atomicAdd(&tot->tile_counter, 1u);
@@ -329,28 +411,30 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
     // Mask words of the tile, expanded 64 at a time; queue entries [qhead, qtail).
     const uint32_t words = (uint32_t)(end - first) << nw_log2;
     uint32_t wcur = 0, qhead = 0, qtail = 0;
-    uint32_t h = 0, cnt = 0, ov = 0, stall = 0;
+    uint32_t h = 0, cnt = 0, ov = 0, stall = 0, nc = 0;
     uint32_t lane_chk = 0;  // checks of the lane's finished candidates
-    // Prefetched candidate of the lane: its leaf (index, base, number, meta) and a.
+    // Prefetched candidate of the lane: its leaf (index, base, number, meta) and a, loaded
+    // one refill ahead.
     bool pf = false;
     uint32_t pf_leaf = 0, pf_a = 0, pf_meta = 0;
-    uint64_t pf_lb = 0, pf_ln = 0;
+    uint64_t pf_lb = 0, pf_ln = 0, pf_lnh = 0;
     // Running candidate of the lane.
     bool act = false, strict = false;
     Num<L> v;
     uint64_t n_lo = 0;
     uint32_t checks = 0, ch = 0, cur_a = 0, cur_leaf = 0;
+    auto escalate = [&]() {
+      const unsigned long long i = atomicAdd(&tot->esc_count, 1ull);
+      if (i < esc_cap)
+        esc[i] = cur_leaf | ((uint64_t)cur_a << 32) | (strict ? (1ull << 48) : 0ull);
+      cnt -= 1;
+      act = false;
+    };
     auto start_prefetched = [&]() {
       pf = false;
       const uint32_t f = meta_f(pf_meta);
-      // number + a * 3^f (Lemma 2.3; collatz_cudav7.cu:262-278)
-      const unsigned __int128 x = (unsigned __int128)pf_a * p3f[f] + pf_ln;
-      v.w[0] = (uint32_t)x;
-      v.w[1] = (uint32_t)(x >> 32);
-      v.w[2] = (uint32_t)(x >> 64);
-      if constexpr (L > 3) v.w[3] = (uint32_t)(x >> 96);
-#pragma unroll
-      for (int i = 4; i < L; ++i) v.w[i] = 0;
+      const unsigned __int128 x = ref_start(pf_ln, pf_lnh, pf_a, p3w, f);
+      set_words<L>(v, x);
       n_lo = pf_lb + ((uint64_t)pf_a << mbi);
       // Leaves without a bit vector run on the CPU in the reference (search2/search3,
       // collatz_cudav7.cu:517-519), whose second overflow test is '>' (:702).
@@ -361,16 +445,9 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
       ch = 0;
       act = true;
       cnt += 1;
+      // A start at or above the 96-bit check bound goes to the wide pass at once.
       if constexpr (L == 3) {
-        // A start at or above the 96-bit check bound goes to the wide pass at once
-        // (a * 3^f < 2^76 and number < 2^64, so the start fits 96 bits).
-        if (v.w[2] >= (1u << (kNarrowCheckLog2 - 64))) {
-          const unsigned long long i = atomicAdd(&tot->esc_count, 1ull);
-          if (i < esc_cap)
-            esc[i] = cur_leaf | ((uint64_t)cur_a << 32) | (strict ? (1ull << 48) : 0ull);
-          cnt -= 1;
-          act = false;
-        }
+        if ((x >> kNarrowCheckLog2) != 0) escalate();
       }
     };
     // Iterations of the tile's loop (and of the take loop per refill), uniform: a wave
@@ -404,9 +481,10 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
               const uint2 e = Q[qhead + rank];
               pf_leaf = e.x;
               pf_a = e.y;
-              pf_lb = base[e.x];
-              pf_ln = num[e.x];
-              pf_meta = meta[e.x];
+              pf_lb = lv.base[e.x];
+              pf_ln = lv.num[e.x];
+              pf_lnh = lv.num_hi ? lv.num_hi[e.x] : 0;
+              pf_meta = lv.meta[e.x];
               pf = true;
             } else {
               rank -= take;
@@ -421,7 +499,7 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
           if (wi < words) {
             leaf = (uint32_t)first + (wi >> nw_log2);
             w = wi & (T.nw - 1u);
-            m = leaf_mask_word(T, base[leaf], num[leaf], meta[leaf], w);
+            m = leaf_mask_word(T, lv.base[leaf], lv.num[leaf], lv.meta[leaf], w);
           }
           const uint32_t pc = __popc(m);
           // Inclusive prefix sum of the counts across the wave.
@@ -465,42 +543,59 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
       //     prefetched candidate at once.
       for (uint32_t u = 0; u < U; ++u) {
         if (act) {
-#pragma unroll
-          for (int half = 0; half < 2; ++half) {
-            odd_run<L>(v, p3s);
-            if constexpr (L > 3) {
-              if (over_fast<L, LOWCUT>(v, thr) && reaches<L>(v, cutoff, strict && half == 1)) {
-                ov += 1;
-                if (ovf) {
-                  const unsigned long long i = atomicAdd(&tot->overflow, 1ull);
-                  if (i < ovf_cap) {
-                    ovf[2 * i] = n_lo;
-                    ovf[2 * i + 1] = 0;
-                  }
+          bool done, stop = false;
+          if constexpr (RUST) {
+            // One block of main.rs's search3: 16 steps, then the two tests (main.rs:333-339).
+            step8<L>(v, st8);
+            step8<L>(v, st8);
+            done = lt_base<L>(v, n_lo, 0);
+            if (!done && (L > 3 || rust_cut96) && ge_pow2<L>(v, cutoff)) {
+              ov += 1;
+              if (ovf) {
+                const unsigned long long i = atomicAdd(&tot->overflow, 1ull);
+                if (i < ovf_cap) {
+                  ovf[2 * i] = n_lo;
+                  ovf[2 * i + 1] = 0;
                 }
-                v.w[0] = 2;
-#pragma unroll
-                for (int i = 1; i < L; ++i) v.w[i] = 0;
               }
+              stop = true;
             }
-            even_run<L>(v);
+          } else {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+              odd_run<L>(v, p3s);
+              if constexpr (L > 3) {
+                if (over_fast<L, LOWCUT>(v, thr) && reaches<L>(v, cutoff, strict && half == 1)) {
+                  ov += 1;
+                  if (ovf) {
+                    const unsigned long long i = atomicAdd(&tot->overflow, 1ull);
+                    if (i < ovf_cap) {
+                      ovf[2 * i] = n_lo;
+                      ovf[2 * i + 1] = 0;
+                    }
+                  }
+                  v.w[0] = 2;
+#pragma unroll
+                  for (int i = 1; i < L; ++i) v.w[i] = 0;
+                }
+              }
+              even_run<L>(v);
+            }
+            ch ^= v.w[0];
+            done = lt_base<L>(v, n_lo, 0);
           }
-          ch ^= v.w[0];
-          bool done = lt_base<L>(v, n_lo, 0);
           bool wide = false;
-          if constexpr (L == 3) wide = v.w[2] >= (1u << (kNarrowCheckLog2 - 64));
+          if constexpr (L == 3) wide = !done && !stop && v.w[2] >= (1u << (kNarrowCheckLog2 - 64));
           ++checks;
-          if (done || wide) {
+          if (done || stop || wide) {
             lane_chk += checks;
             if (wide) {
-              const unsigned long long i = atomicAdd(&tot->esc_count, 1ull);
-              if (i < esc_cap)
-                esc[i] = cur_leaf | ((uint64_t)cur_a << 32) | (strict ? (1ull << 48) : 0ull);
-              cnt -= 1;
+              escalate();
             } else {
-              h ^= ch;
+              if constexpr (RUST) nc += checks;
+              else h ^= ch;
+              act = false;
             }
-            act = false;
           }
         }
         if (u + 1 < U && !act && pf) start_prefetched();
@@ -513,12 +608,14 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, const uint64_
       cnt += __shfl_xor(cnt, k, 64);
       stall += __shfl_xor(stall, k, 64);
       lane_chk += __shfl_xor(lane_chk, k, 64);
+      nc += __shfl_xor(nc, k, 64);
       if (!ovf) ov += __shfl_xor(ov, k, 64);
     }
     if (lane == 0) {
       atomicXor(&tot->hash, h);
       atomicAdd(&tot->count, (unsigned long long)cnt);
       atomicAdd(&tot->checks, (unsigned long long)lane_chk);
+      if (RUST) atomicAdd(&tot->numcases, (unsigned long long)nc);
       if (stall) atomicAdd(&tot->stall, (unsigned long long)stall);
       if (!ovf && ov) atomicAdd(&tot->overflow, (unsigned long long)ov);
       if (aborted) atomicAdd(&tot->aborts, 1ull);
@@ -550,47 +647,71 @@ hipError_t launch_leaf_bins(const uint32_t* meta, uint64_t n, uint32_t tpo,
   return hipGetLastError();
 }
 
-// limbs = 2 requires cutoff <= 123 (no escalation in this mode); three limbs otherwise.
-hipError_t launch_ref_iterate(int limbs, const uint64_t* cands, uint64_t ncand,
-                              const uint64_t* base, const uint64_t* num, const uint32_t* meta,
-                              uint32_t mbi, const Limits& lim, const uint64_t* pow3, uint32_t tile,
-                              uint32_t nblocks, void* tot, uint64_t* ovf, uint64_t ovf_cap,
-                              hipStream_t s) {
-  if (!ncand) return hipSuccess;
+template <bool RUST>
+static void launch_iterate_sched(int limbs, const uint64_t* cands, uint64_t ncand,
+                                 const RefLeaves& lv, uint32_t mbi, const Limits& lim,
+                                 const uint64_t* pow3w, uint32_t tile, uint32_t nblocks,
+                                 RefTotals* tot, uint64_t* ovf, uint64_t ovf_cap, hipStream_t s) {
   if (limbs == 2)
-    hipLaunchKernelGGL((k_ref_iterate<4, false>), dim3(nblocks), dim3(256), 0, s, cands, ncand,
-                       base, num, meta, mbi, lim, pow3, tile, (RefTotals*)tot, ovf, ovf_cap);
+    hipLaunchKernelGGL((k_ref_iterate<4, false, RUST>), dim3(nblocks), dim3(256), 0, s, cands,
+                       ncand, lv, mbi, lim, pow3w, tile, tot, ovf, ovf_cap);
   else if (lim.cutoff >= 128)
-    hipLaunchKernelGGL((k_ref_iterate<6, false>), dim3(nblocks), dim3(256), 0, s, cands, ncand,
-                       base, num, meta, mbi, lim, pow3, tile, (RefTotals*)tot, ovf, ovf_cap);
+    hipLaunchKernelGGL((k_ref_iterate<6, false, RUST>), dim3(nblocks), dim3(256), 0, s, cands,
+                       ncand, lv, mbi, lim, pow3w, tile, tot, ovf, ovf_cap);
   else
-    hipLaunchKernelGGL((k_ref_iterate<6, true>), dim3(nblocks), dim3(256), 0, s, cands, ncand,
-                       base, num, meta, mbi, lim, pow3, tile, (RefTotals*)tot, ovf, ovf_cap);
+    hipLaunchKernelGGL((k_ref_iterate<6, true, RUST>), dim3(nblocks), dim3(256), 0, s, cands,
+                       ncand, lv, mbi, lim, pow3w, tile, tot, ovf, ovf_cap);
+}
+
+// limbs = 2 requires cutoff <= 123 for the reference schedule and cutoff <= 118 for main.rs's
+// blocks (a block multiplies by less than 2^9.4); three limbs otherwise.
+hipError_t launch_ref_iterate(int limbs, bool rust, const uint64_t* cands, uint64_t ncand,
+                              const RefLeaves& lv, uint32_t mbi, const Limits& lim,
+                              const uint64_t* pow3w, uint32_t tile, uint32_t nblocks, void* tot,
+                              uint64_t* ovf, uint64_t ovf_cap, hipStream_t s) {
+  if (!ncand) return hipSuccess;
+  if (rust)
+    launch_iterate_sched<true>(limbs, cands, ncand, lv, mbi, lim, pow3w, tile, nblocks,
+                               (RefTotals*)tot, ovf, ovf_cap, s);
+  else
+    launch_iterate_sched<false>(limbs, cands, ncand, lv, mbi, lim, pow3w, tile, nblocks,
+                                (RefTotals*)tot, ovf, ovf_cap, s);
   return hipGetLastError();
 }
 
-hipError_t launch_ref_fused(int limbs, const RefTablesDev& T, const uint64_t* base,
-                            const uint64_t* num, const uint32_t* meta, uint64_t nleaves,
-                            uint32_t mbi, const Limits& lim, const uint64_t* pow3, uint32_t tile,
-                            uint32_t U, uint32_t nblocks, void* tot, uint64_t* ovf,
-                            uint64_t ovf_cap, uint64_t* esc, uint64_t esc_cap, hipStream_t s) {
-  if (!nleaves) return hipSuccess;
+template <bool RUST>
+static void launch_fused_sched(int limbs, const RefTablesDev& T, const RefLeaves& lv,
+                               uint64_t nleaves, uint32_t mbi, const Limits& lim,
+                               const uint64_t* pow3w, uint32_t tile, uint32_t U,
+                               uint32_t nblocks, RefTotals* tot, uint64_t* ovf,
+                               uint64_t ovf_cap, uint64_t* esc, uint64_t esc_cap,
+                               hipStream_t s) {
   if (limbs == 1)
-    hipLaunchKernelGGL((k_ref_fused<3, false>), dim3(nblocks), dim3(256), 0, s, T, base, num,
-                       meta, nleaves, mbi, lim, pow3, tile, U, (RefTotals*)tot, ovf, ovf_cap,
-                       esc, esc_cap);
+    hipLaunchKernelGGL((k_ref_fused<3, false, RUST>), dim3(nblocks), dim3(256), 0, s, T, lv,
+                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, esc_cap);
   else if (limbs == 2)
-    hipLaunchKernelGGL((k_ref_fused<4, false>), dim3(nblocks), dim3(256), 0, s, T, base, num,
-                       meta, nleaves, mbi, lim, pow3, tile, U, (RefTotals*)tot, ovf, ovf_cap,
-                       esc, 0);
+    hipLaunchKernelGGL((k_ref_fused<4, false, RUST>), dim3(nblocks), dim3(256), 0, s, T, lv,
+                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, 0);
   else if (lim.cutoff >= 128)
-    hipLaunchKernelGGL((k_ref_fused<6, false>), dim3(nblocks), dim3(256), 0, s, T, base, num,
-                       meta, nleaves, mbi, lim, pow3, tile, U, (RefTotals*)tot, ovf, ovf_cap,
-                       esc, 0);
+    hipLaunchKernelGGL((k_ref_fused<6, false, RUST>), dim3(nblocks), dim3(256), 0, s, T, lv,
+                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, 0);
   else
-    hipLaunchKernelGGL((k_ref_fused<6, true>), dim3(nblocks), dim3(256), 0, s, T, base, num,
-                       meta, nleaves, mbi, lim, pow3, tile, U, (RefTotals*)tot, ovf, ovf_cap,
-                       esc, 0);
+    hipLaunchKernelGGL((k_ref_fused<6, true, RUST>), dim3(nblocks), dim3(256), 0, s, T, lv,
+                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, 0);
+}
+
+hipError_t launch_ref_fused(int limbs, bool rust, const RefTablesDev& T, const RefLeaves& lv,
+                            uint64_t nleaves, uint32_t mbi, const Limits& lim,
+                            const uint64_t* pow3w, uint32_t tile, uint32_t U, uint32_t nblocks,
+                            void* tot, uint64_t* ovf, uint64_t ovf_cap, uint64_t* esc,
+                            uint64_t esc_cap, hipStream_t s) {
+  if (!nleaves) return hipSuccess;
+  if (rust)
+    launch_fused_sched<true>(limbs, T, lv, nleaves, mbi, lim, pow3w, tile, U, nblocks,
+                             (RefTotals*)tot, ovf, ovf_cap, esc, esc_cap, s);
+  else
+    launch_fused_sched<false>(limbs, T, lv, nleaves, mbi, lim, pow3w, tile, U, nblocks,
+                              (RefTotals*)tot, ovf, ovf_cap, esc, esc_cap, s);
   return hipGetLastError();
 }
 

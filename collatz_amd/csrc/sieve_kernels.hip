@@ -19,17 +19,23 @@
 
 namespace cz {
 
+// The walk's number type: 64 bits for depths up to 40 (T^40(n) + 1 < 3^40 + 1 < 2^64, all
+// of range mode), 128 bits beyond (reference mode up to N - A = 62).
+using u128d = unsigned __int128;
+
+template <typename NT>
 struct Walk {
-  uint64_t base;    // n mod 2^s
-  uint64_t number;  // T^s(n) + 1
-  uint64_t p3;      // 3^f
+  uint64_t base;  // n mod 2^s
+  NT number;      // T^s(n) + 1
+  NT p3;          // 3^f
   uint32_t f, de, ne;
 };
 
 // Take bit `bit` at depth s and apply the prunes of the child (collatz_cudav7.cu:506-511,
 // :542-561; main.rs:216 when rust).  Returns false when the child is pruned or not taken.
-__device__ __forceinline__ bool walk_step(Walk& x, uint32_t s, uint32_t bit, uint32_t rust) {
-  const uint64_t nb = x.number + (bit ? x.p3 : 0ull);
+template <typename NT>
+__device__ __forceinline__ bool walk_step(Walk<NT>& x, uint32_t s, uint32_t bit, uint32_t rust) {
+  const NT nb = x.number + (bit ? x.p3 : (NT)0);
   if (bit) x.base |= 1ull << s;
   const uint32_t inplay = (kNumerator * (s + 1) >= kDenominator * x.f) ? 1u : 0u;
   if ((nb & 1) == 0) {
@@ -45,7 +51,7 @@ __device__ __forceinline__ bool walk_step(Walk& x, uint32_t s, uint32_t bit, uin
     x.ne += 1;
   }
   const uint32_t s1 = s + 1;
-  if (rust && x.number <= x.base) return false;
+  if (rust && x.number <= (NT)x.base) return false;
   if (kDenominator * x.f <= kNumerator * s1) return false;
   if (x.ne >= 2 && (x.ne & 1) == 0 && kNumerator * (s1 - 1) >= kDenominator * (x.f - 1))
     return false;
@@ -57,22 +63,46 @@ __device__ __forceinline__ uint32_t pack_meta(uint32_t f, uint32_t mod9, uint32_
   return f | (mod9 << 8) | (de << 12) | ((ne & 0xFFu) << 16) | (alive ? 0x80000000u : 0u);
 }
 
-__device__ __forceinline__ uint64_t pow3_u64(uint32_t f) {
-  uint64_t x = 1;
+template <typename NT>
+__device__ __forceinline__ NT pow3_of(uint32_t f) {
+  NT x = 1;
   for (uint32_t i = 0; i < f; ++i) x *= 3;
   return x;
 }
 
+// A walk's number as stored per prefix or leaf: the low word, and the high word in a second
+// array for the 128-bit walk (null for the 64-bit one).
+template <typename NT>
+__device__ __forceinline__ NT load_num(const uint64_t* lo, const uint64_t* hi, uint64_t i) {
+  if constexpr (sizeof(NT) > 8) return (NT)lo[i] | ((NT)hi[i] << 64);
+  else return lo[i];
+}
+
+template <typename NT>
+__device__ __forceinline__ void store_num(uint64_t* lo, uint64_t* hi, uint64_t i, NT v) {
+  lo[i] = (uint64_t)v;
+  if constexpr (sizeof(NT) > 8) hi[i] = (uint64_t)(v >> 64);
+}
+
+template <typename NT>
+__device__ __forceinline__ NT root_num(const SieveRoot& root) {
+  if constexpr (sizeof(NT) > 8) return (NT)root.number | ((NT)root.number_hi << 64);
+  else return root.number;
+}
+
+constexpr int kWalkPow3 = 64;  // 3^f for f < 64 (f <= depth <= 62)
+
+template <typename NT>
 __global__ __launch_bounds__(256) void k_sieve_prefix(SieveRoot root, uint32_t p,
-                                                      uint64_t* pnum, uint32_t* pmeta,
-                                                      uint64_t* alive_bits) {
+                                                      uint64_t* pnum, uint64_t* pnum_hi,
+                                                      uint32_t* pmeta, uint64_t* alive_bits) {
   const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t total = 1ull << p;
   bool alive = false;
-  Walk x;
+  Walk<NT> x;
   x.base = root.base;
-  x.number = root.number;
-  x.p3 = pow3_u64(root.threepower);
+  x.number = root_num<NT>(root);
+  x.p3 = pow3_of<NT>(root.threepower);
   x.f = root.threepower;
   x.de = root.doubleeven;
   x.ne = root.numevens;
@@ -80,7 +110,7 @@ __global__ __launch_bounds__(256) void k_sieve_prefix(SieveRoot root, uint32_t p
     alive = true;
     for (uint32_t j = 0; j < p && alive; ++j)
       alive = walk_step(x, root.sigfig + j, (uint32_t)(h >> j) & 1u, root.rust);
-    pnum[h] = alive ? x.number : 0;
+    store_num<NT>(pnum, pnum_hi, h, alive ? x.number : (NT)0);
     pmeta[h] = alive ? pack_meta(x.f, 0, x.de, x.ne, true) : 0u;
   }
   const uint64_t bal = __ballot(alive);
@@ -90,13 +120,15 @@ __global__ __launch_bounds__(256) void k_sieve_prefix(SieveRoot root, uint32_t p
   }
 }
 
+template <typename NT>
 __global__ __launch_bounds__(256) void k_sieve_expand(SieveRoot root, uint32_t p,
                                                       const uint64_t* pnum,
+                                                      const uint64_t* pnum_hi,
                                                       const uint32_t* pmeta,
                                                       const uint32_t* alive32, uint32_t* bits,
                                                       uint64_t nwords) {
-  __shared__ uint64_t p3[41];
-  if (threadIdx.x < 41) p3[threadIdx.x] = pow3_u64(threadIdx.x);
+  __shared__ NT p3[kWalkPow3];
+  if (threadIdx.x < kWalkPow3) p3[threadIdx.x] = pow3_of<NT>(threadIdx.x);
   __syncthreads();
   const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nwords) return;
@@ -111,12 +143,12 @@ __global__ __launch_bounds__(256) void k_sieve_expand(SieveRoot root, uint32_t p
     const uint32_t j = __builtin_ctz(mask);
     mask &= mask - 1;
     const uint32_t m = pmeta[hl0 + j];
-    Walk x;
+    Walk<NT> x;
     x.f = meta_f(m);
     x.de = meta_de(m);
     x.ne = (m >> 16) & 0xFFu;
     x.p3 = p3[x.f];
-    x.number = pnum[hl0 + j];
+    x.number = load_num<NT>(pnum, pnum_hi, hl0 + j);
     x.base = root.base + ((hl0 + j) << root.sigfig);
     bool alive = true;
     for (uint32_t t = 0; t < rest && alive; ++t)
@@ -218,13 +250,15 @@ __global__ __launch_bounds__(256) void k_compact_write(const uint4* bits, uint64
   }
 }
 
+template <typename NT>
 __global__ __launch_bounds__(256) void k_sieve_fill(SieveRoot root, uint32_t p,
                                                     const uint64_t* pnum,
+                                                    const uint64_t* pnum_hi,
                                                     const uint32_t* pmeta, uint64_t* res,
-                                                    uint64_t* num, uint32_t* meta,
-                                                    uint64_t count) {
-  __shared__ uint64_t p3[41];
-  if (threadIdx.x < 41) p3[threadIdx.x] = pow3_u64(threadIdx.x);
+                                                    uint64_t* num, uint64_t* num_hi,
+                                                    uint32_t* meta, uint64_t count) {
+  __shared__ NT p3[kWalkPow3];
+  if (threadIdx.x < kWalkPow3) p3[threadIdx.x] = pow3_of<NT>(threadIdx.x);
   __syncthreads();
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= count) return;
@@ -232,17 +266,17 @@ __global__ __launch_bounds__(256) void k_sieve_fill(SieveRoot root, uint32_t p,
   const uint64_t hl = h & ((1ull << p) - 1), hh = h >> p;
   const uint32_t m = pmeta[hl];
   const uint32_t P = root.sigfig + p;
-  Walk x;
+  Walk<NT> x;
   x.f = meta_f(m);
   x.de = meta_de(m);
   x.ne = (m >> 16) & 0xFFu;
   x.p3 = p3[x.f];
-  x.number = pnum[hl];
+  x.number = load_num<NT>(pnum, pnum_hi, hl);
   x.base = root.base + (hl << root.sigfig);
   for (uint32_t s = P; s < root.depth; ++s) walk_step(x, s, (uint32_t)(hh >> (s - P)) & 1u, root.rust);
   const uint64_t r = root.base + (h << root.sigfig);
   res[t] = r;
-  num[t] = x.number;
+  store_num<NT>(num, num_hi, t, x.number);
   meta[t] = pack_meta(x.f, (uint32_t)(r % 9), x.de, x.ne, false);
 }
 
@@ -317,21 +351,35 @@ __global__ __launch_bounds__(256) void k_lower_bound(const uint4* a,
 
 static inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
+// The walk runs on 64-bit numbers up to depth 40 and on 128-bit ones beyond (the _hi
+// arrays are used, and must be given, only then).
+static bool wide_walk(const SieveRoot& root) { return root.depth > kWalk64MaxDepth; }
+
 hipError_t launch_sieve_prefix(const SieveRoot& root, uint32_t p, uint64_t* pnum,
-                               uint32_t* pmeta, uint64_t* alive_bits, hipStream_t s) {
+                               uint64_t* pnum_hi, uint32_t* pmeta, uint64_t* alive_bits,
+                               hipStream_t s) {
   const uint64_t total = 1ull << p;
-  hipLaunchKernelGGL(k_sieve_prefix, dim3(blocks_for(total, 256)), dim3(256), 0, s, root, p,
-                     pnum, pmeta, alive_bits);
+  if (wide_walk(root))
+    hipLaunchKernelGGL(k_sieve_prefix<u128d>, dim3(blocks_for(total, 256)), dim3(256), 0, s,
+                       root, p, pnum, pnum_hi, pmeta, alive_bits);
+  else
+    hipLaunchKernelGGL(k_sieve_prefix<uint64_t>, dim3(blocks_for(total, 256)), dim3(256), 0, s,
+                       root, p, pnum, nullptr, pmeta, alive_bits);
   return hipGetLastError();
 }
 
 hipError_t launch_sieve_expand(const SieveRoot& root, uint32_t p, const uint64_t* pnum,
-                               const uint32_t* pmeta, const uint64_t* alive_bits,
-                               uint32_t* bits, hipStream_t s) {
+                               const uint64_t* pnum_hi, const uint32_t* pmeta,
+                               const uint64_t* alive_bits, uint32_t* bits, hipStream_t s) {
   const uint32_t e = root.depth - root.sigfig;
   const uint64_t nwords = 1ull << (e - 5);
-  hipLaunchKernelGGL(k_sieve_expand, dim3(blocks_for(nwords, 256)), dim3(256), 0, s, root, p,
-                     pnum, pmeta, (const uint32_t*)alive_bits, bits, nwords);
+  if (wide_walk(root))
+    hipLaunchKernelGGL(k_sieve_expand<u128d>, dim3(blocks_for(nwords, 256)), dim3(256), 0, s,
+                       root, p, pnum, pnum_hi, pmeta, (const uint32_t*)alive_bits, bits, nwords);
+  else
+    hipLaunchKernelGGL(k_sieve_expand<uint64_t>, dim3(blocks_for(nwords, 256)), dim3(256), 0,
+                       s, root, p, pnum, nullptr, pmeta, (const uint32_t*)alive_bits, bits,
+                       nwords);
   return hipGetLastError();
 }
 
@@ -357,11 +405,16 @@ hipError_t launch_compact_write(const uint32_t* bits, uint64_t nwords,
 }
 
 hipError_t launch_sieve_fill(const SieveRoot& root, uint32_t p, const uint64_t* pnum,
-                             const uint32_t* pmeta, uint64_t* res_inout, uint64_t* num,
+                             const uint64_t* pnum_hi, const uint32_t* pmeta,
+                             uint64_t* res_inout, uint64_t* num, uint64_t* num_hi,
                              uint32_t* meta, uint64_t count, hipStream_t s) {
   if (count == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sieve_fill, dim3(blocks_for(count, 256)), dim3(256), 0, s, root, p,
-                     pnum, pmeta, res_inout, num, meta, count);
+  if (wide_walk(root))
+    hipLaunchKernelGGL(k_sieve_fill<u128d>, dim3(blocks_for(count, 256)), dim3(256), 0, s, root,
+                       p, pnum, pnum_hi, pmeta, res_inout, num, num_hi, meta, count);
+  else
+    hipLaunchKernelGGL(k_sieve_fill<uint64_t>, dim3(blocks_for(count, 256)), dim3(256), 0, s,
+                       root, p, pnum, nullptr, pmeta, res_inout, num, nullptr, meta, count);
   return hipGetLastError();
 }
 

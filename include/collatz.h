@@ -18,7 +18,7 @@
 extern "C" {
 #endif
 
-#define COLLATZ_ABI_VERSION 1
+#define COLLATZ_ABI_VERSION 2
 
 enum collatz_status {
   COLLATZ_OK = 0,
@@ -185,16 +185,22 @@ typedef struct {
 } collatz_node;
 
 typedef struct {
-  uint32_t N;            /* MAXBITS (N - A <= 40) */
+  uint32_t N;            /* MAXBITS: N <= 64 and N - A <= 62 */
   uint32_t A;            /* LASTBITS, 1..12 */
   uint32_t B;            /* STEPSIZEINITIAL, 24; 8..28 and at least A */
   uint32_t num_bv;       /* bit vectors, 16 (CUDA/OpenCL) or 8 (Rust) */
   uint32_t semantics;    /* collatz_semantics */
-  uint32_t cutoff_log2;  /* CUTOFFPOWER, 64..187 */
+  uint32_t cutoff_log2;  /* CUTOFFPOWER, 64..187 (main.rs semantics: 64..180, main.rs uses
+                            120: the threshold of its (number >> 64) >= 2^56 test) */
   int32_t device;
   uint32_t reserved;
 } collatz_ref_params;
 
+/* Under COLLATZ_SEM_RUST Step 3 is main.rs's own search3 (main.rs:321-341): each candidate
+ * runs in blocks of 16 steps until it falls below its start, or until a block ends at or
+ * above 2^cutoff_log2, which reports its start (main.rs:336-338 prints it in decimal) and
+ * stops it.  numcases is the sum of the blocks (main.rs:97-98, "numcases = %d") and
+ * complete_hash is 0 (main.rs has no hash). */
 typedef struct {
   uint32_t complete_hash;     /* "Complete hash = %X" */
   uint32_t pad;
@@ -208,6 +214,7 @@ typedef struct {
   uint64_t overflow_cap;
   uint64_t overflow_written;
   double device_ms;
+  uint64_t numcases;          /* main.rs semantics: its "numcases"; 0 otherwise */
 } collatz_ref_result;
 
 int collatz_reference_run(const collatz_ref_params* params, const collatz_node* root,

@@ -58,7 +58,8 @@ class TrajOut(C.Structure):
 class CaseOut(C.Structure):
     _fields_ = [("complete_hash", C.c_uint32), ("pad", C.c_uint32), ("launches", C.c_uint64),
                 ("leaves", C.c_uint64), ("candidates", C.c_uint64),
-                ("bin_leaves", C.c_uint64 * 17), ("overflow_count", C.c_uint64)]
+                ("bin_leaves", C.c_uint64 * 17), ("overflow_count", C.c_uint64),
+                ("numcases", C.c_uint64)]
 
 
 class Chunk(C.Structure):
@@ -111,6 +112,12 @@ def lib():
         L.or_leaf_hash.argtypes = [_P(Knobs), _u64p, _u64p, _P(Node), _u64p]
         L.or_leaf_hash.restype = C.c_uint32
         L.or_case_run.argtypes = [_P(Knobs), _u64p, _u64p, _P(Node), _P(CaseOut)]
+        L.or_case_run_starts.argtypes = [_P(Knobs), _u64p, _u64p, _P(Node), _P(CaseOut),
+                                         _u64p, C.c_uint64]
+        L.or_case_run_mt.argtypes = [_P(Knobs), _u64p, _u64p, _P(Node), _P(CaseOut),
+                                     _u64p, C.c_uint64, C.c_int]
+        L.or_rust_search3.argtypes = [_u64p, _u64p, C.c_uint32, _u64p]
+        L.or_rust_search3.restype = C.c_int
         L.or_rust_count.argtypes = [_P(Knobs), _u64p, _u64p, _P(Node)]
         L.or_rust_count.restype = C.c_int64
         L.or_verify_range.argtypes = [C.c_uint32, C.c_int64, _u64p, _u64p, _P(C.c_uint8),
@@ -212,6 +219,26 @@ def case_run(kn, bv, mod9, root: Node) -> CaseOut:
     assert lib().or_case_run(C.byref(kn), _ptr(bv, C.c_uint64), _ptr(mod9, C.c_uint64),
                              C.byref(root), C.byref(out)) == 0
     return out
+
+
+def case_run_starts(kn, bv, mod9, root: Node, cap: int = 1 << 16, threads: int = 1):
+    """case_run plus the start n of up to `cap` overflow reports (on `threads` threads)."""
+    out = CaseOut()
+    buf = np.zeros(2 * max(cap, 1), np.uint64)
+    assert lib().or_case_run_mt(C.byref(kn), _ptr(bv, C.c_uint64), _ptr(mod9, C.c_uint64),
+                                C.byref(root), C.byref(out), _ptr(buf, C.c_uint64), cap,
+                                threads) == 0
+    k = min(out.overflow_count, cap)
+    return out, [int(buf[2 * i]) | (int(buf[2 * i + 1]) << 64) for i in range(k)]
+
+
+def rust_search3(base: int, num0: int, cutoff: int = 120):
+    """main.rs's search3 from start value num0: (overflowed, tally of 16-step blocks)."""
+    b = (C.c_uint64 * 2)(base & MASK64, base >> 64)
+    v = (C.c_uint64 * 3)(num0 & MASK64, (num0 >> 64) & MASK64, num0 >> 128)
+    t = C.c_uint64()
+    r = lib().or_rust_search3(b, v, cutoff, C.byref(t))
+    return bool(r), t.value
 
 
 def rust_count(kn, bv, mod9, root: Node) -> int:

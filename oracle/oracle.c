@@ -390,6 +390,68 @@ void or_traj(const uint64_t base[2], const uint64_t num0[3], uint32_t cutoff, in
 }
 
 /* ------------------------------------------------------------------------------------
+ * main.rs's search3 (main.rs:321-341) with its 16-step table (precompute, main.rs:111-123):
+ * entry[n0] for n0 < 2^16 holds t | S^16(n0) << 5, t the odd steps among 16 steps of the
+ * shifted map S (m even: 3m/2; m odd: m/2 + 1, rounding down).  A block is
+ * number = (number >> 16) * 3^t + S^16(number mod 2^16).  After each block the tally grows
+ * by one; the candidate stops when number < base, or when number >= 2^cutoff (main.rs:336
+ * tests (number >> 64) >= 2^56, i.e. cutoff 120, and prints the start).  Kept exactly in
+ * three u64 limbs: main.rs's u128 would wrap for a block that starts in [2^118.6, 2^120).
+ * ---------------------------------------------------------------------------------- */
+static uint32_t g_ms16[1u << 16];
+static pthread_once_t g_ms16_once = PTHREAD_ONCE_INIT;
+
+static void ms16_init(void) {
+  for (uint32_t n0 = 0; n0 < (1u << 16); ++n0) {
+    uint32_t t = 0;
+    uint64_t n = n0;
+    for (int i = 0; i < 16; ++i) {
+      if ((n & 1) == 0) {
+        n = n + n / 2;
+        t++;
+      } else {
+        n = n / 2 + 1;
+      }
+    }
+    g_ms16[n0] = t | ((uint32_t)n << 5);
+  }
+}
+
+static inline void big_add_small(big3* x, uint64_t a) {
+  u128 s = (u128)x->w[0] + a;
+  x->w[0] = (uint64_t)s;
+  if ((s >> 64) == 0) return;
+  for (int i = 1; i < 3; ++i)
+    if (++x->w[i] != 0) break;
+}
+
+int or_rust_search3(const uint64_t base[2], const uint64_t num0[3], uint32_t cutoff,
+                    uint64_t* tally) {
+  pthread_once(&g_ms16_once, ms16_init);
+  if (cutoff > 180) cutoff = 180;
+  big3 v = {{num0[0], num0[1], num0[2]}};
+  const big3 b = {{base[0], base[1], 0}};
+  uint64_t t = 0;
+  for (;;) {
+    const uint32_t e = g_ms16[v.w[0] & 0xFFFFu];
+    uint64_t p3 = 1;
+    for (uint32_t i = 0; i < (e & 31u); ++i) p3 *= 3;
+    big_shr(&v, 16);
+    big_mul_small(&v, p3);
+    big_add_small(&v, e >> 5);
+    t++;
+    if (big_cmp(&v, &b) < 0) {
+      *tally = t;
+      return 0;
+    }
+    if (big_ge_pow2(&v, cutoff)) {
+      *tally = t;
+      return 1;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------------------
  * Reference mode: search2 (collatz_cudav7.cu:654-682, main.rs:280-319) and whole case
  * lines (collatz_cudav7.cu:95-183 with launchkernel :565-649).
  * ---------------------------------------------------------------------------------- */
@@ -403,7 +465,18 @@ typedef struct {
   uint64_t cand;
   uint64_t ovf;
   int count_only;
+  uint64_t numcases;     /* main.rs semantics: 16-step blocks of search3, summed */
+  uint64_t* ovf_starts;  /* optional: the start n of each overflow report, two words each */
+  uint64_t ovf_cap;
 } leaf_ctx;
+
+static void note_overflow(leaf_ctx* c, u128 n) {
+  if (c->ovf_starts && c->ovf < c->ovf_cap) {
+    c->ovf_starts[2 * c->ovf] = (uint64_t)n;
+    c->ovf_starts[2 * c->ovf + 1] = (uint64_t)(n >> 64);
+  }
+  c->ovf++;
+}
 
 static void leaf_candidates(leaf_ctx* c, const or_node* leaf) {
   const or_knobs* kn = c->kn;
@@ -436,18 +509,25 @@ static void leaf_candidates(leaf_ctx* c, const or_node* leaf) {
     u128 v = number + (u128)a * p3;
     uint64_t b2[2] = {(uint64_t)n, (uint64_t)(n >> 64)};
     uint64_t v3[3] = {(uint64_t)v, (uint64_t)(v >> 64), 0};
+    if (kn->semantics == OR_SEM_RUST) {
+      /* main.rs:314 -> search3 (:321-341): 16-step blocks, no hash. */
+      uint64_t tally = 0;
+      if (or_rust_search3(b2, v3, kn->cutoff, &tally)) note_overflow(c, n);
+      c->numcases += tally;
+      continue;
+    }
     or_traj_out o;
     /* Leaves without a bit vector run on the CPU in the reference (search2/search3,
      * collatz_cudav7.cu:517-519), whose second overflow test is '>'. */
     or_traj(b2, v3, kn->cutoff, !use_bv, &o);
     c->hash ^= o.hash;
-    c->ovf += o.overflow;
+    for (uint32_t i = 0; i < o.overflow; ++i) note_overflow(c, n);
   }
 }
 
 uint32_t or_leaf_hash(const or_knobs* kn, const uint64_t* bv, const uint64_t* mod9,
                       const or_node* leaf, uint64_t* ncand) {
-  leaf_ctx c = {kn, bv, mod9, 0, 0, 0, 0};
+  leaf_ctx c = {kn, bv, mod9, 0, 0, 0, 0, 0, NULL, 0};
   leaf_candidates(&c, leaf);
   if (ncand) *ncand = c.cand;
   return c.hash;
@@ -466,21 +546,125 @@ static void case_leaf(void* vctx, const or_node* leaf) {
   leaf_candidates(&c->lc, leaf);
 }
 
-int or_case_run(const or_knobs* kn, const uint64_t* bv, const uint64_t* mod9,
-                const or_node* root, or_case_out* out) {
+int or_case_run_starts(const or_knobs* kn, const uint64_t* bv, const uint64_t* mod9,
+                       const or_node* root, or_case_out* out, uint64_t* ovf_starts,
+                       uint64_t ovf_cap) {
   if (!kn || !bv || !mod9 || !root || !out || kn->A > kn->N) return -1;
   memset(out, 0, sizeof(*out));
-  case_ctx c = {{kn, bv, mod9, 0, 0, 0, 0}, out, or_tpo(kn->N - kn->A)};
+  case_ctx c = {{kn, bv, mod9, 0, 0, 0, 0, 0, ovf_starts, ovf_cap}, out, or_tpo(kn->N - kn->A)};
   int64_t n = s1_walk(root, kn->N - kn->A, kn->semantics, case_leaf, &c);
   if (n < 0) return -1;
   out->leaves = (uint64_t)n;
   out->complete_hash = c.lc.hash;
   out->candidates = c.lc.cand;
   out->overflow_count = c.lc.ovf;
+  out->numcases = c.lc.numcases;
   /* One launch per full batch of 2^16 leaves plus one flush per non-empty bin
    * (collatz_cudav7.cu:532-533, :177-179). */
   for (int b = 0; b < 16; ++b) out->launches += (out->bin_leaves[b] + 65535) / 65536;
   return 0;
+}
+
+int or_case_run(const or_knobs* kn, const uint64_t* bv, const uint64_t* mod9,
+                const or_node* root, or_case_out* out) {
+  return or_case_run_starts(kn, bv, mod9, root, out, NULL, 0);
+}
+
+/* The same case run on `threads` threads: the subtree is cut at a frontier a few levels
+ * below the root (search1's own nodes there), the frontier nodes are shared out in
+ * interleaved order, each thread runs its nodes as case lines of their own, and the
+ * results are merged (sums, XOR of hashes, per-bin leaves; the overflow starts of each
+ * thread in turn).  The same numbers as or_case_run_starts; only the order of the listed
+ * starts differs. */
+typedef struct {
+  const or_knobs* kn;
+  const uint64_t* bv;
+  const uint64_t* mod9;
+  const or_node* nodes;
+  int64_t nnodes;
+  int tid, nthreads;
+  or_case_out out;
+  uint64_t* starts;
+  uint64_t cap;
+  int rc;
+} case_job;
+
+static void* case_worker(void* vj) {
+  case_job* j = (case_job*)vj;
+  memset(&j->out, 0, sizeof(j->out));
+  j->rc = 0;
+  for (int64_t i = j->tid; i < j->nnodes; i += j->nthreads) {
+    or_case_out o;
+    const uint64_t used = j->out.overflow_count;
+    const uint64_t room = used < j->cap ? j->cap - used : 0;
+    if (or_case_run_starts(j->kn, j->bv, j->mod9, &j->nodes[i], &o,
+                           room ? j->starts + 2 * used : NULL, room) != 0) {
+      j->rc = -1;
+      return NULL;
+    }
+    j->out.leaves += o.leaves;
+    j->out.candidates += o.candidates;
+    j->out.overflow_count += o.overflow_count;
+    j->out.numcases += o.numcases;
+    j->out.complete_hash ^= o.complete_hash;
+    for (int b = 0; b < 17; ++b) j->out.bin_leaves[b] += o.bin_leaves[b];
+  }
+  return NULL;
+}
+
+int or_case_run_mt(const or_knobs* kn, const uint64_t* bv, const uint64_t* mod9,
+                   const or_node* root, or_case_out* out, uint64_t* ovf_starts,
+                   uint64_t ovf_cap, int threads) {
+  if (!kn || !root || !out || kn->A > kn->N) return -1;
+  const uint32_t depth = kn->N - kn->A;
+  if (threads <= 1 || depth < root->sigfig + 12)
+    return or_case_run_starts(kn, bv, mod9, root, out, ovf_starts, ovf_cap);
+  if (threads > 256) threads = 256;
+  const uint32_t split = root->sigfig + 10;
+  const int64_t nn = or_search1(root, split, kn->semantics, NULL, 0);
+  if (nn < 0) return -1;
+  or_node* nodes = (or_node*)malloc(sizeof(or_node) * (size_t)(nn > 0 ? nn : 1));
+  case_job* jobs = (case_job*)calloc((size_t)threads, sizeof(case_job));
+  pthread_t* th = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
+  uint64_t** bufs = (uint64_t**)calloc((size_t)threads, sizeof(uint64_t*));
+  int rc = (nodes && jobs && th && bufs) ? 0 : -1;
+  if (rc == 0) or_search1(root, split, kn->semantics, nodes, nn);
+  for (int t = 0; rc == 0 && t < threads; ++t) {
+    bufs[t] = ovf_starts ? (uint64_t*)malloc(16 * (size_t)(ovf_cap ? ovf_cap : 1)) : NULL;
+    jobs[t] = (case_job){kn, bv, mod9, nodes, nn, t, threads, {0}, bufs[t],
+                         ovf_starts ? ovf_cap : 0, 0};
+    if (pthread_create(&th[t], NULL, case_worker, &jobs[t]) != 0) {
+      rc = -1;
+      threads = t;
+    }
+  }
+  memset(out, 0, sizeof(*out));
+  uint64_t listed = 0;
+  for (int t = 0; t < threads; ++t) {
+    pthread_join(th[t], NULL);
+    if (jobs[t].rc) rc = -1;
+    const or_case_out* o = &jobs[t].out;
+    out->leaves += o->leaves;
+    out->candidates += o->candidates;
+    out->overflow_count += o->overflow_count;
+    out->numcases += o->numcases;
+    out->complete_hash ^= o->complete_hash;
+    for (int b = 0; b < 17; ++b) out->bin_leaves[b] += o->bin_leaves[b];
+    if (ovf_starts && bufs[t]) {
+      const uint64_t k = o->overflow_count < ovf_cap ? o->overflow_count : ovf_cap;
+      for (uint64_t i = 0; i < k && listed < ovf_cap; ++i, ++listed) {
+        ovf_starts[2 * listed] = bufs[t][2 * i];
+        ovf_starts[2 * listed + 1] = bufs[t][2 * i + 1];
+      }
+    }
+    free(bufs[t]);
+  }
+  for (int b = 0; b < 16; ++b) out->launches += (out->bin_leaves[b] + 65535) / 65536;
+  free(nodes);
+  free(jobs);
+  free(th);
+  free(bufs);
+  return rc;
 }
 
 int64_t or_rust_count(const or_knobs* kn, const uint64_t* bv, const uint64_t* mod9,
@@ -488,7 +672,7 @@ int64_t or_rust_count(const or_knobs* kn, const uint64_t* bv, const uint64_t* mo
   case_ctx c;
   or_case_out scratch;
   memset(&scratch, 0, sizeof(scratch));
-  leaf_ctx lc = {kn, bv, mod9, 0, 0, 0, 1};
+  leaf_ctx lc = {kn, bv, mod9, 0, 0, 0, 1, 0, NULL, 0};
   c.lc = lc;
   c.out = &scratch;
   c.tpo = or_tpo(kn->N - kn->A);

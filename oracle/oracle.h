@@ -29,6 +29,10 @@
  *   or_search1         against the compiled reference search1 leaves (case lines, N=45),
  *                      the case-file partition identity, and the paper's Fig 5.1 counts.
  *   or_leaf_hash       against the compiled reference search2.
+ *   or_rust_search3    against the shifted map iterated step by step with Python integers
+ *                      (the tally is the first multiple of 16 steps at which the value is
+ *                      below the start or at least 2^cutoff), including starts near the
+ *                      cutoff; the 16-step table itself against the same step-by-step map.
  *   or_verify_range    against brute-force T iteration (final value, max excursion,
  *                      descent) and a brute-force candidate set on small ranges.
  */
@@ -130,14 +134,31 @@ typedef struct {
   uint64_t candidates;      /* candidates iterated */
   uint64_t bin_leaves[17];  /* leaves per bigindex 0..15, and >= 16 */
   uint64_t overflow_count;
+  uint64_t numcases;        /* main.rs semantics: the printed "numcases" (16-step blocks) */
 } or_case_out;
 
 /* One leaf (search2): hash of all candidates of the leaf. */
 uint32_t or_leaf_hash(const or_knobs* kn, const uint64_t* bv, const uint64_t* mod9,
                       const or_node* leaf, uint64_t* ncand);
-/* A whole case line.  bv/mod9 as produced above for the same knobs. */
+/* A whole case line.  bv/mod9 as produced above for the same knobs.  Under OR_SEM_RUST
+ * Step 3 is main.rs's search3 (or_rust_search3): complete_hash stays 0, numcases is the
+ * sum of its tallies and cutoff is its overflow threshold (main.rs: 120). */
 int or_case_run(const or_knobs* kn, const uint64_t* bv, const uint64_t* mod9,
                 const or_node* root, or_case_out* out);
+/* The same, also storing the start n of up to ovf_cap overflow reports (two words each). */
+int or_case_run_starts(const or_knobs* kn, const uint64_t* bv, const uint64_t* mod9,
+                       const or_node* root, or_case_out* out, uint64_t* ovf_starts,
+                       uint64_t ovf_cap);
+/* or_case_run_starts on `threads` threads (the subtree split at a frontier 10 levels below
+ * the root); the same results, the listed starts in another order. */
+int or_case_run_mt(const or_knobs* kn, const uint64_t* bv, const uint64_t* mod9,
+                   const or_node* root, or_case_out* out, uint64_t* ovf_starts,
+                   uint64_t ovf_cap, int threads);
+/* main.rs's search3 (main.rs:321-341) from start value num0 for start base: returns 1 if a
+ * block ended at or above 2^cutoff (main.rs prints the start then), 0 if the value fell
+ * below base; *tally = blocks of 16 steps run.  cutoff <= 180. */
+int or_rust_search3(const uint64_t base[2], const uint64_t num0[3], uint32_t cutoff,
+                    uint64_t* tally);
 /* main.rs semantics: number of candidates that reach search3 (paper Fig 5.1). */
 int64_t or_rust_count(const or_knobs* kn, const uint64_t* bv, const uint64_t* mod9,
                       const or_node* root);

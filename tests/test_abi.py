@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_error_string():
     import collatz_amd as cz
     lib = cz.load()
-    assert lib.collatz_version() == 1
+    assert lib.collatz_version() == 2
     assert isinstance(lib.collatz_last_error(), bytes)
 
 
@@ -66,14 +66,17 @@ def test_sieve_params_are_checked_before_the_device():
 
 
 def test_reference_params_are_checked_before_the_device():
-    """Reference mode refuses knobs outside the reference's domain (here a candidate window
-    longer than a bit vector, A > B) with EINVAL before any HIP call."""
+    """Reference mode refuses knobs outside the reference's domain (a candidate window
+    longer than a bit vector, A > B) and outside a run's word widths (N > 64, N - A > 62,
+    a main.rs cutoff above 2^180) with EINVAL before any HIP call."""
     import collatz_amd as cz
     lib = cz.load()
     node = cz.Node((ctypes.c_uint64 * 2)(3, 0), (ctypes.c_uint64 * 2)(9, 0), 2, 2, 1, 0)
     res = cz.RefResult()
-    for N, A, B in ((30, 10, 8), (40, 13, 24), (30, 0, 24), (60, 10, 24)):
-        p = cz.RefParams(N, A, B, 16, 0, 100, -1, 0)
+    for N, A, B, sem, cut in ((30, 10, 8, 0, 100), (40, 13, 24, 0, 100), (30, 0, 24, 0, 100),
+                              (70, 10, 24, 0, 100), (65, 2, 24, 0, 100),
+                              (40, 6, 24, cz.SEM_RUST, 181)):
+        p = cz.RefParams(N, A, B, 16, sem, cut, -1, 0)
         assert lib.collatz_reference_run(ctypes.byref(p), ctypes.byref(node), None,
                                          ctypes.byref(res)) == cz.EINVAL, (N, A, B)
     p = cz.RefParams(30, 10, 8, 16, 0, 100, -1, 0)

@@ -100,3 +100,24 @@ def test_case_file_driver():
     m = re.search(r"XOR of complete hashes = ([0-9A-F]+)", p.stderr)
     full = cz.reference_run((3, 2, 2, 9, 1, 0), 40, 10, semantics=cz.SEM_CUDA)
     assert m and int(m.group(1), 16) == full.complete_hash
+
+
+@pytest.mark.gpu
+def test_rust_protocol(oracle):
+    """main.rs's protocol: the first four numbers of stdin with doubleeven and numevens 0
+    (main.rs:88-97), "Done with precompute" and "numcases = X" on stderr, and the start of
+    every trajectory whose 16-step block reached 2^cutoff in decimal on stdout
+    (main.rs:336-338); X and the starts equal the oracle's main.rs restatement."""
+    N, cutoff = 32, 44
+    p = run(["--N", str(N), "--semantics", "rust", "--cutoff", str(cutoff)], "3 2 2 9\n")
+    assert p.returncode == 0, p.stderr
+    assert p.stderr.startswith("Done with precompute\n")
+    m = re.search(r"numcases = (\d+)\n$", p.stderr)
+    assert m, p.stderr
+    kn = oracle.knobs(N, 6, num_bv=8, semantics=oracle.SEM_RUST, cutoff=cutoff)
+    want, starts = oracle.case_run_starts(kn, oracle.precompute_bv(kn), oracle.mod9_rows(N - 6, 6),
+                                          oracle.Node.make(3, 2, 2, 9, 0, 0), cap=1 << 20,
+                                          threads=16)
+    assert int(m.group(1)) == want.numcases > 0
+    got = sorted(int(x) for x in p.stdout.split())
+    assert got == sorted(starts) and len(got) == want.overflow_count > 0

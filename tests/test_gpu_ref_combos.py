@@ -34,6 +34,7 @@ def test_random_reference_runs(oracle, seed):
                                cutoff_log2=cutoff, overflow_cap=1 << 16)
         tag = f"N={N} A={A} B={B} bv={num_bv} sem={sem} cutoff={cutoff}"
         assert got.complete_hash == want.complete_hash, tag
+        assert got.numcases == want.numcases, tag
         assert got.candidates == want.candidates, tag
         assert list(got.bin_leaves) == list(want.bin_leaves), tag
         assert got.overflow_count == want.overflow_count, tag
