@@ -268,6 +268,42 @@ def rust_leg(cz, Ns):
     return out
 
 
+# The reference's n < 2^60 run: 24.4 h on an RTX 3060 (paper p.3-4 section 1.1), done as
+# the 278,699 lines of cases_big (paper p.4).
+CASES_BIG_LINES = 278699
+PAPER_N60_SECONDS = 24.4 * 3600
+
+
+def deep_lines_leg(cz, N=60):
+    """The reference program (CUDA semantics, A = 10, 16 bit vectors, CUTOFFPOWER 100) for
+    the 240 sampled lines of cases_big at MAXBITS = N (leaves at depth N - 10, 128 bits for
+    N > 50), one collatz_reference_run per line; the mean time per line times the 278,699
+    lines projects the whole n < 2^60 run."""
+    path = os.path.join(ROOT, "tests", "golden", "cases_big_sample.txt")
+    with open(path) as f:
+        lines = [tuple(int(x) for x in ln.split()) for ln in f if ln.strip()]
+    cz.reference_run(lines[0], N, 10, semantics=cz.SEM_CUDA)  # tables, untimed
+    t = time.perf_counter()
+    cand = leaves = 0
+    h = 0
+    for ln in lines:
+        r = cz.reference_run(ln, N, 10, semantics=cz.SEM_CUDA)
+        cand += int(r.candidates)
+        leaves += int(r.leaves)
+        h ^= r.complete_hash
+    d = time.perf_counter() - t
+    per_line = d / len(lines)
+    return {"N": N, "lines": len(lines), "seconds": d, "seconds_per_line": per_line,
+            "candidates": cand, "leaves": leaves, "xor_of_hashes": "%08X" % h,
+            "projected_full_run_hours": per_line * CASES_BIG_LINES / 3600,
+            "published_full_run_hours": PAPER_N60_SECONDS / 3600 if N == 60 else None,
+            "published_hardware": "RTX 3060 (paper p.3-4)" if N == 60 else None,
+            "projected_speedup_vs_published":
+                PAPER_N60_SECONDS / (per_line * CASES_BIG_LINES) if N == 60 else None,
+            "note": "a projection from a 240-line sample of cases_big (the sample is the "
+                    "fixture tests/golden/cases_big_sample.txt), not a full run"}
+
+
 def valu_probe():
     """The int32 VALU ceiling measured on this GPU by collatz_amd/valu_peak (a child process,
     about a second): per-instruction lane-op rates at full occupancy and the rate of the
@@ -340,6 +376,9 @@ def main():
     ap.add_argument("--only-tree", action="store_true",
                     help="run only the reference-program leg and print its record")
     ap.add_argument("--no-rust", action="store_true")
+    ap.add_argument("--no-deep", action="store_true")
+    ap.add_argument("--only-deep", action="store_true",
+                    help="run only the n < 2^60 case-line leg and print it")
     ap.add_argument("--rust-n", default="40,50",
                     help="MAXBITS of the main.rs-configuration legs (comma-separated)")
     ap.add_argument("--only-rust", action="store_true",
@@ -371,6 +410,9 @@ def main():
 
     if args.only_tree:
         print(json.dumps(tree_leg(cz, args.tree_n)), flush=True)
+        return
+    if args.only_deep:
+        print(json.dumps(deep_lines_leg(cz)), flush=True)
         return
     if args.only_rust:
         print(json.dumps(rust_leg(cz, [int(x) for x in args.rust_n.split(",") if x])),
@@ -547,6 +589,10 @@ def main():
 
     if not args.no_rust:
         line["rust_mode"] = rust_leg(cz, [int(x) for x in args.rust_n.split(",") if x])
+
+    if not args.no_deep:
+        line["reference_n60_lines"] = deep_lines_leg(cz)
+        log(f"reference program, n < 2^60 case lines: {line['reference_n60_lines']}")
 
     if not args.no_serve:
         reqs = W.request_stream(1, 20000)

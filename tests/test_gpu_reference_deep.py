@@ -61,3 +61,13 @@ def test_rust_whole_tree_equals_oracle(oracle, N, cutoff):
     assert sorted(starts_of(r)) == sorted(starts)
     if cutoff == 40:
         assert want.overflow_count > 0
+
+
+def test_paper_n50_main_rs_count():
+    """paper p.3 section 1.1: for all n < 2^50 main.rs's method iterates 434,413,363,553
+    integers (A = 6, eight bit vectors, its search2 filters; the leaves at depth 44 take
+    128 bits)."""
+    import collatz_amd as cz
+    r = cz.reference_run(ROOT_NODE, 50, 6, num_bv=8, semantics=cz.SEM_RUST)
+    assert r.candidates == 434413363553
+    assert r.numcases > r.candidates and r.overflow_count == 0
