@@ -347,6 +347,61 @@ def tree_leg(cz, N):
     return tree
 
 
+def range_leg(cz, sv, steps, forms=("plain", "persistent")):
+    """The single range [2^40, 2^40 + 2^36) with the k = 32 sieve and 2^24-wide chunk
+    records, in the plain drop-in form and the persistent form of collatz_verify_range."""
+    rng = {}
+    by_name = {"plain": cz.FORM_PLAIN, "persistent": cz.FORM_PERSISTENT}
+    for name in forms:
+        form = by_name[name]
+        sv.verify_range(RANGE_LO, RANGE_LO + RANGE_WIDTH, RANGE_CHUNK, form=form)
+        ms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            o = sv.verify_range(RANGE_LO, RANGE_LO + RANGE_WIDTH, RANGE_CHUNK, form=form)
+            ms.append(o.device_ms)
+        d = (time.perf_counter() - t0) / steps
+        dms = sum(ms) / len(ms)
+        Lr = words_for(RANGE_LO + RANGE_WIDTH)
+        ach = (int(o.checks) * ops_dr(Lr) + int(o.total["count"]) * ops_setup(Lr)) / \
+            (dms * 1e-3) / 1e12
+        rng[name] = {"n_per_s": RANGE_WIDTH / d, "ms_per_call": d * 1e3,
+                     "device_ms": dms, "device_n_per_s": RANGE_WIDTH / (dms * 1e-3),
+                     "candidates": int(o.total["count"]), "checks": int(o.checks),
+                     "roofline_frac": ach / VALU_PEAK_TOPS,
+                     "hash": "%08X" % int(o.total["hash"])}
+        log(f"range, {name}: {rng[name]}")
+    return rng
+
+
+def large_leg(cz, steps):
+    """Large sieve: [2^60, 2^60 + 2^44) with k = 37 (|S_37| records resident), 2^30-wide
+    chunks, batched form."""
+    t = time.time()
+    s37 = cz.Sieve(37)
+    build_s = time.time() - t
+    lo, hi = 1 << 60, (1 << 60) + (1 << 44)
+    s37.verify_batch([(lo, hi)], chunk_log2=30, use_graph=True)
+    ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o = s37.verify_batch([(lo, hi)], chunk_log2=30, use_graph=True)[0]
+        ms.append(o.device_ms)
+    d = (time.perf_counter() - t0) / steps
+    kms = sum(ms) / len(ms)
+    L37 = words_for(hi)
+    ach = (int(o.checks) * ops_dr(L37) + int(o.total["count"]) * ops_setup(L37)) / \
+        (kms * 1e-3) / 1e12
+    out = {"range": "[2^60, 2^60+2^44)", "chunk_log2": 30, "n_per_s": (hi - lo) / (kms * 1e-3),
+           "host_inclusive_n_per_s": (hi - lo) / d,
+           "ms_per_step": d * 1e3, "kernel_ms": kms, "candidates": int(o.total["count"]),
+           "checks": int(o.checks), "survivors": int(s37.info().survivors),
+           "sieve_build_s": build_s, "escalated": int(o.escalated),
+           "hash": "%08X" % int(o.total["hash"]), "roofline_frac": ach / VALU_PEAK_TOPS}
+    log(f"large sieve k=37: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -385,6 +440,11 @@ def main():
                     help="run only the main.rs-configuration legs and print them")
     ap.add_argument("--only-lookahead", action="store_true",
                     help="run only the look-ahead leg (batch workload, B = 16) and print it")
+    ap.add_argument("--only-range", default=None, metavar="FORMS",
+                    help="run only the [2^40, 2^40+2^36) range leg in the given forms "
+                         "(comma-separated: plain, persistent) and print it")
+    ap.add_argument("--only-large", action="store_true",
+                    help="run only the k = 37 leg ([2^60, 2^60+2^44)) and print it")
     ap.add_argument("--large-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-intervals", type=int, default=128)
@@ -416,6 +476,16 @@ def main():
         return
     if args.only_rust:
         print(json.dumps(rust_leg(cz, [int(x) for x in args.rust_n.split(",") if x])),
+              flush=True)
+        return
+
+    if args.only_large:
+        print(json.dumps(large_leg(cz, args.large_steps)), flush=True)
+        return
+    if args.only_range:
+        s32 = cz.Sieve(K)
+        print(json.dumps(range_leg(cz, s32, args.range_steps,
+                                   tuple(x for x in args.only_range.split(",") if x))),
               flush=True)
         return
 
@@ -529,54 +599,10 @@ def main():
             f"identical records {same}")
 
     if not args.no_range:
-        rng = {}
-        for name, form in (("plain", cz.FORM_PLAIN), ("persistent", cz.FORM_PERSISTENT)):
-            sv.verify_range(RANGE_LO, RANGE_LO + RANGE_WIDTH, RANGE_CHUNK, form=form)
-            ms = []
-            t0 = time.perf_counter()
-            for _ in range(args.range_steps):
-                o = sv.verify_range(RANGE_LO, RANGE_LO + RANGE_WIDTH, RANGE_CHUNK, form=form)
-                ms.append(o.device_ms)
-            d = (time.perf_counter() - t0) / args.range_steps
-            dms = sum(ms) / len(ms)
-            Lr = words_for(RANGE_LO + RANGE_WIDTH)
-            ach = (int(o.checks) * ops_dr(Lr) + int(o.total["count"]) * ops_setup(Lr)) / \
-                (dms * 1e-3) / 1e12
-            rng[name] = {"n_per_s": RANGE_WIDTH / d, "ms_per_call": d * 1e3,
-                         "device_ms": dms, "device_n_per_s": RANGE_WIDTH / (dms * 1e-3),
-                         "candidates": int(o.total["count"]), "checks": int(o.checks),
-                         "roofline_frac": ach / VALU_PEAK_TOPS,
-                         "hash": "%08X" % int(o.total["hash"])}
-        line["range_2p40_w2p36"] = rng
-        log(f"range: plain {rng['plain']['n_per_s']:.3e}, persistent "
-            f"{rng['persistent']['n_per_s']:.3e} n/s")
+        line["range_2p40_w2p36"] = range_leg(cz, sv, args.range_steps)
 
     if not args.no_large:
-        # Large sieve: [2^60, 2^60 + 2^44) with k = 37 (|S_37| records resident), 2^30-wide
-        # chunks, batched form.
-        t = time.time()
-        s37 = cz.Sieve(37)
-        build_s = time.time() - t
-        lo, hi = 1 << 60, (1 << 60) + (1 << 44)
-        s37.verify_batch([(lo, hi)], chunk_log2=30, use_graph=True)
-        ms = []
-        t0 = time.perf_counter()
-        for _ in range(args.large_steps):
-            o = s37.verify_batch([(lo, hi)], chunk_log2=30, use_graph=True)[0]
-            ms.append(o.device_ms)
-        d = (time.perf_counter() - t0) / args.large_steps
-        kms = sum(ms) / len(ms)
-        L37 = words_for(hi)
-        ach = (int(o.checks) * ops_dr(L37) + int(o.total["count"]) * ops_setup(L37)) / (kms * 1e-3) / 1e12
-        line["large_sieve_k37"] = {
-            "range": "[2^60, 2^60+2^44)", "chunk_log2": 30, "n_per_s": (hi - lo) / (kms * 1e-3),
-            "host_inclusive_n_per_s": (hi - lo) / d,
-            "ms_per_step": d * 1e3, "kernel_ms": kms, "candidates": int(o.total["count"]),
-            "survivors": int(s37.info().survivors), "sieve_build_s": build_s,
-            "escalated": int(o.escalated), "hash": "%08X" % int(o.total["hash"]),
-            "roofline_frac": ach / VALU_PEAK_TOPS}
-        log(f"large sieve k=37: {line['large_sieve_k37']}")
-        del s37
+        line["large_sieve_k37"] = large_leg(cz, args.large_steps)
 
     if not args.no_lookahead:
         line["lookahead"] = lookahead_leg(cz, ivs, total_n, args.steps)

@@ -162,7 +162,7 @@ struct PlainView {
   DevRec* esc_recs = nullptr;    // records of the three-limb pass
   uint64_t* ovf = nullptr;       // overflow reports, 4 words each
   uint64_t ovf_cap = 0;
-  unsigned long long* cnt = nullptr;  // escalations, overflow reports, stalls
+  unsigned long long* cnt = nullptr;  // escalations, overflow reports, stalls, loop aborts
   uint32_t* stall = nullptr;     // slots of abandoned candidates (kStallCap)
   hipEvent_t ev[2] = {nullptr, nullptr};
 };

@@ -25,8 +25,10 @@ namespace {
 constexpr uint64_t kNaRows = 32768;               // row descriptors (q, chunk, residue class)
 constexpr uint64_t kNaKeys = 2 * kNaRows;         // residue bounds to look up
 constexpr uint64_t kNaChunks = 4096;              // chunk records
-constexpr uint64_t kNaTargetBlocks = 65536;       // blocks the launch aims at
-constexpr uint64_t kNaBlocks = kNaTargetBlocks + kNaRows;  // a row adds at most one block
+constexpr uint64_t kNaTargetBlocks = 16384;       // blocks the launch aims at
+constexpr uint64_t kNaBlocks = 98304;             // blocks of one launch, at most
+// A launch aims at kNaTargetBlocks blocks and each row adds at most one more.
+static_assert(kNaBlocks >= kNaTargetBlocks + kNaRows, "block records too few");
 constexpr uint64_t kNaEsc = 1ull << 16;           // escalated candidates (more: one width up)
 constexpr uint64_t kNaOvf = 4096;                 // overflow starts listed
 
@@ -44,7 +46,7 @@ constexpr uint64_t kOffEsc = align256(kOffChunk + kNaChunks * sizeof(DevRec));
 constexpr uint64_t kOffEscRecs = align256(kOffEsc + kNaEsc * sizeof(EscEntry));
 constexpr uint64_t kOffOvf = align256(kOffEscRecs + kNaEsc * sizeof(DevRec));
 constexpr uint64_t kOffCnt = align256(kOffOvf + kNaOvf * 32);
-constexpr uint64_t kOffStall = align256(kOffCnt + 3 * 8);
+constexpr uint64_t kOffStall = align256(kOffCnt + 4 * 8);
 constexpr uint64_t kScratchBytes = align256(kOffStall + kStallCap * 4);
 static_assert(kScratchBytes < (24ull << 20), "scratch array larger than intended");
 

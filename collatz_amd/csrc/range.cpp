@@ -90,8 +90,10 @@ int verify_plain_with(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs
   uint64_t slots = 0;
   for (const RowDesc& r : plan.rows) slots += r.i_hi - r.i_lo;
   const uint64_t target_blocks = B.target_blocks();
+  // Survivors per wave: 64 R.  A wave's share ends with a tail in which its lanes idle
+  // while the share's last trajectories finish, so shares are large (R up to 4096).
   const uint32_t R = (uint32_t)std::min<uint64_t>(
-      256, std::max<uint64_t>(1, (slots + 256 * target_blocks - 1) / (256 * target_blocks)));
+      4096, std::max<uint64_t>(1, (slots + 256 * target_blocks - 1) / (256 * target_blocks)));
   const uint64_t span = 256ull * R;
   uint64_t nblocks = 0;
   for (RowDesc& r : plan.rows) {
@@ -110,7 +112,7 @@ int verify_plain_with(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs
                         hipMemcpyHostToDevice, s));
   CZ_HIP(hipMemcpyAsync(v.part, part_start.data(), part_start.size() * 8,
                         hipMemcpyHostToDevice, s));
-  CZ_HIP(hipMemsetAsync(v.cnt, 0, 24, s));
+  CZ_HIP(hipMemsetAsync(v.cnt, 0, 32, s));
   DeviceLists lists;
   lists.esc = v.esc;
   lists.esc_count = v.cnt;
@@ -121,7 +123,7 @@ int verify_plain_with(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs
   lists.stall_count = v.cnt + 2;
   lists.stall_slots = v.stall;
   lists.stall_cap = kStallCap;
-  lists.loop_aborts = nullptr;  // (the plain kernel's loops have fixed trip counts)
+  lists.loop_aborts = v.cnt + 3;
   const Limits lim = make_limits((int)a.width, a.cutoff);
   // Device time: the verification and reduction kernels, then the three-limb pass.
   CZ_HIP(hipEventRecord(v.ev[0], s));
@@ -130,13 +132,16 @@ int verify_plain_with(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs
   CZ_HIP(launch_reduce_chunks(v.block, v.part, (uint32_t)plan.nchunks, v.chunk, s));
   CZ_HIP(hipEventRecord(v.ev[1], s));
   std::vector<DevRec> recs(plan.nchunks);
-  unsigned long long cnt[3];
+  unsigned long long cnt[4];
   CZ_HIP(hipMemcpyAsync(recs.data(), v.chunk, recs.size() * sizeof(DevRec),
                         hipMemcpyDeviceToHost, s));
-  CZ_HIP(hipMemcpyAsync(cnt, v.cnt, 24, hipMemcpyDeviceToHost, s));
+  CZ_HIP(hipMemcpyAsync(cnt, v.cnt, 32, hipMemcpyDeviceToHost, s));
   CZ_HIP(hipStreamSynchronize(s));
   float ms = 0;
   (void)hipEventElapsedTime(&ms, v.ev[0], v.ev[1]);
+  if (cnt[3])
+    return fail(COLLATZ_EDEVICE, "a work loop of the plain kernel reached its iteration bound (" +
+                                     std::to_string(cnt[3]) + " waves abandoned)");
   if (a.width < 3 && cnt[0] > lists.esc_cap) {
     // More escalations than the list holds: run the whole range one width up.
     RangeArgs b = a;
@@ -201,7 +206,7 @@ class ScratchBuffers final : public PlainBuffers {
   explicit ScratchBuffers(PlainScratch& s) : scr_(s) {}
   uint64_t max_rows() const override { return 1ull << 26; }
   uint64_t max_chunks() const override { return 0x7FFFFFFFull; }
-  uint64_t target_blocks() const override { return 1ull << 18; }
+  uint64_t target_blocks() const override { return 1ull << 14; }
   int keys(uint64_t n, PlainView& v) override {
     CZ_HIP(scr_.keys.reserve(n));
     CZ_HIP(scr_.key_cls.reserve(n));
@@ -220,7 +225,7 @@ class ScratchBuffers final : public PlainBuffers {
     CZ_HIP(scr_.chunk.reserve(std::max<uint64_t>(1, nchunks)));
     CZ_HIP(scr_.esc.reserve(esc ? esc_cap : 1));
     CZ_HIP(scr_.ovf.reserve(4 * ovf_cap));
-    CZ_HIP(scr_.cnt.reserve(3));
+    CZ_HIP(scr_.cnt.reserve(4));
     CZ_HIP(scr_.stall.reserve(kStallCap));
     for (hipEvent_t& e : scr_.ev)
       if (!e) CZ_HIP(hipEventCreate(&e));

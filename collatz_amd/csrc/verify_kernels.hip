@@ -345,7 +345,18 @@ __device__ __forceinline__ bool run_candidate(Cand<L>& c, Acc<L>& acc, const Lim
   return true;
 }
 
-// ---- plain kernel: one lane per (residue, q) slot --------------------------------------
+// ---- plain kernel: a fixed share of one row per wave, lanes refilled --------------------
+// Block b covers up to 256 R consecutive survivors of one row (the row whose block range
+// holds b); each of its four waves takes a quarter of them.  A lane runs one candidate at
+// a time, one check per loop iteration.  A lane whose candidate has ended (descended,
+// escalated or abandoned at the check bound) starts the survivor record it loaded one
+// iteration earlier, and lanes without a loaded record take the next survivors of the
+// wave's share (consecutive indices by rank among them), so no lane waits for its wave's
+// slowest trajectory until the share is used up, and the record's load overlaps a check.
+// The loop ends when no lane holds a candidate or a record and the share is used up.  It
+// also ends at lim.max_loop iterations (counted in lists.loop_aborts: the host then fails
+// the call); a wave's loop normally runs about share / 64 x (checks per candidate)
+// iterations plus its tail, far below the default bound of 2^31.
 template <int L, bool LOWCUT>
 __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowDesc* rows,
                                                       uint32_t nrows, uint32_t R, uint32_t mod9,
@@ -368,26 +379,80 @@ __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowD
 
   Acc<L> acc;
   acc.init();
-  const uint64_t first = row.i_lo + (blk - row.block_start) * (256ull * R) + threadIdx.x;
-  for (uint32_t it = 0; it < R; ++it) {
-    const uint64_t i = first + (uint64_t)it * 256;
-    if (i >= row.i_hi) break;
-    const uint4 sr = sv.rec[i];
-    const uint64_t r = rec_r(sr);
-    const uint64_t M = rec_M(sr);
-    Cand<L> c;
-    const uint32_t f = rec_f(sr);
-    if (sv.la && !la_keep(sv, f, (uint32_t)M + q3[f][0])) continue;
-    c.slot = row.slot;
-    uint32_t bigw = 0;
-#pragma unroll
-    for (int i = L; i < 6; ++i) bigw |= q3[f][i];
-    const bool big = bigw != 0;
-    if (!cand_start<L>(c, M, q3[f], big, row.qb_lo, row.qb_hi, r) ||
-        !run_candidate<L, LOWCUT>(c, acc, lim, lists, sv.k, p3s)) {
-      push_escalate(lists, c.n_lo, c.n_hi, i, row.slot);
+  SinkOne<L> sink{acc};
+  // The block's survivors [bfirst, bfirst + blen), split evenly over its four waves: the
+  // wave's share is [next, end), empty where a block holds fewer survivors than waves.
+  const uint64_t bfirst = row.i_lo + (blk - row.block_start) * (256ull * R);
+  const uint64_t blen = row.i_hi - bfirst < 256ull * R ? row.i_hi - bfirst : 256ull * R;
+  const uint64_t share = (blen + 3) >> 2;
+  uint64_t next = bfirst + (threadIdx.x >> 6) * share;
+  const uint64_t end = next + share < bfirst + blen ? next + share : bfirst + blen;
+  Cand<L> c;
+  c.slot = row.slot;
+  uint64_t cur_i = 0, pf_i = 0;  // survivor index of the candidate and of the loaded record
+  uint4 pf = make_uint4(0, 0, 0, 0);
+  bool have = false, pf_ok = false;
+  uint32_t nchk = 0;  // checks of the lane's candidate so far
+  uint32_t iters = 0;
+  bool aborted = false;
+  for (;;) {
+    if (__builtin_expect(++iters > lim.max_loop, 0)) {
+      aborted = true;
+      break;
     }
+    // Start the loaded record.
+    if (!have && pf_ok) {
+      pf_ok = false;
+      const uint64_t r = rec_r(pf);
+      const uint64_t M = rec_M(pf);
+      const uint32_t f = rec_f(pf);
+      if (!sv.la || la_keep(sv, f, (uint32_t)M + q3[f][0])) {
+        uint32_t bigw = 0;
+#pragma unroll
+        for (int i = L; i < 6; ++i) bigw |= q3[f][i];
+        cur_i = pf_i;
+        if (cand_start<L>(c, M, q3[f], bigw != 0, row.qb_lo, row.qb_hi, r) &&
+            cand_enter<L, LOWCUT>(c)) {
+          have = true;
+          nchk = 0;
+        } else {
+          push_escalate(lists, c.n_lo, c.n_hi, cur_i, row.slot);
+        }
+      }
+    }
+    // Lanes without a record load the next survivors of the share.
+    const uint64_t want = ballot(!pf_ok);
+    if (want != 0 && next < end) {
+      const uint32_t rank = lane_rank(want);
+      const uint64_t avail = end - next;
+      const uint32_t nwant = (uint32_t)__popcll(want);
+      const uint32_t take = avail < nwant ? (uint32_t)avail : nwant;
+      if (!pf_ok && rank < take) {
+        pf_i = next + rank;
+        pf = sv.rec[pf_i];
+        pf_ok = true;
+      }
+      next += take;
+    }
+    // One check.
+    if (have) {
+      bool esc, done;
+      one_check<L, LOWCUT>(c, c.n_hi, sink, lim, lists, p3s, esc, done);
+      acc.chk += 1;
+      if (esc) {
+        push_escalate(lists, c.n_lo, c.n_hi, cur_i, row.slot);
+        have = false;
+      } else if (done) {
+        finish<L>(c, acc, sv.k);
+        have = false;
+      } else if (++nchk >= lim.max_checks) {
+        push_stall(lists, c.slot);
+        have = false;
+      }
+    }
+    if (ballot(have) == 0 && ballot(pf_ok) == 0 && next >= end) break;
   }
+  if (aborted && (threadIdx.x & 63u) == 0) atomicAdd(lists.loop_aborts, 1ull);
   DevRec rec = block_merge<4>(acc_to_rec<L>(acc), lrec);
   if (threadIdx.x == 0) recs[blk] = rec;
 }

@@ -1,6 +1,6 @@
 """The failure paths of the kernels' own bounds, forced from the host: with
 COLLATZ_CHECK_LIMIT set to a few checks, candidates that need more are abandoned, and with
-COLLATZ_LOOP_LIMIT set to one iteration the persistent kernels abandon their tiles; every
+COLLATZ_LOOP_LIMIT set to one iteration the kernels abandon their work loops; every
 kernel still exits normally and each form reports the failure instead of a result.  Also the
 served form's isolation of such a failure to the request it belongs to, and the lifetime
 rules of its tickets."""
@@ -38,10 +38,11 @@ def test_check_bound_failure_is_reported(tiny_check_limit):
 
 
 def test_loop_bound_failure_is_reported():
-    """The work-loop bound of the persistent kernels (iterations per tile, and of the take
-    loop per refill), forced to one iteration with COLLATZ_LOOP_LIMIT: waves abandon their
-    tiles and exit normally, and the persistent, batched, served and reference-mode forms
-    fail with the bound named; the plain kernel has no such loop and still succeeds."""
+    """The work-loop bound of the kernels (iterations per tile of the persistent kernels and
+    of their take loop per refill, iterations per wave share of the plain kernel), forced to
+    one iteration with COLLATZ_LOOP_LIMIT: waves abandon their work and exit normally, and
+    the plain, no-allocation, persistent, batched, served and reference-mode forms fail
+    with the bound named."""
     import collatz_amd as cz
     os.environ["COLLATZ_LOOP_LIMIT"] = "1"
     try:
@@ -49,8 +50,12 @@ def test_loop_bound_failure_is_reported():
         sla = cz.Sieve(20, lookahead_bits=16)
         lo, hi = (1 << 40) + 1, (1 << 40) + (1 << 24)
         for s in (sv, sla):
+            for form in (cz.FORM_PERSISTENT, cz.FORM_PLAIN):
+                with pytest.raises(cz.CollatzError) as e:
+                    s.verify_range(lo, hi, 20, form=form)
+                assert e.value.code == cz.EDEVICE and "iteration bound" in str(e.value)
             with pytest.raises(cz.CollatzError) as e:
-                s.verify_range(lo, hi, 20, form=cz.FORM_PERSISTENT)
+                s.verify_range(lo, hi, 20, form=cz.FORM_PLAIN, noalloc=True)
             assert e.value.code == cz.EDEVICE and "iteration bound" in str(e.value)
             with pytest.raises(cz.CollatzError) as e:
                 s.verify_batch([(lo, hi), (lo << 8, (lo << 8) + (1 << 22))], chunk_log2=20,
@@ -65,9 +70,12 @@ def test_loop_bound_failure_is_reported():
             srv.wait(t, 60000)
         assert "iteration bound" in str(e.value)
         srv.close()
-        assert int(sv.verify_range(lo, hi, 20, form=cz.FORM_PLAIN).total["count"]) > 0
     finally:
         del os.environ["COLLATZ_LOOP_LIMIT"]
+    # The bound is read per call: without the variable the same calls succeed.
+    assert int(sv.verify_range(lo, hi, 20, form=cz.FORM_PLAIN).total["count"]) > 0
+    assert int(sv.verify_range(lo, hi, 20, form=cz.FORM_PLAIN, noalloc=True)
+               .total["count"]) > 0
 
 
 def test_default_bound_after_the_hook():
