@@ -66,13 +66,14 @@ class RangeResult(C.Structure):
     _fields_ = [("total", Chunk), ("chunks", C.POINTER(Chunk)), ("chunks_cap", C.c_uint64),
                 ("chunks_written", C.c_uint64), ("overflow_starts", C.POINTER(C.c_uint64)),
                 ("overflow_cap", C.c_uint64), ("overflow_written", C.c_uint64),
-                ("escalated", C.c_uint64), ("checks", C.c_uint64), ("device_ms", C.c_double)]
+                ("escalated", C.c_uint64), ("checks", C.c_uint64), ("device_ms", C.c_double),
+                ("spot_checks", C.c_uint64)]
 
 
 RESULT_DTYPE = np.dtype([
     ("total", CHUNK_DTYPE), ("chunks", "<u8"), ("chunks_cap", "<u8"), ("chunks_written", "<u8"),
     ("overflow_starts", "<u8"), ("overflow_cap", "<u8"), ("overflow_written", "<u8"),
-    ("escalated", "<u8"), ("checks", "<u8"), ("device_ms", "<f8"),
+    ("escalated", "<u8"), ("checks", "<u8"), ("device_ms", "<f8"), ("spot_checks", "<u8"),
 ])
 assert RESULT_DTYPE.itemsize == C.sizeof(RangeResult)
 
@@ -90,7 +91,7 @@ class BatchOpts(C.Structure):
     _fields_ = [("chunk_log2", C.c_uint32), ("limbs", C.c_uint32), ("cutoff_log2", C.c_uint32),
                 ("use_graph", C.c_uint32), ("tile_residues", C.c_uint32),
                 ("waves_per_cu", C.c_uint32), ("checks_per_refill", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("spot_checks", C.c_uint32)]
 
 
 class Node(C.Structure):
@@ -102,7 +103,7 @@ class Node(C.Structure):
 class RefParams(C.Structure):
     _fields_ = [("N", C.c_uint32), ("A", C.c_uint32), ("B", C.c_uint32), ("num_bv", C.c_uint32),
                 ("semantics", C.c_uint32), ("cutoff_log2", C.c_uint32), ("device", C.c_int32),
-                ("reserved", C.c_uint32)]
+                ("spot_check_freq", C.c_uint32)]
 
 
 class RefResult(C.Structure):
@@ -111,7 +112,8 @@ class RefResult(C.Structure):
                 ("bin_leaves", C.c_uint64 * 17), ("overflow_count", C.c_uint64),
                 ("overflow_starts", C.POINTER(C.c_uint64)), ("overflow_cap", C.c_uint64),
                 ("overflow_written", C.c_uint64), ("device_ms", C.c_double),
-                ("numcases", C.c_uint64)]
+                ("numcases", C.c_uint64), ("spot_checks", C.c_uint64),
+                ("spot_check_failures", C.c_uint64)]
 
 
 class ServerOpts(C.Structure):
@@ -192,6 +194,7 @@ class RangeOutput:
     device_ms: float
     checks: int = 0
     overflow_written: int = 0  # overflow reports made (may exceed the starts listed)
+    spot_checks: int = 0       # chunk records re-run by the plain kernel and found equal
 
 
 class Sieve:
@@ -270,9 +273,10 @@ class Sieve:
     def verify_batch(self, intervals, chunk_log2: int = 24, limbs: int = 2,
                      cutoff_log2: int = 187, use_graph: bool = False, tile_residues: int = 0,
                      waves_per_cu: int = 0, checks_per_refill: int = 0, overflow_cap: int = 0,
-                     stream=None):
+                     stream=None, spot_checks: int = 0):
         """collatz_verify_batch over [(lo, hi), ...]; one RangeOutput per interval.  The
-        chunk records of all intervals share one array (each output holds a view)."""
+        chunk records of all intervals share one array (each output holds a view).  With
+        spot_checks, that many chunk records are re-run by the plain kernel and compared."""
         m = len(intervals)
         iv = np.array([(lo & MASK64, lo >> 64, hi & MASK64, hi >> 64) for lo, hi in intervals],
                       dtype=np.uint64).reshape(m, 4)
@@ -291,7 +295,7 @@ class Sieve:
                 16 * overflow_cap)
             res["overflow_cap"] = overflow_cap
         opts = BatchOpts(chunk_log2, limbs, cutoff_log2, int(use_graph), tile_residues,
-                         waves_per_cu, checks_per_refill, 0)
+                         waves_per_cu, checks_per_refill, spot_checks)
         _check(load().collatz_verify_batch(self._h, m, iv.ctypes.data_as(C.POINTER(Interval)),
                                            C.byref(opts), stream,
                                            res.ctypes.data_as(C.POINTER(RangeResult))))
@@ -300,15 +304,16 @@ class Sieve:
         dms = res["device_ms"].tolist()
         chk = res["checks"].tolist()
         ow = res["overflow_written"].tolist()
+        sc = res["spot_checks"].tolist()
         o = offs.tolist()
         if ovf is None:
             starts = [[] for _ in range(m)]
         else:
             starts = [[int(ovf[i, 2 * j]) | (int(ovf[i, 2 * j + 1]) << 64)
                        for j in range(min(ow[i], overflow_cap))] for i in range(m)]
-        return [RangeOutput(t, chunks[a:b], st, e, d, c, w)
-                for t, a, b, st, e, d, c, w in zip(totals, o[:-1], o[1:], starts, esc, dms, chk,
-                                                   ow)]
+        return [RangeOutput(t, chunks[a:b], st, e, d, c, w, k)
+                for t, a, b, st, e, d, c, w, k in zip(totals, o[:-1], o[1:], starts, esc, dms,
+                                                      chk, ow, sc)]
 
 
 class Server:
@@ -367,8 +372,10 @@ def reference_tables(N=45, A=10, B=24, num_bv=16, semantics=SEM_CUDA, device=-1)
 
 
 def reference_run(t, N=45, A=10, B=24, num_bv=16, semantics=SEM_CUDA, cutoff_log2=100,
-                  overflow_cap=1024, device=-1, stream=None) -> RefResult:
-    """One case line (base, sigfig, threepower, number, doubleeven, numevens)."""
+                  overflow_cap=1024, device=-1, stream=None, spot_check_freq=0) -> RefResult:
+    """One case line (base, sigfig, threepower, number, doubleeven, numevens).  With
+    spot_check_freq, one tile of 256 leaves in every spot_check_freq is re-run by a second
+    kernel and compared (RefResult.spot_checks)."""
     base, sigfig, tp, number, de, ne = t
     node = Node((C.c_uint64 * 2)(base & MASK64, base >> 64),
                 (C.c_uint64 * 2)(number & MASK64, number >> 64), sigfig, tp, de, ne)
@@ -376,7 +383,7 @@ def reference_run(t, N=45, A=10, B=24, num_bv=16, semantics=SEM_CUDA, cutoff_log
     out = RefResult()
     out.overflow_starts = ovf.ctypes.data_as(C.POINTER(C.c_uint64))
     out.overflow_cap = overflow_cap
-    params = RefParams(N, A, B, num_bv, semantics, cutoff_log2, device, 0)
+    params = RefParams(N, A, B, num_bv, semantics, cutoff_log2, device, spot_check_freq)
     _check(load().collatz_reference_run(C.byref(params), C.byref(node), stream, C.byref(out)))
     out._ovf = ovf
     return out

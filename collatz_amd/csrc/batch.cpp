@@ -7,7 +7,9 @@
 // (counter reset, row resolution, persistent kernel, per-chunk reduction) can be captured
 // in a hipGraph and is replayed for later batches.
 #include <algorithm>
+#include <cstring>
 #include <optional>
+#include <set>
 #include <vector>
 
 #include "engine.hpp"
@@ -107,6 +109,7 @@ int verify_batch_impl(const collatz_sieve* sv, uint64_t n, const collatz_interva
     out->total.chunk_index = first_chunk[i];
     out->chunks_written = nc;
     out->escalated = 0;
+    out->spot_checks = 0;
     out->device_ms = res.device_ms;
     // The exact number of overflow reports is the interval's overflow count (every report
     // is one overflow event of a candidate that finished); the device list, which holds at
@@ -160,6 +163,68 @@ int verify_batch_tuned(const collatz_sieve* sv, u128 lo, u128 hi, int limbs, Tun
   return rc == COLLATZ_ECAPACITY ? COLLATZ_OK : rc;
 }
 
+// Spot checks of a batch (the reference re-runs a random launch's candidates on the CPU and
+// compares the hash, collatz_cudav7.cu:619-635): `count` distinct chunk records among those
+// the caller stored (all of them if fewer), chosen by a SplitMix64 stream seeded from the
+// intervals, are re-run by the plain kernel over that chunk's part of its interval and
+// compared field by field.  outs[i].spot_checks counts interval i's records found equal; a
+// difference fails the call and names the interval and chunk.
+static int spot_check_batch(const collatz_sieve* sv, uint64_t n, const collatz_interval* iv,
+                            const BatchArgs& a, uint32_t count, hipStream_t s,
+                            collatz_range_result* outs) {
+  std::vector<uint64_t> pre(n + 1, 0);
+  for (uint64_t i = 0; i < n; ++i)
+    pre[i + 1] = pre[i] + (outs[i].chunks ? std::min(outs[i].chunks_written, outs[i].chunks_cap)
+                                          : 0);
+  const uint64_t stored = pre[n];
+  std::vector<uint64_t> picks;
+  if (stored <= count) {
+    for (uint64_t p = 0; p < stored; ++p) picks.push_back(p);
+  } else {
+    uint64_t z = 0xCBF29CE484222325ull ^ n;
+    for (uint64_t i = 0; i < n; ++i)
+      for (int w = 0; w < 2; ++w) z = (z ^ iv[i].lo[w] ^ (iv[i].hi[w] << 1)) * 0x100000001B3ull;
+    std::set<uint64_t> seen;
+    while (picks.size() < count) {
+      z += 0x9E3779B97F4A7C15ull;
+      uint64_t x = z;
+      x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+      x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+      x ^= x >> 31;
+      const uint64_t p = x % stored;
+      if (seen.insert(p).second) picks.push_back(p);
+    }
+  }
+  for (const uint64_t p : picks) {
+    const uint64_t i = std::upper_bound(pre.begin(), pre.end(), p) - pre.begin() - 1;
+    const collatz_chunk& want = outs[i].chunks[p - pre[i]];
+    const u128 lo = mk128(iv[i].lo[0], iv[i].lo[1]), hi = mk128(iv[i].hi[0], iv[i].hi[1]);
+    // (chunk_index holds the low 64 bits of the chunk number: recompute it in full.)
+    const u128 c_lo = ((lo >> a.C) + (p - pre[i])) << a.C;
+    const u128 sub_lo = std::max(lo, c_lo);
+    const u128 sub_hi = hi - c_lo > ((u128)1 << a.C) ? c_lo + ((u128)1 << a.C) : hi;
+    const uint64_t l2[2] = {lo64(sub_lo), hi64(sub_lo)}, h2[2] = {lo64(sub_hi), hi64(sub_hi)};
+    collatz_verify_opts o;
+    o.chunk_log2 = a.C;
+    o.limbs = a.limbs;
+    o.cutoff_log2 = a.cutoff;
+    o.form = COLLATZ_FORM_PLAIN;
+    collatz_chunk got;
+    collatz_range_result r;
+    std::memset(&r, 0, sizeof(r));
+    r.chunks = &got;
+    r.chunks_cap = 1;
+    const int rc = collatz_verify_range(sv, l2, h2, &o, s, &r);
+    if (rc != COLLATZ_OK && rc != COLLATZ_ECAPACITY) return rc;
+    if (r.chunks_written != 1 || std::memcmp(&got, &want, sizeof(got)) != 0)
+      return fail(COLLATZ_EDEVICE, "spot check failed: interval " + std::to_string(i) +
+                                       ", chunk record " + std::to_string(p - pre[i]) +
+                                       ": the persistent launch and the plain kernel differ");
+    outs[i].spot_checks++;
+  }
+  return COLLATZ_OK;
+}
+
 }  // namespace cz
 
 using namespace cz;
@@ -187,5 +252,9 @@ extern "C" int collatz_verify_batch(const collatz_sieve* sv, uint64_t n,
   if (a.U > 64) return fail(COLLATZ_EINVAL, "checks_per_refill must be <= 64");
   if (a.tile > (1u << 24)) return fail(COLLATZ_EINVAL, "tile_residues must be <= 2^24");
   HipDeviceGuard guard(sv->device);
-  return verify_batch_impl(sv, n, intervals, a, (hipStream_t)stream, outs, true);
+  const hipStream_t s = call_stream(stream);
+  const int rc = verify_batch_impl(sv, n, intervals, a, s, outs, true);
+  if ((rc != COLLATZ_OK && rc != COLLATZ_ECAPACITY) || !opts || !opts->spot_checks) return rc;
+  const int src = spot_check_batch(sv, n, intervals, a, opts->spot_checks, s, outs);
+  return src ? src : rc;
 }

@@ -226,9 +226,10 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
     (void)hipEventElapsedTime(&ems, e2_, e3_);
     out.device_ms += ems;
     std::vector<EscEntry> h(esc_n);
-    CZ_HIP(hipMemcpy(h.data(), esc_.p, esc_n * sizeof(EscEntry), hipMemcpyDeviceToHost));
+    CZ_HIP(hipMemcpyAsync(h.data(), esc_.p, esc_n * sizeof(EscEntry), hipMemcpyDeviceToHost, s));
+    CZ_HIP(hipMemcpyAsync(h_cnt_.p, cnt_.p, 32, hipMemcpyDeviceToHost, s));
+    CZ_HIP(hipStreamSynchronize(s));
     for (const EscEntry& e : h) out.esc_slots.push_back(e.slot);
-    CZ_HIP(hipMemcpy(h_cnt_.p, cnt_.p, 32, hipMemcpyDeviceToHost));
   }
   out.stalled = h_cnt_.p[2];
   out.loop_aborts = h_cnt_.p[3];
@@ -238,13 +239,17 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   out.stall_slots.clear();
   if (out.stalled) {
     out.stall_slots.resize(std::min<uint64_t>(out.stalled, kStallCap));
-    CZ_HIP(hipMemcpy(out.stall_slots.data(), stall_.p, out.stall_slots.size() * 4,
-                     hipMemcpyDeviceToHost));
+    CZ_HIP(hipMemcpyAsync(out.stall_slots.data(), stall_.p, out.stall_slots.size() * 4,
+                          hipMemcpyDeviceToHost, s));
+    CZ_HIP(hipStreamSynchronize(s));
   }
   out.ovf_total = h_cnt_.p[1];
   const uint64_t nov = std::min<uint64_t>(out.ovf_total, ovf_cap);
   out.ovf.resize(4 * nov);
-  if (nov) CZ_HIP(hipMemcpy(out.ovf.data(), ovf_.p, nov * 32, hipMemcpyDeviceToHost));
+  if (nov) {
+    CZ_HIP(hipMemcpyAsync(out.ovf.data(), ovf_.p, nov * 32, hipMemcpyDeviceToHost, s));
+    CZ_HIP(hipStreamSynchronize(s));
+  }
   return COLLATZ_OK;
 }
 

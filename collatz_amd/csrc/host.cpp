@@ -358,7 +358,7 @@ int collatz_sieve_build(const collatz_sieve_params* params, collatz_sieve** out)
   root.numevens = 0;
   root.depth = params->k;
   root.rust = 0;
-  hipStream_t s = nullptr;
+  hipStream_t s = hipStreamPerThread;
   LeafSet leaves;
   int rc = build_leaves(root, s, leaves);
   if (rc != COLLATZ_OK) {
@@ -386,8 +386,9 @@ int collatz_sieve_build(const collatz_sieve_params* params, collatz_sieve** out)
                                           sv->rec.p, counts.p, offs.p,
                                           sv->class_start, s);
     if (e == hipSuccess)
-      e = hipMemcpy(sv->class_start_dev.p, sv->class_start, sizeof(sv->class_start),
-                    hipMemcpyHostToDevice);
+      e = hipMemcpyAsync(sv->class_start_dev.p, sv->class_start, sizeof(sv->class_start),
+                         hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
       delete sv;
       return fail(COLLATZ_EDEVICE, std::string("class partition: ") + hipGetErrorString(e));
@@ -400,7 +401,8 @@ int collatz_sieve_build(const collatz_sieve_params* params, collatz_sieve** out)
     x *= 3;
   }
   if (sv->pow3.alloc(kPow3Max) != hipSuccess ||
-      hipMemcpy(sv->pow3.p, p3, sizeof(p3), hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpyAsync(sv->pow3.p, p3, sizeof(p3), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
     delete sv;
     return fail(COLLATZ_EDEVICE, "pow3 table upload failed");
   }
@@ -448,7 +450,11 @@ int collatz_sieve_copy(const collatz_sieve* sv, uint64_t* r, uint64_t* number, u
   // Survivors are stored class-major; return them sorted by residue.
   const uint64_t all = sv->count;
   std::vector<uint4> hrec(all);
-  if (all) CZ_HIP(hipMemcpy(hrec.data(), sv->rec.p, all * sizeof(uint4), hipMemcpyDeviceToHost));
+  if (all) {
+    CZ_HIP(hipMemcpyAsync(hrec.data(), sv->rec.p, all * sizeof(uint4), hipMemcpyDeviceToHost,
+                          hipStreamPerThread));
+    CZ_HIP(hipStreamSynchronize(hipStreamPerThread));
+  }
   std::vector<uint64_t> hr(all);
   for (uint64_t i = 0; i < all; ++i) hr[i] = rec_r(hrec[i]);
   std::vector<uint64_t> order(all);

@@ -32,6 +32,13 @@ double now_ms();
 int fail(int code, const std::string& msg);
 
 inline u128 mk128(uint64_t lo, uint64_t hi) { return ((u128)hi << 64) | lo; }
+
+// The stream a call runs on: the caller's, or for NULL the calling thread's per-thread
+// default stream, which orders itself after the caller's earlier default-stream work.  The
+// library enqueues nothing on the legacy default stream and makes no synchronous copy (which
+// would run there): the runtime refuses a legacy-stream operation while another thread
+// captures a hipGraph (a server's engine does), with hipErrorStreamCaptureImplicit.
+inline hipStream_t call_stream(void* s) { return s ? (hipStream_t)s : hipStreamPerThread; }
 inline uint64_t lo64(u128 x) { return (uint64_t)x; }
 inline uint64_t hi64(u128 x) { return (uint64_t)(x >> 64); }
 

@@ -178,6 +178,20 @@ struct RefLeaves {
   const uint32_t* meta;
 };
 
+// Spot checks of a reference-mode launch (the reference re-runs a random launch's
+// candidates on the CPU and compares the hash, collatz_cudav7.cu:619-635): tiles
+// t = off + j freq with j < cap record their own results in rec[j], and the host compares
+// them with a rerun of the same tiles' candidates by the list kernel.
+struct RefSpotRec {
+  uint32_t hash, count, numcases, overflow;
+};
+struct RefSpot {
+  uint32_t freq;  // 0: no tile is sampled
+  uint32_t off;   // < freq
+  uint32_t cap;
+  RefSpotRec* rec;
+};
+
 // Look-ahead tables of reference mode on the device.
 struct RefTablesDev {
   const uint32_t* bv;      // num_bv vectors of words_per_bv words (2^B bits + mirrored tail)
@@ -258,7 +272,12 @@ hipError_t launch_ref_fused(int limbs, bool rust, const RefTablesDev& T, const R
                             uint64_t nleaves, uint32_t mbi, const Limits& lim,
                             const uint64_t* pow3w, uint32_t tile, uint32_t U, uint32_t nblocks,
                             void* tot, uint64_t* ovf, uint64_t ovf_cap, uint64_t* esc,
-                            uint64_t esc_cap, hipStream_t s);
+                            uint64_t esc_cap, const RefSpot& spot, hipStream_t s);
+// Lists the candidates of the sampled tiles (tile leaves each) of a fused launch as
+// (leaf | a << 32 | no-bit-vector << 48) entries, appending at *count (at most cap kept).
+hipError_t launch_ref_spot_list(const RefTablesDev& T, const RefLeaves& lv, uint64_t nleaves,
+                                uint32_t tile, const RefSpot& spot, uint64_t* list,
+                                unsigned long long* count, uint64_t cap, hipStream_t s);
 hipError_t launch_ref_iterate(int limbs, bool rust, const uint64_t* cands, uint64_t ncand,
                               const RefLeaves& lv, uint32_t mbi, const Limits& lim,
                               const uint64_t* pow3w, uint32_t tile, uint32_t nblocks, void* tot,

@@ -158,5 +158,5 @@ extern "C" int collatz_verify_range_noalloc(const collatz_sieve* sv, const uint6
   if (rc) return rc;
   a.width = (uint32_t)launch_width((int)a.limbs, H, a.cutoff);
   StaticBuffers B(d);
-  return verify_plain_with(sv, L, H, a, (hipStream_t)stream, out, B);
+  return verify_plain_with(sv, L, H, a, call_stream(stream), out, B);
 }

@@ -160,16 +160,19 @@ int verify_plain_with(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs
     float ems = 0;
     (void)hipEventElapsedTime(&ems, v.ev[0], v.ev[1]);
     ms += ems;
-    CZ_HIP(hipMemcpy(&cnt[1], v.cnt + 1, 16, hipMemcpyDeviceToHost));
+    CZ_HIP(hipMemcpyAsync(&cnt[1], v.cnt + 1, 16, hipMemcpyDeviceToHost, s));
+    CZ_HIP(hipStreamSynchronize(s));
     ovf_n = cnt[1];
   }
   if (cnt[2]) {
     uint32_t slot = 0;
-    CZ_HIP(hipMemcpy(&slot, v.stall, 4, hipMemcpyDeviceToHost));
+    CZ_HIP(hipMemcpyAsync(&slot, v.stall, 4, hipMemcpyDeviceToHost, s));
+    CZ_HIP(hipStreamSynchronize(s));
     return fail(COLLATZ_EDEVICE, "a trajectory did not descend within the check bound (chunk " +
                                      std::to_string(plan.first_chunk + slot) + ")");
   }
   out->escalated = cnt[0];
+  out->spot_checks = 0;
   out->device_ms = ms;
   out->checks = 0;
   for (const DevRec& r : recs) out->checks += r.checks;
@@ -187,7 +190,10 @@ int verify_plain_with(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs
   if (out->overflow_starts && out->overflow_cap) {
     listed = std::min<uint64_t>(std::min<uint64_t>(ovf_n, v.ovf_cap), out->overflow_cap);
     std::vector<uint64_t> h(4 * listed);
-    if (listed) CZ_HIP(hipMemcpy(h.data(), v.ovf, listed * 32, hipMemcpyDeviceToHost));
+    if (listed) {
+      CZ_HIP(hipMemcpyAsync(h.data(), v.ovf, listed * 32, hipMemcpyDeviceToHost, s));
+      CZ_HIP(hipStreamSynchronize(s));
+    }
     for (uint64_t i = 0; i < listed; ++i) {
       out->overflow_starts[2 * i] = h[4 * i];
       out->overflow_starts[2 * i + 1] = h[4 * i + 1];
@@ -267,7 +273,7 @@ extern "C" int collatz_verify_range(const collatz_sieve* sv, const uint64_t lo[2
   const u128 L = mk128(lo[0], lo[1]), H = mk128(hi[0], hi[1]);
   if (L >= H) return fail(COLLATZ_EINVAL, "empty range (lo >= hi)");
   HipDeviceGuard guard(sv->device);
-  hipStream_t s = (hipStream_t)stream;
+  hipStream_t s = call_stream(stream);
   if (opts && opts->form == COLLATZ_FORM_PERSISTENT) {
     collatz_interval iv;
     iv.lo[0] = lo[0];

@@ -118,12 +118,13 @@ static int get_tables(const collatz_ref_params* p, int device, RefTables*& out) 
   CZ_HIP(t->mod9.alloc(mod9.size()));
   CZ_HIP(t->magic.alloc(p->num_bv));
   CZ_HIP(t->pow3w.alloc(2 * kPow3Max));
-  CZ_HIP(hipMemset(t->bv.p, 0, t->bv.bytes()));
-  CZ_HIP(hipMemcpy(t->mod9.p, mod9.data(), mod9.size() * 4, hipMemcpyHostToDevice));
-  CZ_HIP(hipMemcpy(t->magic.p, magic.data(), magic.size() * 4, hipMemcpyHostToDevice));
-  CZ_HIP(hipMemcpy(t->pow3w.p, p3w, sizeof(p3w), hipMemcpyHostToDevice));
-  CZ_HIP(launch_bv_build(p->B, p->num_bv, first, p->semantics, t->magic.p, t->bv.p, wpb, nw, nullptr));
-  CZ_HIP(hipDeviceSynchronize());
+  const hipStream_t s = hipStreamPerThread;
+  CZ_HIP(hipMemsetAsync(t->bv.p, 0, t->bv.bytes(), s));
+  CZ_HIP(hipMemcpyAsync(t->mod9.p, mod9.data(), mod9.size() * 4, hipMemcpyHostToDevice, s));
+  CZ_HIP(hipMemcpyAsync(t->magic.p, magic.data(), magic.size() * 4, hipMemcpyHostToDevice, s));
+  CZ_HIP(hipMemcpyAsync(t->pow3w.p, p3w, sizeof(p3w), hipMemcpyHostToDevice, s));
+  CZ_HIP(launch_bv_build(p->B, p->num_bv, first, p->semantics, t->magic.p, t->bv.p, wpb, nw, s));
+  CZ_HIP(hipStreamSynchronize(s));
   t->dev.bv = t->bv.p;
   t->dev.mod9 = t->mod9.p;
   t->dev.magic = t->magic.p;
@@ -211,9 +212,12 @@ extern "C" int collatz_reference_tables(const collatz_ref_params* params, uint32
   const uint64_t words = 1ull << (params->B - 5);
   if (bv)
     for (uint32_t i = 0; i < params->num_bv; ++i)
-      CZ_HIP(hipMemcpy(bv + i * words, t->bv.p + i * t->dev.words_per_bv, words * 4,
-                       hipMemcpyDeviceToHost));
-  if (mod9) CZ_HIP(hipMemcpy(mod9, t->mod9.p, t->mod9.bytes(), hipMemcpyDeviceToHost));
+      CZ_HIP(hipMemcpyAsync(bv + i * words, t->bv.p + i * t->dev.words_per_bv, words * 4,
+                            hipMemcpyDeviceToHost, hipStreamPerThread));
+  if (mod9)
+    CZ_HIP(hipMemcpyAsync(mod9, t->mod9.p, t->mod9.bytes(), hipMemcpyDeviceToHost,
+                          hipStreamPerThread));
+  CZ_HIP(hipStreamSynchronize(hipStreamPerThread));
   return COLLATZ_OK;
 }
 
@@ -228,7 +232,7 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
       (root->sigfig <= kWalk64MaxDepth && root->number[1]))
     return fail(COLLATZ_EINVAL, "case line out of range");
   HipDeviceGuard guard(params->device);
-  hipStream_t s = (hipStream_t)stream;
+  hipStream_t s = call_stream(stream);
   int dev = 0;
   CZ_HIP(hipGetDevice(&dev));
   RefTables* T = nullptr;
@@ -269,13 +273,92 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
   CZ_HIP(desc.alloc(esc_cap));
   const uint32_t nblocks = (uint32_t)device_cu_count(dev) * 32 / 4;
   uint64_t leaves = 0;
+  // Spot checks: sampled tiles of each fused launch keep their own results (dspot), and the
+  // same tiles' candidates are listed (dlist) and rerun by k_ref_iterate (dstot), together
+  // with a rerun of the sampled tiles' escalated candidates (dsesc, whose results the fused
+  // pass leaves to the wide pass).
+  constexpr uint32_t kTile = 256;
+  constexpr uint64_t kSpotEntries = 1ull << 23;
+  const uint32_t freq = params->spot_check_freq;
+  const uint32_t max_spot = (uint32_t)(kSpotEntries / ((uint64_t)kTile << params->A));
+  DevBuf<RefSpotRec> dspot;
+  DevBuf<uint64_t> dlist, dsel;
+  DevBuf<RefTotalsHost> dstot;
+  DevBuf<unsigned long long> dcount;
+  if (freq) CZ_HIP(dspot.alloc(max_spot));
+  uint64_t spot_checks = 0, spot_failures = 0;
+  std::string spot_msg;
+  // Runs the spot check of one root's fused launch: sp describes its sampled tiles, nesc
+  // the length of its escalation list.
+  auto spot_check = [&](size_t root_index, const RefLeaves& lv, uint64_t nleaves,
+                        const RefSpot& sp, uint64_t nesc) -> int {
+    CZ_HIP(dlist.reserve(kSpotEntries));
+    CZ_HIP(dstot.reserve(2));
+    CZ_HIP(dcount.reserve(1));
+    CZ_HIP(hipMemsetAsync(dcount.p, 0, 8, s));
+    CZ_HIP(hipMemsetAsync(dstot.p, 0, 2 * sizeof(RefTotalsHost), s));
+    CZ_HIP(launch_ref_spot_list(T->dev, lv, nleaves, kTile, sp, dlist.p, dcount.p,
+                                kSpotEntries, s));
+    unsigned long long nlist = 0;
+    CZ_HIP(hipMemcpyAsync(&nlist, dcount.p, 8, hipMemcpyDeviceToHost, s));
+    CZ_HIP(hipStreamSynchronize(s));
+    if (nlist > kSpotEntries) return fail(COLLATZ_ECAPACITY, "spot-check list overflowed");
+    CZ_HIP(launch_ref_iterate(limbs, rust, dlist.p, nlist, lv, mbi, lim, T->pow3w.p, 4096,
+                              nblocks, dstot.p, nullptr, 0, s));
+    // The sampled tiles' escalated candidates, from the fused pass's list.
+    std::vector<uint64_t> sel;
+    if (nesc) {
+      std::vector<uint64_t> e(nesc);
+      CZ_HIP(hipMemcpyAsync(e.data(), desc.p, nesc * 8, hipMemcpyDeviceToHost, s));
+      CZ_HIP(hipStreamSynchronize(s));
+      for (uint64_t x : e) {
+        const uint64_t t = (uint32_t)x / kTile;
+        if (t >= sp.off && (t - sp.off) % sp.freq == 0 && (t - sp.off) / sp.freq < sp.cap)
+          sel.push_back(x);
+      }
+    }
+    if (!sel.empty()) {
+      CZ_HIP(dsel.reserve(sel.size()));
+      CZ_HIP(hipMemcpyAsync(dsel.p, sel.data(), sel.size() * 8, hipMemcpyHostToDevice, s));
+      CZ_HIP(launch_ref_iterate(limbs, rust, dsel.p, sel.size(), lv, mbi, lim, T->pow3w.p,
+                                4096, nblocks, dstot.p + 1, nullptr, 0, s));
+    }
+    std::vector<RefSpotRec> recs(sp.cap);
+    RefTotalsHost st[2];
+    CZ_HIP(hipMemcpyAsync(recs.data(), dspot.p, sp.cap * sizeof(RefSpotRec),
+                          hipMemcpyDeviceToHost, s));
+    CZ_HIP(hipMemcpyAsync(st, dstot.p, sizeof(st), hipMemcpyDeviceToHost, s));
+    CZ_HIP(hipStreamSynchronize(s));
+    // The fused pass's tiles plus their escalated candidates against the rerun of all.
+    uint32_t h = st[1].hash;
+    uint64_t cnt = st[1].count, nc = st[1].numcases, ov = st[1].overflow;
+    for (const RefSpotRec& r : recs) {
+      h ^= r.hash;
+      cnt += r.count;
+      nc += r.numcases;
+      ov += r.overflow;
+    }
+    spot_checks += sp.cap;
+    if (h != st[0].hash || cnt != st[0].count || nc != st[0].numcases ||
+        ov != st[0].overflow || st[0].aborts || st[1].aborts) {
+      spot_failures += sp.cap;
+      if (spot_msg.empty())
+        spot_msg = "spot check failed: root " + std::to_string(root_index) + ", tiles " +
+                   std::to_string(sp.off) + " + j * " + std::to_string(sp.freq) + " (j < " +
+                   std::to_string(sp.cap) + "): fused pass and rerun differ";
+    }
+    return COLLATZ_OK;
+  };
   // The whole run at one word width of the fused pass: 96 bits (escalating to `limbs`
   // above 2^86) when the cutoff is at least 2^96, `limbs` otherwise.
   auto run_all = [&](int width) -> int {
     CZ_HIP(hipMemsetAsync(dbins.p, 0, 17 * 8, s));
     CZ_HIP(hipMemsetAsync(dtot.p, 0, sizeof(RefTotalsHost), s));
     leaves = 0;
-    for (const SieveRoot& sr : roots) {
+    spot_checks = spot_failures = 0;
+    spot_msg.clear();
+    for (size_t ri = 0; ri < roots.size(); ++ri) {
+      const SieveRoot& sr = roots[ri];
       LeafSet ls;
       int rc2 = build_leaves(sr, s, ls);
       if (rc2) return rc2;
@@ -293,13 +376,33 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
       lv.num = ls.num.p;
       lv.num_hi = ls.wide ? ls.num_hi.p : nullptr;
       lv.meta = ls.meta.p;
-      CZ_HIP(launch_ref_fused(width, rust, T->dev, lv, ls.count, mbi, lim, T->pow3w.p, 256, 5,
-                              nblocks, dtot.p, ovf_cap ? dovf.p : nullptr, ovf_cap, desc.p,
-                              esc_cap, s));
+      // Sampled tiles: t = off + j freq, off from a SplitMix64 step of the root's index.
+      RefSpot sp{};
+      const uint64_t ntiles = (ls.count + kTile - 1) / kTile;
+      if (freq) {
+        uint64_t z = 0x9E3779B97F4A7C15ull * (ri + 1);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        sp.freq = freq;
+        sp.off = (uint32_t)(z % freq);
+        sp.cap = sp.off < ntiles
+                     ? (uint32_t)std::min<uint64_t>(max_spot, (ntiles - 1 - sp.off) / freq + 1)
+                     : 0;
+        sp.rec = dspot.p;
+        if (!sp.cap) sp.freq = 0;
+      }
+      CZ_HIP(launch_ref_fused(width, rust, T->dev, lv, ls.count, mbi, lim, T->pow3w.p, kTile,
+                              5, nblocks, dtot.p, ovf_cap ? dovf.p : nullptr, ovf_cap, desc.p,
+                              esc_cap, sp, s));
       unsigned long long nesc = 0;
       CZ_HIP(hipMemcpyAsync(&nesc, &dtot.p->esc_count, 8, hipMemcpyDeviceToHost, s));
       CZ_HIP(hipStreamSynchronize(s));
       if (nesc > esc_cap) return fail(COLLATZ_ECAPACITY, "escalation list overflowed");
+      if (sp.cap) {
+        rc2 = spot_check(ri, lv, ls.count, sp, nesc);
+        if (rc2) return rc2;
+      }
       if (nesc) {
         // Candidates that left 96 bits: rerun from their start on two or three limbs.
         CZ_HIP(hipMemsetAsync(&dtot.p->tile_counter, 0, 4, s));
@@ -337,9 +440,15 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
   out->overflow_written = tot.overflow;
   if (ovf_cap) {
     const uint64_t nov = std::min<uint64_t>(tot.overflow, ovf_cap);
-    if (nov) CZ_HIP(hipMemcpy(out->overflow_starts, dovf.p, nov * 16, hipMemcpyDeviceToHost));
+    if (nov) {
+      CZ_HIP(hipMemcpyAsync(out->overflow_starts, dovf.p, nov * 16, hipMemcpyDeviceToHost, s));
+      CZ_HIP(hipStreamSynchronize(s));
+    }
   }
+  out->spot_checks = spot_checks;
+  out->spot_check_failures = spot_failures;
   out->device_ms = now_ms() - t0;
+  if (spot_failures) return fail(COLLATZ_EDEVICE, spot_msg);
   if (ovf_cap && tot.overflow > ovf_cap)
     return fail(COLLATZ_ECAPACITY, "overflow buffer smaller than the result");
   return COLLATZ_OK;

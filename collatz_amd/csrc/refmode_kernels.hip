@@ -14,7 +14,12 @@
 //                    hash, count and overflow count are folded with one atomic per wave and
 //                    tile.
 //   k_ref_iterate    Step 3 over a list of (leaf, a) candidates: the two- or three-limb rerun
-//                    of the candidates that left 96 bits in k_ref_fused.
+//                    of the candidates that left 96 bits in k_ref_fused, and of the
+//                    candidates of spot-checked tiles.
+//   k_ref_spot_list  the candidate list of the tiles a k_ref_fused launch sampled for a spot
+//                    check (one thread per mask word, no queue).
+#include <algorithm>
+
 #include "schedule.hpp"
 
 namespace cz {
@@ -384,7 +389,7 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, RefLeaves lv,
                                                    const uint64_t* pow3w, uint32_t tile,
                                                    uint32_t U, RefTotals* tot, uint64_t* ovf,
                                                    uint64_t ovf_cap, uint64_t* esc,
-                                                   uint64_t esc_cap) {
+                                                   uint64_t esc_cap, RefSpot spot) {
   __shared__ Pow3Lds p3s;
   __shared__ Step8Lds st8;
   __shared__ uint64_t p3w[2 * kPow3Max];
@@ -601,6 +606,9 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, RefLeaves lv,
         if (u + 1 < U && !act && pf) start_prefetched();
       }
     }
+    // A sampled tile (spot check) keeps its own results as well.  (t is uniform.)
+    const bool spot_tile = spot.freq != 0 && t % spot.freq == spot.off &&
+                           t / spot.freq < spot.cap;
     // One atomic per wave and tile.
 #pragma unroll
     for (int k = 32; k >= 1; k >>= 1) {
@@ -609,7 +617,7 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, RefLeaves lv,
       stall += __shfl_xor(stall, k, 64);
       lane_chk += __shfl_xor(lane_chk, k, 64);
       nc += __shfl_xor(nc, k, 64);
-      if (!ovf) ov += __shfl_xor(ov, k, 64);
+      if (!ovf || spot_tile) ov += __shfl_xor(ov, k, 64);
     }
     if (lane == 0) {
       atomicXor(&tot->hash, h);
@@ -619,6 +627,44 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, RefLeaves lv,
       if (stall) atomicAdd(&tot->stall, (unsigned long long)stall);
       if (!ovf && ov) atomicAdd(&tot->overflow, (unsigned long long)ov);
       if (aborted) atomicAdd(&tot->aborts, 1ull);
+      if (spot_tile) {
+        RefSpotRec r;
+        r.hash = h;
+        r.count = cnt;
+        r.numcases = nc;
+        r.overflow = ov;
+        spot.rec[t / spot.freq] = r;
+      }
+    }
+  }
+}
+
+// The candidates of the sampled tiles of a fused launch, in the entry format of the
+// escalation list (leaf | a << 32 | no-bit-vector << 48), for a rerun by k_ref_iterate: one
+// thread per (sampled tile, leaf, mask word), the words computed as k_ref_fused computes
+// them.  Entries past cap are counted but not written.
+__global__ __launch_bounds__(256) void k_ref_spot_list(RefTablesDev T, RefLeaves lv,
+                                                       uint64_t nleaves, uint32_t tile,
+                                                       RefSpot spot, uint64_t* list,
+                                                       unsigned long long* count, uint64_t cap) {
+  const uint64_t per_tile = (uint64_t)tile * T.nw;
+  const uint64_t total = per_tile * spot.cap;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t j = g / per_tile;
+    const uint64_t r = g - j * per_tile;
+    const uint64_t t = spot.off + j * spot.freq;
+    const uint64_t leaf = t * tile + r / T.nw;
+    if (leaf >= nleaves) continue;
+    const uint32_t w = (uint32_t)(r % T.nw);
+    const uint32_t meta = lv.meta[leaf];
+    uint32_t m = leaf_mask_word(T, lv.base[leaf], lv.num[leaf], meta, w);
+    const uint64_t strict = (meta_f(meta) - T.tpo) >= 16 ? 1ull : 0ull;
+    while (m) {
+      const uint32_t b = __builtin_ctz(m);
+      m &= m - 1u;
+      const unsigned long long i = atomicAdd(count, 1ull);
+      if (i < cap) list[i] = leaf | ((uint64_t)(32u * w + b) << 32) | (strict << 48);
     }
   }
 }
@@ -685,33 +731,44 @@ static void launch_fused_sched(int limbs, const RefTablesDev& T, const RefLeaves
                                const uint64_t* pow3w, uint32_t tile, uint32_t U,
                                uint32_t nblocks, RefTotals* tot, uint64_t* ovf,
                                uint64_t ovf_cap, uint64_t* esc, uint64_t esc_cap,
-                               hipStream_t s) {
+                               const RefSpot& sp, hipStream_t s) {
   if (limbs == 1)
     hipLaunchKernelGGL((k_ref_fused<3, false, RUST>), dim3(nblocks), dim3(256), 0, s, T, lv,
-                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, esc_cap);
+                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, esc_cap, sp);
   else if (limbs == 2)
     hipLaunchKernelGGL((k_ref_fused<4, false, RUST>), dim3(nblocks), dim3(256), 0, s, T, lv,
-                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, 0);
+                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, 0, sp);
   else if (lim.cutoff >= 128)
     hipLaunchKernelGGL((k_ref_fused<6, false, RUST>), dim3(nblocks), dim3(256), 0, s, T, lv,
-                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, 0);
+                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, 0, sp);
   else
     hipLaunchKernelGGL((k_ref_fused<6, true, RUST>), dim3(nblocks), dim3(256), 0, s, T, lv,
-                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, 0);
+                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, 0, sp);
 }
 
 hipError_t launch_ref_fused(int limbs, bool rust, const RefTablesDev& T, const RefLeaves& lv,
                             uint64_t nleaves, uint32_t mbi, const Limits& lim,
                             const uint64_t* pow3w, uint32_t tile, uint32_t U, uint32_t nblocks,
                             void* tot, uint64_t* ovf, uint64_t ovf_cap, uint64_t* esc,
-                            uint64_t esc_cap, hipStream_t s) {
+                            uint64_t esc_cap, const RefSpot& spot, hipStream_t s) {
   if (!nleaves) return hipSuccess;
   if (rust)
     launch_fused_sched<true>(limbs, T, lv, nleaves, mbi, lim, pow3w, tile, U, nblocks,
-                             (RefTotals*)tot, ovf, ovf_cap, esc, esc_cap, s);
+                             (RefTotals*)tot, ovf, ovf_cap, esc, esc_cap, spot, s);
   else
     launch_fused_sched<false>(limbs, T, lv, nleaves, mbi, lim, pow3w, tile, U, nblocks,
-                              (RefTotals*)tot, ovf, ovf_cap, esc, esc_cap, s);
+                              (RefTotals*)tot, ovf, ovf_cap, esc, esc_cap, spot, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_spot_list(const RefTablesDev& T, const RefLeaves& lv, uint64_t nleaves,
+                                uint32_t tile, const RefSpot& spot, uint64_t* list,
+                                unsigned long long* count, uint64_t cap, hipStream_t s) {
+  if (!spot.cap || !nleaves) return hipSuccess;
+  const uint64_t threads = (uint64_t)tile * T.nw * spot.cap;
+  const uint32_t nb = (uint32_t)std::min<uint64_t>(4096, (threads + 255) / 256);
+  hipLaunchKernelGGL(k_ref_spot_list, dim3(nb), dim3(256), 0, s, T, lv, nleaves, tile, spot,
+                     list, count, cap);
   return hipGetLastError();
 }
 

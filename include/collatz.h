@@ -86,7 +86,7 @@ int collatz_sieve_free(collatz_sieve* sieve);
  * is in {0,1,3,6,7} is iterated with the reference schedule (collatz_cudav7.cu:280-366)
  * from T^k(n)+1 (Lemma 2.3) until the check number < n passes. */
 typedef struct {
-  uint64_t chunk_index;    /* j */
+  uint64_t chunk_index;    /* j (its low 64 bits, for chunks beyond 2^(64 + C)) */
   uint64_t n_checked;      /* integers of the range in this chunk */
   uint64_t count;          /* candidates iterated */
   uint64_t stop_sum;       /* sum over candidates of J = k + T-steps to the passing check */
@@ -111,6 +111,8 @@ typedef struct {
   double device_ms;           /* device time of the launch: the kernels that set up, verify
                                  and reduce (inputs already in device memory) plus the
                                  three-limb pass of escalated candidates */
+  uint64_t spot_checks;       /* chunk records of this result re-run by the other kernel form
+                                 and found equal (collatz_batch_opts.spot_checks) */
 } collatz_range_result;
 
 enum collatz_form {
@@ -166,7 +168,10 @@ typedef struct {
   uint32_t tile_residues;    /* residues per work tile; 0 = autotuned */
   uint32_t waves_per_cu;     /* persistent waves per CU; 0 = autotuned */
   uint32_t checks_per_refill;/* checks between lane refills; 0 = autotuned */
-  uint32_t reserved;
+  uint32_t spot_checks;      /* chunk records (seeded choice among the stored ones) re-run by
+                                the plain kernel after the launch and compared field by field
+                                (the reference's spot check, collatz_cudav7.cu:619-635); a
+                                difference returns COLLATZ_EDEVICE with the outputs filled in */
 } collatz_batch_opts;
 
 int collatz_verify_batch(const collatz_sieve* sieve, uint64_t n,
@@ -193,7 +198,12 @@ typedef struct {
   uint32_t cutoff_log2;  /* CUTOFFPOWER, 64..187 (main.rs semantics: 64..180, main.rs uses
                             120: the threshold of its (number >> 64) >= 2^56 test) */
   int32_t device;
-  uint32_t reserved;
+  uint32_t spot_check_freq; /* 0: none.  Else spot checks (collatz_cudav7.cu:619-635): one
+                               tile of 256 leaves in every spot_check_freq, at a seeded offset,
+                               is re-run by a second kernel and its hash, count, block tally
+                               and overflow count compared (at most 2^23 candidates per root
+                               launch).  The reference checks one launch of 2^16 leaves in
+                               10000 (SPOTCHECKFREQUENCY, :55), i.e. one tile in 2,560,000. */
 } collatz_ref_params;
 
 /* Under COLLATZ_SEM_RUST Step 3 is main.rs's own search3 (main.rs:321-341): each candidate
@@ -215,6 +225,10 @@ typedef struct {
   uint64_t overflow_written;
   double device_ms;
   uint64_t numcases;          /* main.rs semantics: its "numcases"; 0 otherwise */
+  uint64_t spot_checks;       /* tiles re-run and compared ("Spot checks = %u") */
+  uint64_t spot_check_failures; /* of them, tiles whose rerun differed ("Spot check failed");
+                                   the call then returns COLLATZ_EDEVICE with every output
+                                   filled in */
 } collatz_ref_result;
 
 int collatz_reference_run(const collatz_ref_params* params, const collatz_node* root,
