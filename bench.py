@@ -153,6 +153,93 @@ def cpu_baseline(intervals, threads: int, n_intervals: int, width_log2: int):
             "cpu_model": model, "host_cpus": os.cpu_count()}
 
 
+C4_LO, C4_WIDTH_LOG2, C4_CHUNK, C4_K = 1 << 60, 44, 30, 37
+
+
+def total_fields_equal(cpu_total, gpu_chunk) -> bool:
+    """A chunk record's results (the CPU total of its 2^26-wide records against the GPU's
+    record of the whole chunk): everything but the index."""
+    import numpy as np
+    keys = ("n_checked", "count", "stop_sum", "stop_xor", "max_exc", "max_start",
+            "overflow_count", "hash", "flags")
+    return all(np.array_equal(cpu_total[k], gpu_chunk[k]) for k in keys)
+
+
+def cpu_configs(threads, gpu):
+    """The oracle on the host cores for the other configurations, each beside the GPU figure
+    of the same workload, with the records compared:
+      c0  [1, 2^32), k = 20, 2^24-wide records: in full;
+      c1  [2^40, 2^40 + 2^36), k = 32, 2^24-wide records: in full;
+      c3  the first requests of the served stream (seed 1) up to 2^36 integers, k = 32: a
+          prefix, so the rate is an extrapolation to the stream;
+      c4  32 of the 16384 2^30-wide chunks of [2^60, 2^60 + 2^44), k = 37 (8 chunks in each
+          of 4 seeded residue windows, from the oracle's windowed sieve): extrapolated.
+    `gpu` holds the GPU side: c0 and c1 chunk records, c4 chunk records, the sieve handle
+    for c3 and the GPU rates."""
+    import random
+
+    from oracle import bind as B
+    from collatz_amd import workloads as W
+    out = {}
+
+    def entry(n, dt, sample, total_n, same, gpu_rate):
+        e = {"value": n / dt, "unit": "n/s", "cores": threads, "kind": "port",
+             "sample": sample, "integers": n, "seconds": dt, "bit_exact_vs_gpu": same,
+             "gpu_n_per_s": gpu_rate}
+        if total_n != n:
+            e["extrapolated_full_run_s"] = total_n / (n / dt)
+        return e
+
+    s20 = B.sieve(20)
+    t0 = time.time()
+    ch, _ = B.verify_range(s20, 1, 1 << 32, 24, threads=threads)
+    dt = time.time() - t0
+    out["c0"] = entry((1 << 32) - 1, dt, "in full: [1, 2^32), k = 20, 256 records of 2^24",
+                      (1 << 32) - 1, records_equal(ch, gpu["c0_chunks"]), gpu["c0_rate"])
+    del s20
+    s32 = B.sieve(K)
+    t0 = time.time()
+    ch, _ = B.verify_range(s32, RANGE_LO, RANGE_LO + RANGE_WIDTH, RANGE_CHUNK, threads=threads)
+    dt = time.time() - t0
+    out["c1"] = entry(RANGE_WIDTH, dt, "in full: [2^40, 2^40+2^36), k = 32, 4096 records of "
+                      "2^24", RANGE_WIDTH, records_equal(ch, gpu["c1_chunks"]), gpu["c1_rate"])
+    reqs = W.request_stream(1, 20000)
+    sample, n = [], 0
+    for r in reqs:
+        if n >= 1 << 36:
+            break
+        sample.append(r)
+        n += r[2] - r[1]
+    t0 = time.time()
+    cpu = [B.verify_range(s32, lo, hi, CPU_CHUNK, threads=threads)[1] for _, lo, hi in sample]
+    dt = time.time() - t0
+    same = True
+    for (_, lo, hi), tot in zip(sample, cpu):
+        g = gpu["sieve32"].verify_range(lo, hi, 40).total
+        same = same and total_fields_equal(tot, g)
+    out["c3"] = entry(n, dt, f"the first {len(sample)} requests of the served stream (seed 1), "
+                      "k = 32, 2^26-wide records", sum(hi - lo for _, lo, hi in reqs), same,
+                      gpu["c3_rate"])
+    del s32
+    rng = random.Random(4)
+    windows = rng.sample(range(1 << (C4_K - C4_CHUNK)), 4)
+    per_q = 1 << (C4_WIDTH_LOG2 - C4_K)
+    chunks = [(q << (C4_K - C4_CHUNK)) + w for w in windows for q in rng.sample(range(per_q), 8)]
+    t0 = time.time()
+    sw = B.sieve_windows(C4_K, C4_CHUNK, windows, threads=threads)
+    build_s = time.time() - t0
+    t0 = time.time()
+    cpu = [B.verify_range(sw, C4_LO + (j << C4_CHUNK), C4_LO + ((j + 1) << C4_CHUNK), CPU_CHUNK,
+                          threads=threads)[1] for j in chunks]
+    dt = time.time() - t0
+    same = all(total_fields_equal(t, gpu["c4_chunks"][j]) for t, j in zip(cpu, chunks))
+    out["c4"] = entry(len(chunks) << C4_CHUNK, dt,
+                      f"32 of the 16384 2^30-wide chunks of [2^60, 2^60+2^44), k = 37: 8 in "
+                      f"each of the residue windows {sorted(windows)} (sieve of those windows "
+                      f"{build_s:.1f} s, excluded)", 1 << C4_WIDTH_LOG2, same, gpu["c4_rate"])
+    return out
+
+
 def served_leg(sv, reqs, rate, chunk_log2, check_n=0):
     """Replays `reqs` (gap, lo, hi) at `rate` requests/s (rate None: all at once).  The
     results of the first `check_n` requests are compared with the plain kernel's records of
@@ -347,9 +434,10 @@ def tree_leg(cz, N):
     return tree
 
 
-def range_leg(cz, sv, steps, forms=("plain", "persistent")):
+def range_leg(cz, sv, steps, forms=("plain", "persistent"), keep=None):
     """The single range [2^40, 2^40 + 2^36) with the k = 32 sieve and 2^24-wide chunk
-    records, in the plain drop-in form and the persistent form of collatz_verify_range."""
+    records, in the plain drop-in form and the persistent form of collatz_verify_range
+    (keep, a dict, receives the plain form's chunk records)."""
     rng = {}
     by_name = {"plain": cz.FORM_PLAIN, "persistent": cz.FORM_PERSISTENT}
     for name in forms:
@@ -371,12 +459,15 @@ def range_leg(cz, sv, steps, forms=("plain", "persistent")):
                      "roofline_frac": ach / VALU_PEAK_TOPS,
                      "hash": "%08X" % int(o.total["hash"])}
         log(f"range, {name}: {rng[name]}")
+        if keep is not None and name == "plain":
+            keep["c1_chunks"] = o.chunks.copy()
+            keep["c1_rate"] = rng[name]["device_n_per_s"]
     return rng
 
 
-def large_leg(cz, steps):
+def large_leg(cz, steps, keep=None):
     """Large sieve: [2^60, 2^60 + 2^44) with k = 37 (|S_37| records resident), 2^30-wide
-    chunks, batched form."""
+    chunks, batched form (keep, a dict, receives its chunk records)."""
     t = time.time()
     s37 = cz.Sieve(37)
     build_s = time.time() - t
@@ -399,6 +490,32 @@ def large_leg(cz, steps):
            "sieve_build_s": build_s, "escalated": int(o.escalated),
            "hash": "%08X" % int(o.total["hash"]), "roofline_frac": ach / VALU_PEAK_TOPS}
     log(f"large sieve k=37: {out}")
+    if keep is not None:
+        keep["c4_chunks"] = o.chunks.copy()
+        keep["c4_rate"] = out["n_per_s"]
+    return out
+
+
+def c0_leg(cz, steps, keep=None):
+    """[1, 2^32) with the k = 20 sieve and 2^24-wide records (the CPU-runnable
+    configuration) in the plain drop-in form (keep, a dict, receives its records)."""
+    s20 = cz.Sieve(20)
+    o = s20.verify_range(1, 1 << 32, 24)
+    ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o = s20.verify_range(1, 1 << 32, 24)
+        ms.append(o.device_ms)
+    d = (time.perf_counter() - t0) / steps
+    dms = sum(ms) / len(ms)
+    out = {"range": "[1, 2^32)", "k": 20, "chunk_log2": 24, "form": "plain",
+           "device_ms": dms, "ms_per_call": d * 1e3, "n_per_s": ((1 << 32) - 1) / (dms * 1e-3),
+           "host_inclusive_n_per_s": ((1 << 32) - 1) / d, "candidates": int(o.total["count"]),
+           "hash": "%08X" % int(o.total["hash"])}
+    if keep is not None:
+        keep["c0_chunks"] = o.chunks.copy()
+        keep["c0_rate"] = out["n_per_s"]
+    log(f"c0: {out}")
     return out
 
 
@@ -598,11 +715,13 @@ def main():
             f"{value_host / plain_rate:.2f} ({value / plain_dev_rate:.2f} by device time), "
             f"identical records {same}")
 
+    keep = {"sieve32": sv}
     if not args.no_range:
-        line["range_2p40_w2p36"] = range_leg(cz, sv, args.range_steps)
+        line["range_2p40_w2p36"] = range_leg(cz, sv, args.range_steps, keep=keep)
+        line["range_1_2p32_k20"] = c0_leg(cz, args.range_steps, keep=keep)
 
     if not args.no_large:
-        line["large_sieve_k37"] = large_leg(cz, args.large_steps)
+        line["large_sieve_k37"] = large_leg(cz, args.large_steps, keep=keep)
 
     if not args.no_lookahead:
         line["lookahead"] = lookahead_leg(cz, ivs, total_n, args.steps)
@@ -647,6 +766,7 @@ def main():
                             "2^32-wide records; each rate replays the first `requests` of the "
                             "stream, saturated submits all of them at once")
         line["served"] = served
+        keep["c3_rate"] = served["saturated"]["n_per_s"]
         line["bit_exact_served_vs_plain"] = all(
             v["bit_exact_vs_plain"] for v in served.values() if isinstance(v, dict))
 
@@ -654,6 +774,9 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(ivs, threads, args.cpu_intervals, 32)
         log(f"cpu baseline: {line['cpu_baseline']['value']:.3e} n/s")
+        if all(x in keep for x in ("c0_chunks", "c1_chunks", "c3_rate", "c4_chunks")):
+            line["cpu_baseline"]["per_config"] = cpu_configs(threads, keep)
+            log(f"cpu baselines per configuration: {line['cpu_baseline']['per_config']}")
     print(json.dumps(line), flush=True)
 
 

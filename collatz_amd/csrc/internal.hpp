@@ -137,10 +137,13 @@ struct Limits {
   // path from the host).
   uint32_t max_checks;
   uint32_t max_steps;
-  // Iterations of a persistent kernel's work loop per tile (and of its take loop per
+  // Iterations of a kernel's work loop per tile or wave share (and of a take loop per
   // refill): kMaxLoop, or less under COLLATZ_LOOP_LIMIT (a test hook).  Every such loop makes
   // progress on each iteration, so real work never comes near the bound; a wave that reaches
-  // it abandons its tile, counts the loop abort, and exits normally, and the call fails.
+  // it abandons its work, counts the loop abort, and exits normally, and the call fails.  The
+  // plain kernel and the look-ahead form of the persistent kernel always carry the bound;
+  // the persistent kernel's plain form and the fused reference-mode kernel carry it only
+  // when it is below kMaxLoop (their counters cost 6% and an occupancy step respectively).
   uint32_t max_loop;
 };
 
@@ -180,8 +183,9 @@ struct RefLeaves {
 
 // Spot checks of a reference-mode launch (the reference re-runs a random launch's
 // candidates on the CPU and compares the hash, collatz_cudav7.cu:619-635): tiles
-// t = off + j freq with j < cap record their own results in rec[j], and the host compares
-// them with a rerun of the same tiles' candidates by the list kernel.
+// t = off + j freq with j < cap (freq a power of two) record their own results in rec[j],
+// and the host compares them with a rerun of the same tiles' candidates by the list kernel.
+// The fused kernel reads these from its totals block in device memory.
 struct RefSpotRec {
   uint32_t hash, count, numcases, overflow;
 };
@@ -272,7 +276,7 @@ hipError_t launch_ref_fused(int limbs, bool rust, const RefTablesDev& T, const R
                             uint64_t nleaves, uint32_t mbi, const Limits& lim,
                             const uint64_t* pow3w, uint32_t tile, uint32_t U, uint32_t nblocks,
                             void* tot, uint64_t* ovf, uint64_t ovf_cap, uint64_t* esc,
-                            uint64_t esc_cap, const RefSpot& spot, hipStream_t s);
+                            uint64_t esc_cap, hipStream_t s);
 // Lists the candidates of the sampled tiles (tile leaves each) of a fused launch as
 // (leaf | a << 32 | no-bit-vector << 48) entries, appending at *count (at most cap kept).
 hipError_t launch_ref_spot_list(const RefTablesDev& T, const RefLeaves& lv, uint64_t nleaves,

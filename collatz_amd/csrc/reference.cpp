@@ -33,6 +33,8 @@ struct RefTotalsHost {
   unsigned long long checks;
   unsigned long long aborts;
   unsigned long long numcases;
+  unsigned long long spot_rec;
+  unsigned int spot_mask, spot_off, spot_last, spot_shift;
 };
 
 struct RefTables {
@@ -279,7 +281,9 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
   // pass leaves to the wide pass).
   constexpr uint32_t kTile = 256;
   constexpr uint64_t kSpotEntries = 1ull << 23;
-  const uint32_t freq = params->spot_check_freq;
+  // The stride: the largest power of two not above spot_check_freq.
+  const uint32_t freq = params->spot_check_freq
+                            ? 1u << (31 - __builtin_clz(params->spot_check_freq)) : 0u;
   const uint32_t max_spot = (uint32_t)(kSpotEntries / ((uint64_t)kTile << params->A));
   DevBuf<RefSpotRec> dspot;
   DevBuf<uint64_t> dlist, dsel;
@@ -392,9 +396,25 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
         sp.rec = dspot.p;
         if (!sp.cap) sp.freq = 0;
       }
+      {
+        // The fused pass reads its spot-check fields from the totals block.
+        struct {
+          unsigned long long rec;
+          unsigned int mask, off, last, shift;
+        } f = {0, 0, 0, 0, 0};
+        if (sp.cap) {
+          f.rec = (unsigned long long)(uintptr_t)sp.rec;
+          f.mask = sp.freq - 1;
+          f.off = sp.off;
+          f.last = sp.off + (sp.cap - 1) * sp.freq;
+          f.shift = (uint32_t)__builtin_ctz(sp.freq);
+        }
+        static_assert(sizeof(f) == 24, "spot fields");
+        CZ_HIP(hipMemcpyAsync(&dtot.p->spot_rec, &f, sizeof(f), hipMemcpyHostToDevice, s));
+      }
       CZ_HIP(launch_ref_fused(width, rust, T->dev, lv, ls.count, mbi, lim, T->pow3w.p, kTile,
                               5, nblocks, dtot.p, ovf_cap ? dovf.p : nullptr, ovf_cap, desc.p,
-                              esc_cap, sp, s));
+                              esc_cap, s));
       unsigned long long nesc = 0;
       CZ_HIP(hipMemcpyAsync(&nesc, &dtot.p->esc_count, 8, hipMemcpyDeviceToHost, s));
       CZ_HIP(hipStreamSynchronize(s));

@@ -158,6 +158,11 @@ struct RefTotals {
   unsigned long long checks;     // checks run (counted per wave from its active-lane mask)
   unsigned long long aborts;     // tiles abandoned at the work-loop bound (Limits::max_loop)
   unsigned long long numcases;   // main.rs semantics: 16-step blocks of finished candidates
+  // Spot checks of a fused launch, set by the host: tiles t with t & spot_mask == spot_off
+  // and t <= spot_last write their own results to spot_rec[t >> spot_shift] (spot_rec 0:
+  // none).  Read from memory at the end of a tile, so they take no registers in the loop.
+  unsigned long long spot_rec;
+  unsigned int spot_mask, spot_off, spot_last, spot_shift;
 };
 
 // num >= 2^c, or (strict) num > 2^c.
@@ -383,13 +388,24 @@ __global__ __launch_bounds__(256) void k_ref_iterate(const uint64_t* cands, uint
 // below it); main.rs's blocks test their cutoff on 96 bits when it is below 2^86.
 constexpr uint32_t kRefQueue = 512;  // queue entries per wave: (leaf, a)
 
-template <int L, bool LOWCUT, bool RUST>
-__global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, RefLeaves lv,
+// WIDE: leaf numbers or 3^f may exceed 64 bits (N - A > 40); otherwise a start is
+// number + a 3^f from 64-bit words, with no high words loaded.
+// BOUNDED: the tile loop and the take loop carry the host-set iteration bound
+// (Limits::max_loop), only when a bound below the default is set (COLLATZ_LOOP_LIMIT): its
+// counters take scalar registers that the kernel needs for eight waves per SIMD.  Without
+// it the loops end: every iteration runs a check on each running candidate (whose check
+// count ends it at the check bound), starts prefetched candidates, takes queue entries
+// (the queue head advances), expands mask words (the word cursor advances by at least 16),
+// or leaves; the exit tests are ballots (uniform).
+// Eight waves per SIMD (the scalar registers this asks for are what the kernel had before
+// its deep-leaf, main.rs and spot-check paths; without the limit it took seven).
+template <int L, bool LOWCUT, bool RUST, bool WIDE, bool BOUNDED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 3 && !RUST ? 8 : 1))) void k_ref_fused(RefTablesDev T, RefLeaves lv,
                                                    uint64_t nleaves, uint32_t mbi, Limits lim,
                                                    const uint64_t* pow3w, uint32_t tile,
                                                    uint32_t U, RefTotals* tot, uint64_t* ovf,
                                                    uint64_t ovf_cap, uint64_t* esc,
-                                                   uint64_t esc_cap, RefSpot spot) {
+                                                   uint64_t esc_cap) {
   __shared__ Pow3Lds p3s;
   __shared__ Step8Lds st8;
   __shared__ uint64_t p3w[2 * kPow3Max];
@@ -438,7 +454,8 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, RefLeaves lv,
     auto start_prefetched = [&]() {
       pf = false;
       const uint32_t f = meta_f(pf_meta);
-      const unsigned __int128 x = ref_start(pf_ln, pf_lnh, pf_a, p3w, f);
+      const unsigned __int128 x = WIDE ? ref_start(pf_ln, pf_lnh, pf_a, p3w, f)
+                                       : (unsigned __int128)pf_a * p3w[2 * f] + pf_ln;
       set_words<L>(v, x);
       n_lo = pf_lb + ((uint64_t)pf_a << mbi);
       // Leaves without a bit vector run on the CPU in the reference (search2/search3,
@@ -460,7 +477,7 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, RefLeaves lv,
     uint32_t iters = 0;
     bool aborted = false;
     for (;;) {
-      if (__builtin_expect(++iters > lim.max_loop, 0)) {
+      if (BOUNDED && __builtin_expect(++iters > lim.max_loop, 0)) {
         aborted = true;
         break;
       }
@@ -475,7 +492,7 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, RefLeaves lv,
                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
         uint32_t takes = 0;
         for (;;) {
-          if (__builtin_expect(++takes > lim.max_loop, 0)) {
+          if (BOUNDED && __builtin_expect(++takes > lim.max_loop, 0)) {
             aborted = true;
             break;
           }
@@ -488,7 +505,7 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, RefLeaves lv,
               pf_a = e.y;
               pf_lb = lv.base[e.x];
               pf_ln = lv.num[e.x];
-              pf_lnh = lv.num_hi ? lv.num_hi[e.x] : 0;
+              if constexpr (WIDE) pf_lnh = lv.num_hi ? lv.num_hi[e.x] : 0;
               pf_meta = lv.meta[e.x];
               pf = true;
             } else {
@@ -607,8 +624,9 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, RefLeaves lv,
       }
     }
     // A sampled tile (spot check) keeps its own results as well.  (t is uniform.)
-    const bool spot_tile = spot.freq != 0 && t % spot.freq == spot.off &&
-                           t / spot.freq < spot.cap;
+    RefSpotRec* const spot_rec = (RefSpotRec*)tot->spot_rec;
+    const bool spot_tile = spot_rec != nullptr && (t & tot->spot_mask) == tot->spot_off &&
+                           t <= tot->spot_last;
     // One atomic per wave and tile.
 #pragma unroll
     for (int k = 32; k >= 1; k >>= 1) {
@@ -633,7 +651,7 @@ __global__ __launch_bounds__(256) void k_ref_fused(RefTablesDev T, RefLeaves lv,
         r.count = cnt;
         r.numcases = nc;
         r.overflow = ov;
-        spot.rec[t / spot.freq] = r;
+        spot_rec[t >> tot->spot_shift] = r;
       }
     }
   }
@@ -725,39 +743,52 @@ hipError_t launch_ref_iterate(int limbs, bool rust, const uint64_t* cands, uint6
   return hipGetLastError();
 }
 
-template <bool RUST>
+template <bool RUST, bool WIDE, bool BOUNDED>
 static void launch_fused_sched(int limbs, const RefTablesDev& T, const RefLeaves& lv,
                                uint64_t nleaves, uint32_t mbi, const Limits& lim,
                                const uint64_t* pow3w, uint32_t tile, uint32_t U,
                                uint32_t nblocks, RefTotals* tot, uint64_t* ovf,
                                uint64_t ovf_cap, uint64_t* esc, uint64_t esc_cap,
-                               const RefSpot& sp, hipStream_t s) {
+                               hipStream_t s) {
+  const dim3 grid(nblocks), block(256);
   if (limbs == 1)
-    hipLaunchKernelGGL((k_ref_fused<3, false, RUST>), dim3(nblocks), dim3(256), 0, s, T, lv,
-                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, esc_cap, sp);
+    hipLaunchKernelGGL((k_ref_fused<3, false, RUST, WIDE, BOUNDED>), grid, block, 0, s, T, lv,
+                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, esc_cap);
   else if (limbs == 2)
-    hipLaunchKernelGGL((k_ref_fused<4, false, RUST>), dim3(nblocks), dim3(256), 0, s, T, lv,
-                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, 0, sp);
+    hipLaunchKernelGGL((k_ref_fused<4, false, RUST, WIDE, BOUNDED>), grid, block, 0, s, T, lv,
+                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, 0);
   else if (lim.cutoff >= 128)
-    hipLaunchKernelGGL((k_ref_fused<6, false, RUST>), dim3(nblocks), dim3(256), 0, s, T, lv,
-                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, 0, sp);
+    hipLaunchKernelGGL((k_ref_fused<6, false, RUST, WIDE, BOUNDED>), grid, block, 0, s, T, lv,
+                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, 0);
   else
-    hipLaunchKernelGGL((k_ref_fused<6, true, RUST>), dim3(nblocks), dim3(256), 0, s, T, lv,
-                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, 0, sp);
+    hipLaunchKernelGGL((k_ref_fused<6, true, RUST, WIDE, BOUNDED>), grid, block, 0, s, T, lv,
+                       nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, 0);
 }
 
 hipError_t launch_ref_fused(int limbs, bool rust, const RefTablesDev& T, const RefLeaves& lv,
                             uint64_t nleaves, uint32_t mbi, const Limits& lim,
                             const uint64_t* pow3w, uint32_t tile, uint32_t U, uint32_t nblocks,
                             void* tot, uint64_t* ovf, uint64_t ovf_cap, uint64_t* esc,
-                            uint64_t esc_cap, const RefSpot& spot, hipStream_t s) {
+                            uint64_t esc_cap, hipStream_t s) {
   if (!nleaves) return hipSuccess;
-  if (rust)
-    launch_fused_sched<true>(limbs, T, lv, nleaves, mbi, lim, pow3w, tile, U, nblocks,
-                             (RefTotals*)tot, ovf, ovf_cap, esc, esc_cap, spot, s);
-  else
-    launch_fused_sched<false>(limbs, T, lv, nleaves, mbi, lim, pow3w, tile, U, nblocks,
-                              (RefTotals*)tot, ovf, ovf_cap, esc, esc_cap, spot, s);
+  RefTotals* t = (RefTotals*)tot;
+  const int wide = lv.num_hi != nullptr || mbi > kWalk64MaxDepth ? 1 : 0;
+  const int bounded = lim.max_loop < kMaxLoop ? 1 : 0;
+  const int sel = (rust ? 4 : 0) | (wide ? 2 : 0) | bounded;
+#define CZ_FUSED(R, W, B)                                                                 \
+  launch_fused_sched<R, W, B>(limbs, T, lv, nleaves, mbi, lim, pow3w, tile, U, nblocks, t, \
+                              ovf, ovf_cap, esc, esc_cap, s)
+  switch (sel) {
+    case 0: CZ_FUSED(false, false, false); break;
+    case 1: CZ_FUSED(false, false, true); break;
+    case 2: CZ_FUSED(false, true, false); break;
+    case 3: CZ_FUSED(false, true, true); break;
+    case 4: CZ_FUSED(true, false, false); break;
+    case 5: CZ_FUSED(true, false, true); break;
+    case 6: CZ_FUSED(true, true, false); break;
+    default: CZ_FUSED(true, true, true); break;
+  }
+#undef CZ_FUSED
   return hipGetLastError();
 }
 

@@ -520,7 +520,16 @@ struct LaStage {
   uint32_t off[64];
 };
 
-template <int L, bool LOWCUT, bool LA>
+// BOUNDED: the work loop carries the host-set iteration bound (Limits::max_loop).  The
+// look-ahead form always does; the plain form only when a bound below the default is set
+// (COLLATZ_LOOP_LIMIT), because the bound's counter costs 6% of its time (the loop is
+// issue-bound, and in four forms tried the counter cost 5-12%), and the loop ends without
+// it: every iteration either runs a check (which adds at least two T-steps to every lane's
+// step count, so a running candidate is abandoned at the step bound), starts the
+// prefetched records of idle lanes (clearing their prefetch bits), takes survivors (the
+// tile cursor advances), or leaves the loop; the masks, the cursor and the exit test are
+// uniform (scalar registers, scalar branches).
+template <int L, bool LOWCUT, bool LA, bool BOUNDED = LA>
 __global__ __launch_bounds__(256) void k_verify_persistent(
     SieveArrays sv, const RowDesc* rows, const TileDesc* tiles, Ctl* ctl, Limits lim,
     const uint64_t* pow3, DevRec* tile_recs, DeviceLists lists, uint32_t U) {
@@ -588,15 +597,16 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
     if constexpr (LA) {
       if (lane < n) nx = tile_rec[lane];
     }
-    // Iterations of the tile's loop (and of the take loop per refill), uniform: a wave
-    // that reaches the bound abandons the tile; the abort is counted and the call fails.
+    // Iterations of the tile's loop (BOUNDED): counted in a vector register (a scalar
+    // counter in this loop costs more) and tested with the step bound, once per refill.  A
+    // wave past the bound treats every running candidate as abandoned, drops its
+    // prefetches and the rest of the tile, and counts the abort; the call then fails.  The
+    // take loop of the look-ahead form has its own bound.
     uint32_t iters = 0;
+    if constexpr (BOUNDED) asm volatile("" : "+v"(iters));
     bool aborted = false;
     for (;;) {
-      if (__builtin_expect(++iters > lim.max_loop, 0)) {
-        aborted = true;
-        break;
-      }
+      if constexpr (BOUNDED) ++iters;
       // Starts the prefetched candidates of the lanes in S (no running candidate, a
       // prefetched record); a start that cannot run on these words is escalated.
       auto start_prefetched = [&](uint64_t S) {
@@ -632,7 +642,7 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
           uint64_t open = need;
           uint32_t takes = 0;
           for (;;) {
-            if (__builtin_expect(++takes > lim.max_loop, 0)) {
+            if (BOUNDED && __builtin_expect(++takes > lim.max_loop, 0)) {
               aborted = true;
               break;
             }
@@ -687,9 +697,21 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
       if ((actm | pfm) == 0) break;
       // A candidate past the step bound is abandoned (tested once per refill: it may run up
       // to U checks beyond the bound first).
-      const uint64_t stalled = ballot(c.steps >= lim.max_steps) & actm;
+      uint64_t stalled;
+      if constexpr (BOUNDED) {
+        const uint32_t pen = iters > lim.max_loop ? lim.max_steps : 0u;
+        stalled = ballot(c.steps + pen >= lim.max_steps) & actm;
+      } else {
+        stalled = ballot(c.steps >= lim.max_steps) & actm;
+      }
       if (__builtin_expect(stalled != 0, 0)) {
-        if (in_mask(stalled)) push_stall(lists, slot);
+        if (BOUNDED && __builtin_amdgcn_readfirstlane(iters) > lim.max_loop) {
+          if (lane == 0) atomicAdd(lists.loop_aborts, 1ull);
+          pfm = 0;
+          cur = n;
+        } else if (in_mask(stalled)) {
+          push_stall(lists, slot);
+        }
         actm &= ~stalled;
       }
       // (3) U checks; a lane whose candidate ends before the last of them starts its
@@ -724,7 +746,7 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
         }
       }
     }
-    if (__builtin_expect(aborted, 0)) {
+    if (BOUNDED && __builtin_expect(aborted, 0)) {
       if (lane == 0) atomicAdd(lists.loop_aborts, 1ull);
     }
     DevRec r = acc_to_rec<L>(acc);
@@ -816,27 +838,40 @@ uint32_t persistent_waves_fit(int limbs, uint32_t cutoff, bool la) {
   return v;
 }
 
-template <bool LA>
-static void launch_persistent_la(int limbs, const SieveArrays& sv, const RowDesc* rows,
+template <bool LA, bool BOUNDED>
+static void launch_persistent_lb(int limbs, const SieveArrays& sv, const RowDesc* rows,
                                  const TileDesc* tiles, Ctl* ctl, uint32_t nblocks, uint32_t U,
                                  const Limits& lim, const uint64_t* pow3, DevRec* tile_recs,
                                  DeviceLists lists, hipStream_t s) {
   const dim3 grid(nblocks), block(256);
   if (limbs == 1)
-    hipLaunchKernelGGL((k_verify_persistent<3, false, LA>), grid, block, 0, s, sv, rows, tiles,
-                       ctl, lim, pow3, tile_recs, lists, U);
+    hipLaunchKernelGGL((k_verify_persistent<3, false, LA, BOUNDED>), grid, block, 0, s, sv,
+                       rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
   else if (limbs == 2 && lim.cutoff >= kWideLog2)
-    hipLaunchKernelGGL((k_verify_persistent<4, false, LA>), grid, block, 0, s, sv, rows, tiles,
-                       ctl, lim, pow3, tile_recs, lists, U);
+    hipLaunchKernelGGL((k_verify_persistent<4, false, LA, BOUNDED>), grid, block, 0, s, sv,
+                       rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
   else if (limbs == 2)
-    hipLaunchKernelGGL((k_verify_persistent<4, true, LA>), grid, block, 0, s, sv, rows, tiles,
-                       ctl, lim, pow3, tile_recs, lists, U);
+    hipLaunchKernelGGL((k_verify_persistent<4, true, LA, BOUNDED>), grid, block, 0, s, sv,
+                       rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
   else if (lim.cutoff >= 128)
-    hipLaunchKernelGGL((k_verify_persistent<6, false, LA>), grid, block, 0, s, sv, rows, tiles,
-                       ctl, lim, pow3, tile_recs, lists, U);
+    hipLaunchKernelGGL((k_verify_persistent<6, false, LA, BOUNDED>), grid, block, 0, s, sv,
+                       rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
   else
-    hipLaunchKernelGGL((k_verify_persistent<6, true, LA>), grid, block, 0, s, sv, rows, tiles,
-                       ctl, lim, pow3, tile_recs, lists, U);
+    hipLaunchKernelGGL((k_verify_persistent<6, true, LA, BOUNDED>), grid, block, 0, s, sv,
+                       rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
+}
+
+template <bool LA>
+static void launch_persistent_la(int limbs, const SieveArrays& sv, const RowDesc* rows,
+                                 const TileDesc* tiles, Ctl* ctl, uint32_t nblocks, uint32_t U,
+                                 const Limits& lim, const uint64_t* pow3, DevRec* tile_recs,
+                                 DeviceLists lists, hipStream_t s) {
+  if (LA || lim.max_loop < kMaxLoop)
+    launch_persistent_lb<LA, true>(limbs, sv, rows, tiles, ctl, nblocks, U, lim, pow3,
+                                   tile_recs, lists, s);
+  else
+    launch_persistent_lb<LA, false>(limbs, sv, rows, tiles, ctl, nblocks, U, lim, pow3,
+                                    tile_recs, lists, s);
 }
 
 hipError_t launch_verify_persistent(int limbs, const SieveArrays& sv, const RowDesc* rows,

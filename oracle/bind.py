@@ -104,6 +104,9 @@ def lib():
         L.or_search1.restype = C.c_int64
         L.or_sieve_sorted.argtypes = [C.c_uint32, _u64p, _u64p, _P(C.c_uint8), C.c_int64]
         L.or_sieve_sorted.restype = C.c_int64
+        L.or_sieve_windows.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, _u64p, C.c_int,
+                                       _u64p, _u64p, _P(C.c_uint8), C.c_int64]
+        L.or_sieve_windows.restype = C.c_int64
         L.or_precompute_bv.argtypes = [_P(Knobs), _u64p]
         L.or_mod9_rows.argtypes = [C.c_uint32, C.c_uint32, _u64p]
         L.or_doubleeven.argtypes = [C.c_uint32] * 4
@@ -175,6 +178,24 @@ def sieve(k: int) -> Sieve:
                                 _ptr(f, C.c_uint8), n)
     assert got == n
     return Sieve(k, r, num, f)
+
+
+def sieve_windows(k: int, w: int, tops, threads: int = 1) -> Sieve:
+    """The survivors of sieve(k) with r >> w in tops: all that chunks whose residues mod 2^k
+    lie in the windows [top 2^w, (top + 1) 2^w) use."""
+    t = np.array(sorted(set(int(x) for x in tops)), np.uint64)
+    cap = len(t) * ((1 << w) // 64) + 4096  # above the survivor density of any k <= 40
+    while True:
+        r = np.empty(cap, np.uint64)
+        num = np.empty(cap, np.uint64)
+        f = np.empty(cap, np.uint8)
+        n = lib().or_sieve_windows(k, w, len(t), _ptr(t, C.c_uint64), threads,
+                                   _ptr(r, C.c_uint64), _ptr(num, C.c_uint64),
+                                   _ptr(f, C.c_uint8), cap)
+        assert n >= 0
+        if n <= cap:
+            return Sieve(k, r[:n].copy(), num[:n].copy(), f[:n].copy())
+        cap = n
 
 
 def knobs(N, A, B=24, num_bv=16, semantics=SEM_CXX, cutoff=100):

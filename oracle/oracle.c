@@ -53,7 +53,24 @@ typedef struct {
   void* ctx;
   int64_t count;
   u128 p3[128];
+  /* Residue windows (or_sieve_windows): at depths s >= win_bits, bits win_bits..s of base
+   * must be the low bits of one of the ntops values tops[] (base >> win_bits == top at the
+   * leaves).  ntops = 0: no window. */
+  uint32_t win_bits;
+  uint32_t ntops;
+  const uint64_t* tops;
 } s1_ctx;
+
+/* Whether a path whose bits win_bits..s-1 are `pre` may take bit s = b. */
+static int s1_win_ok(const s1_ctx* c, u128 base, uint32_t s, uint32_t b) {
+  if (!c->ntops || s < c->win_bits) return 1;
+  const uint32_t d = s - c->win_bits;
+  const uint64_t m = d ? (d >= 64 ? ~0ull : (1ull << d) - 1) : 0;
+  const uint64_t pre = (uint64_t)(base >> c->win_bits) & m;
+  for (uint32_t i = 0; i < c->ntops; ++i)
+    if ((c->tops[i] & m) == pre && ((c->tops[i] >> d) & 1) == b) return 1;
+  return 0;
+}
 
 static void s1_rec(s1_ctx* c, u128 base, uint32_t s, uint32_t f, u128 number, uint32_t de,
                    uint32_t ne) {
@@ -86,20 +103,27 @@ static void s1_rec(s1_ctx* c, u128 base, uint32_t s, uint32_t f, u128 number, ui
   u128 nA = number;
   u128 nB = number + c->p3[f];
   u128 bit = (u128)1 << s;
+  /* In residue windows only the children on the path of some window are walked. */
+  const int w0 = s1_win_ok(c, base, s, 0);
+  const int w1 = s1_win_ok(c, base, s, 1);
   if ((nA & 1) == 0) {
-    s1_rec(c, base, s + 1, f + 1, nA + nA / 2, (de | inplay) & ~2u, 0);
-    if (de < 2) s1_rec(c, base | bit, s + 1, f, nB / 2 + 1, de << 1, ne + 1);
+    if (w0) s1_rec(c, base, s + 1, f + 1, nA + nA / 2, (de | inplay) & ~2u, 0);
+    if (w1 && de < 2) s1_rec(c, base | bit, s + 1, f, nB / 2 + 1, de << 1, ne + 1);
   } else {
-    if (de < 2) s1_rec(c, base, s + 1, f, nA / 2 + 1, de << 1, ne + 1);
-    s1_rec(c, base | bit, s + 1, f + 1, nB + nB / 2, (de | inplay) & ~2u, 0);
+    if (w0 && de < 2) s1_rec(c, base, s + 1, f, nA / 2 + 1, de << 1, ne + 1);
+    if (w1) s1_rec(c, base | bit, s + 1, f + 1, nB + nB / 2, (de | inplay) & ~2u, 0);
   }
 }
 
-static int64_t s1_walk(const or_node* root, uint32_t depth, uint32_t sem, leaf_fn fn,
-                       void* ctx) {
+static int64_t s1_walk_win(const or_node* root, uint32_t depth, uint32_t sem, leaf_fn fn,
+                           void* ctx, uint32_t win_bits, uint32_t ntops,
+                           const uint64_t* tops) {
   if (!root || depth > 100 || root->sigfig > depth) return -1;
   s1_ctx* c = (s1_ctx*)malloc(sizeof(s1_ctx));
   if (!c) return -1;
+  c->win_bits = win_bits;
+  c->ntops = ntops;
+  c->tops = tops;
   c->depth = depth;
   c->sem = sem;
   c->fn = fn;
@@ -115,6 +139,11 @@ static int64_t s1_walk(const or_node* root, uint32_t depth, uint32_t sem, leaf_f
   int64_t n = c->count;
   free(c);
   return n;
+}
+
+static int64_t s1_walk(const or_node* root, uint32_t depth, uint32_t sem, leaf_fn fn,
+                       void* ctx) {
+  return s1_walk_win(root, depth, sem, fn, ctx, 0, 0, NULL);
 }
 
 typedef struct {
@@ -153,14 +182,8 @@ static void sorted_leaf(void* vctx, const or_node* leaf) {
   s->n++;
 }
 
-int64_t or_sieve_sorted(uint32_t k, uint64_t* r, uint64_t* number, uint8_t* f, int64_t cap) {
-  if (k < 2 || k > 40) return -1;
-  or_node root = {3, 0, 9, 0, 2, 2, 1, 0};
-  if (!r || !number || !f) return s1_walk(&root, k, OR_SEM_CXX, NULL, NULL);
-  sorted_ctx s = {r, number, f, cap, 0};
-  int64_t n = s1_walk(&root, k, OR_SEM_CXX, sorted_leaf, &s);
-  if (n < 0 || n > cap) return n;
-  /* LSD radix sort by r, 16-bit digits, carrying number and f along. */
+/* LSD radix sort of n survivors by r (k bits), 16-bit digits, carrying number and f. */
+static int sort_by_r(uint32_t k, int64_t n, uint64_t* r, uint64_t* number, uint8_t* f) {
   uint64_t* r2 = (uint64_t*)malloc((size_t)n * 8 + 8);
   uint64_t* n2 = (uint64_t*)malloc((size_t)n * 8 + 8);
   uint8_t* f2 = (uint8_t*)malloc((size_t)n + 1);
@@ -184,7 +207,124 @@ int64_t or_sieve_sorted(uint32_t k, uint64_t* r, uint64_t* number, uint8_t* f, i
     memcpy(f, f2, (size_t)n);
   }
   free(r2); free(n2); free(f2); free(cnt);
-  return n;
+  return 0;
+}
+
+int64_t or_sieve_sorted(uint32_t k, uint64_t* r, uint64_t* number, uint8_t* f, int64_t cap) {
+  if (k < 2 || k > 40) return -1;
+  or_node root = {3, 0, 9, 0, 2, 2, 1, 0};
+  if (!r || !number || !f) return s1_walk(&root, k, OR_SEM_CXX, NULL, NULL);
+  sorted_ctx s = {r, number, f, cap, 0};
+  int64_t n = s1_walk(&root, k, OR_SEM_CXX, sorted_leaf, &s);
+  if (n < 0 || n > cap) return n;
+  return sort_by_r(k, n, r, number, f) ? -1 : n;
+}
+
+/* One thread of or_sieve_windows: subtrees below frontier nodes taken in turn, leaves
+ * appended to a buffer of its own that grows as needed. */
+typedef struct {
+  const or_node* nodes;
+  int64_t nn;
+  atomic_int_fast64_t* next;
+  uint32_t k, w, ntops;
+  const uint64_t* tops;
+  uint64_t *r, *num;
+  uint8_t* f;
+  int64_t n, cap;
+  int err;
+} win_job;
+
+static void win_leaf(void* vctx, const or_node* leaf) {
+  win_job* j = (win_job*)vctx;
+  if (j->err) return;
+  if (j->n == j->cap) {
+    const int64_t c = j->cap ? 2 * j->cap : (1 << 16);
+    uint64_t* r = (uint64_t*)realloc(j->r, (size_t)c * 8);
+    if (r) j->r = r;
+    uint64_t* num = (uint64_t*)realloc(j->num, (size_t)c * 8);
+    if (num) j->num = num;
+    uint8_t* f = (uint8_t*)realloc(j->f, (size_t)c);
+    if (f) j->f = f;
+    if (!r || !num || !f) {
+      j->err = 1;
+      return;
+    }
+    j->cap = c;
+  }
+  j->r[j->n] = leaf->base_lo;
+  j->num[j->n] = leaf->number_lo;
+  j->f[j->n] = (uint8_t)leaf->threepower;
+  j->n++;
+}
+
+static void* win_worker(void* v) {
+  win_job* j = (win_job*)v;
+  for (;;) {
+    const int64_t i = atomic_fetch_add(j->next, 1);
+    if (i >= j->nn) break;
+    if (s1_walk_win(&j->nodes[i], j->k, OR_SEM_CXX, win_leaf, j, j->w, j->ntops, j->tops) < 0)
+      j->err = 1;
+  }
+  return NULL;
+}
+
+/* The survivors of or_sieve_sorted(k) whose residue r has r >> w equal to one of tops[]:
+ * the same walk, below depth w only along the paths of those windows.  For chunks of a
+ * range narrower than 2^k, whose residues mod 2^k form such windows, these are every
+ * survivor the chunks use, at the cost of the tree to depth w.  The subtrees below depth
+ * min(w, 14) are walked on `threads` threads.  Returns the number of survivors; they are
+ * written (sorted by r) only if that is at most cap, so a caller with too small a buffer
+ * calls again with a larger one. */
+int64_t or_sieve_windows(uint32_t k, uint32_t w, uint32_t ntops, const uint64_t* tops,
+                         int threads, uint64_t* r, uint64_t* number, uint8_t* f,
+                         int64_t cap) {
+  if (k < 2 || k > 40 || w < 2 || w > k || !ntops || !tops) return -1;
+  for (uint32_t i = 0; i < ntops; ++i)
+    if ((tops[i] >> (k - w)) != 0) return -1;
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  or_node root = {3, 0, 9, 0, 2, 2, 1, 0};
+  const uint32_t split = w < 14 ? w : 14;
+  const int64_t nn = or_search1(&root, split, OR_SEM_CXX, NULL, 0);
+  if (nn < 0) return -1;
+  or_node* nodes = (or_node*)malloc(sizeof(or_node) * (size_t)(nn > 0 ? nn : 1));
+  win_job* jobs = (win_job*)calloc((size_t)threads, sizeof(win_job));
+  pthread_t* th = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
+  atomic_int_fast64_t next;
+  atomic_init(&next, 0);
+  int rc = (nodes && jobs && th) ? 0 : -1;
+  if (rc == 0) or_search1(&root, split, OR_SEM_CXX, nodes, nn);
+  int started = 0;
+  for (int t = 0; rc == 0 && t < threads; ++t) {
+    jobs[t] = (win_job){nodes, nn, &next, k, w, ntops, tops, NULL, NULL, NULL, 0, 0, 0};
+    if (pthread_create(&th[t], NULL, win_worker, &jobs[t]) != 0) rc = -1;
+    else started++;
+  }
+  int64_t n = 0;
+  for (int t = 0; t < started; ++t) {
+    pthread_join(th[t], NULL);
+    if (jobs[t].err) rc = -1;
+    n += jobs[t].n;
+  }
+  if (rc == 0 && r && number && f && n <= cap) {
+    int64_t o = 0;
+    for (int t = 0; t < started; ++t) {
+      memcpy(r + o, jobs[t].r, (size_t)jobs[t].n * 8);
+      memcpy(number + o, jobs[t].num, (size_t)jobs[t].n * 8);
+      memcpy(f + o, jobs[t].f, (size_t)jobs[t].n);
+      o += jobs[t].n;
+    }
+    if (sort_by_r(k, n, r, number, f)) rc = -1;
+  }
+  for (int t = 0; t < threads && jobs; ++t) {
+    free(jobs[t].r);
+    free(jobs[t].num);
+    free(jobs[t].f);
+  }
+  free(nodes);
+  free(jobs);
+  free(th);
+  return rc ? -1 : n;
 }
 
 /* ------------------------------------------------------------------------------------

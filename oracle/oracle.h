@@ -35,6 +35,8 @@
  *                      cutoff; the 16-step table itself against the same step-by-step map.
  *   or_verify_range    against brute-force T iteration (final value, max excursion,
  *                      descent) and a brute-force candidate set on small ranges.
+ *   or_sieve_windows   against the windows of or_sieve_sorted (single windows and sets of
+ *                      them at k = 16 and k = 20), which is pinned by the sieve pins above.
  */
 #ifndef COLLATZ_ORACLE_H
 #define COLLATZ_ORACLE_H
@@ -107,6 +109,10 @@ int64_t or_search1(const or_node* root, uint32_t depth, uint32_t semantics,
                    or_node* leaves, int64_t cap);
 /* Leaves from the canonical root (3, 2, 2, 9, 1, 0) at depth k, sorted by residue. */
 int64_t or_sieve_sorted(uint32_t k, uint64_t* r, uint64_t* number, uint8_t* f, int64_t cap);
+/* The survivors with r >> w in tops[0..ntops) (2 <= w <= k), sorted by r, walked on
+ * `threads` threads; written only if their number (the return value) is at most cap. */
+int64_t or_sieve_windows(uint32_t k, uint32_t w, uint32_t ntops, const uint64_t* tops,
+                         int threads, uint64_t* r, uint64_t* number, uint8_t* f, int64_t cap);
 
 /* --- precompute -------------------------------------------------------------- */
 /* bv: num_bv arrays of 2^B bits each, as 2^(B-6) u64 words per array (logical bit order:

@@ -39,6 +39,30 @@ def test_sieve_sorted_and_three_mod_four(s20):
     assert all(int(x) % 4 == 3 for x in r[:5000])
 
 
+@pytest.mark.parametrize("k,w", [(16, 10), (20, 14), (20, 4), (20, 20)])
+def test_sieve_window_is_a_window_of_the_sieve(oracle, k, w):
+    """Residue windows [top 2^w, (top + 1) 2^w) of the windowed walk (every one, or 200
+    seeded ones where there are more) hold exactly the full sieve's survivors there, with
+    the same numbers and odd-step counts."""
+    full = oracle.sieve(k)
+    tops = full.r >> np.uint64(w)
+    n = 1 << (k - w)
+    for top in (range(n) if n <= 256 else random.Random(k * 100 + w).sample(range(n), 200)):
+        win = oracle.sieve_windows(k, w, [top])
+        sel = tops == np.uint64(top)
+        assert np.array_equal(win.r, full.r[sel]), (k, w, top)
+        assert np.array_equal(win.number, full.number[sel])
+        assert np.array_equal(win.f, full.f[sel])
+    # Several windows at once: their union.
+    rng = random.Random(k + w)
+    for _ in range(5):
+        ts = rng.sample(range(n), min(n, rng.choice([2, 5, 17])))
+        win = oracle.sieve_windows(k, w, ts, threads=4)
+        sel = np.isin(tops, np.array(ts, np.uint64))
+        assert np.array_equal(win.r, full.r[sel]), (k, w, ts)
+        assert np.array_equal(win.number, full.number[sel])
+
+
 def test_lemma_2_3(s20):
     """T^k(q 2^k + r) = T^k(r) + q 3^f for q up to 2^90 (paper p.4 Lemma 2.3)."""
     rng = random.Random(2)
