@@ -6,9 +6,10 @@
 // whose trajectory reached 2^cutoff as hex on stdout (collatz_cudav7.cu:298) and the summary
 // line on stderr (collatz_cudav7.cu:181-182; the OpenCL program's two-line form with
 // --semantics opencl, collatz_openclv2.cpp:279-280).  Spot checks (collatz_cudav7.cu:619-635)
-// re-run one tile of 256 leaves in every --spot-freq F (default 2,560,000: the reference's
-// one launch of 2^16 leaves in 10000) by a second kernel and compare; "Spot checks" counts
-// them, and each tile whose rerun differs prints "Spot check failed" as the reference does.
+// re-run one tile of 256 leaves in every P (P the largest power of two not above --spot-freq
+// F; the default F = 2,560,000 is the reference's one launch of 2^16 leaves in 10000, and
+// gives P = 2^21) by a second kernel and compare; "Spot checks" counts them, and each tile
+// whose rerun differs prints "Spot check failed" as the reference does.
 //
 //   echo "3 2 2 9" | collatz_run --semantics rust [--N 40] [--A 6] [--cutoff 120]
 // is main.rs's protocol (main.rs:85-98): the first four numbers on stdin (doubleeven and

@@ -2,7 +2,10 @@
  * collatz.h -- C ABI of the MI355X Collatz range verifier (libcollatz_amd.so).
  *
  * Plain C types only: integers, pointers, sizes.  Streams are HIP streams passed as
- * `void*` (NULL = the default stream).  Every entry point returns COLLATZ_OK (0) or a
+ * `void*`; NULL selects the calling thread's per-thread default stream (hipStreamPerThread),
+ * which runs after the caller's earlier default-stream work: the library enqueues nothing
+ * on the legacy default stream, which the runtime blocks while another thread captures a
+ * hipGraph (a server does).  Every entry point returns COLLATZ_OK (0) or a
  * negative status; collatz_last_error() returns a message for the calling thread.
  * No exceptions cross this boundary and no kernel prints.
  *
@@ -116,7 +119,8 @@ typedef struct {
 } collatz_range_result;
 
 enum collatz_form {
-  COLLATZ_FORM_PLAIN = 0,     /* one lane per (residue, q): the drop-in kernel */
+  COLLATZ_FORM_PLAIN = 0,     /* fixed blocks over the range's survivors, lanes refilled within
+                                 each wave's share: the drop-in kernel */
   COLLATZ_FORM_PERSISTENT = 1 /* persistent waves with lane refill and work stealing */
 };
 
@@ -199,11 +203,12 @@ typedef struct {
                             120: the threshold of its (number >> 64) >= 2^56 test) */
   int32_t device;
   uint32_t spot_check_freq; /* 0: none.  Else spot checks (collatz_cudav7.cu:619-635): one
-                               tile of 256 leaves in every spot_check_freq, at a seeded offset,
-                               is re-run by a second kernel and its hash, count, block tally
-                               and overflow count compared (at most 2^23 candidates per root
-                               launch).  The reference checks one launch of 2^16 leaves in
-                               10000 (SPOTCHECKFREQUENCY, :55), i.e. one tile in 2,560,000. */
+                               tile of 256 leaves in every P, P the largest power of two not
+                               above spot_check_freq, at a seeded offset, is re-run by a second
+                               kernel and its hash, count, block tally and overflow count
+                               compared (at most 2^23 candidates per root launch).  The
+                               reference checks one launch of 2^16 leaves in 10000
+                               (SPOTCHECKFREQUENCY, :55), i.e. one tile in 2,560,000. */
 } collatz_ref_params;
 
 /* Under COLLATZ_SEM_RUST Step 3 is main.rs's own search3 (main.rs:321-341): each candidate
