@@ -743,12 +743,17 @@ def main():
         reqs = W.request_stream(1, 20000)
         full = {int(x) for x in args.serve_full_rates.split(",") if x}
         # Untimed: the autotuner's first sweeps for the (word width, magnitude) buckets the
-        # launches of the stream fall in, once per sieve: the first 300 requests one at a
-        # time (a launch per request, as at low rates) and then all at once (launches that
-        # mix many requests, as at high rates).
+        # launches of the stream fall in, once per sieve: one request of every magnitude
+        # class of the stream (bit length / 8 of its end; starts below 2^48 are one in 16000
+        # requests, and their bucket's first sweep took 60-120 ms of a replay at 100 req/s),
+        # then the first 300 requests one at a time (a launch per request, as at low rates)
+        # and all at once (launches that mix many requests, as at high rates).
         t_warm = time.perf_counter()
         srv = cz.Server(sv, chunk_log2=BATCH_CHUNK)
-        for _, lo, hi in reqs[:300]:
+        classes = {}
+        for _, lo, hi in reqs:
+            classes.setdefault(hi.bit_length() // 8, (lo, hi))
+        for _, lo, hi in [(0,) + v for v in classes.values()] + reqs[:300]:
             srv.wait(srv.submit(lo, hi))
         srv.close()
         served_leg(sv, reqs[:300], None, BATCH_CHUNK)
