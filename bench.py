@@ -742,18 +742,19 @@ def main():
     if not args.no_serve:
         reqs = W.request_stream(1, 20000)
         full = {int(x) for x in args.serve_full_rates.split(",") if x}
-        # Untimed: the autotuner's first sweeps for the (word width, magnitude) buckets the
-        # launches of the stream fall in, once per sieve: one request of every magnitude
-        # class of the stream (bit length / 8 of its end; starts below 2^48 are one in 16000
-        # requests, and their bucket's first sweep took 60-120 ms of a replay at 100 req/s),
-        # then the first 300 requests one at a time (a launch per request, as at low rates)
-        # and all at once (launches that mix many requests, as at high rates).
+        # Untimed: the autotuner's sweeps for the (word width, magnitude) buckets the launches
+        # of the stream fall in, once per sieve.  A server never sweeps inside a launch (it
+        # runs the default shape and sweeps while idle, one timing launch at a time), so the
+        # buckets are tuned here by one batched call each over a 2^32-wide range of that
+        # magnitude (bit length / 8 of the end), which sweeps before it launches; then the
+        # first 300 requests run one at a time (a launch per request, as at low rates) and
+        # all at once (launches that mix many requests, as at high rates).
         t_warm = time.perf_counter()
+        for c in sorted({hi.bit_length() // 8 for _, _, hi in reqs}):
+            lo = max(1 << (8 * c), 1 << 40)
+            sv.verify_batch([(lo, lo + (1 << 32))], chunk_log2=BATCH_CHUNK)
         srv = cz.Server(sv, chunk_log2=BATCH_CHUNK)
-        classes = {}
-        for _, lo, hi in reqs:
-            classes.setdefault(hi.bit_length() // 8, (lo, hi))
-        for _, lo, hi in [(0,) + v for v in classes.values()] + reqs[:300]:
+        for _, lo, hi in reqs[:300]:
             srv.wait(srv.submit(lo, hi))
         srv.close()
         served_leg(sv, reqs[:300], None, BATCH_CHUNK)

@@ -80,10 +80,10 @@ struct DevRec {
   uint32_t pad;
 };
 
-// A candidate handed from the two-limb kernels to the three-limb pass.
+// A candidate handed from the two-limb kernels to the three-limb pass: its start n and its
+// output slot (the pass recomputes the survivor record of n mod 2^k).
 struct EscEntry {
   uint64_t n_lo, n_hi;
-  uint64_t res_index;
   uint32_t slot;
   uint32_t pad;
 };

@@ -72,6 +72,14 @@ struct collatz_server {
   uint64_t launches = 0;
   double t0 = 0;
   uint64_t bytes_in_use = 0;
+  // Tuner sweeps the launches asked for (buckets without a tuned shape), run while the
+  // server is idle, one timing launch at a time: bucket -> (word width, largest end, state).
+  struct WantTune {
+    int limbs;
+    cz::u128 max_hi;
+    cz::TuneSweep st;
+  };
+  std::map<uint32_t, WantTune> want_tune;
 };
 
 namespace cz {
@@ -92,8 +100,27 @@ static void server_loop(collatz_server* S) {
     uint64_t launch_id = 0;
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      S->cv.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->active.empty(); });
+      S->cv.wait(lk, [&] {
+        return S->stop || !S->pending.empty() || !S->active.empty() || !S->want_tune.empty();
+      });
       if (S->stop && S->pending.empty() && S->active.empty()) return;
+      if (S->pending.empty() && S->active.empty()) {
+        // Idle: one timing launch of a wanted tuner sweep (a few ms), then look for work
+        // again, so a request that arrives meanwhile waits for one timing launch at most.
+        auto it = S->want_tune.begin();
+        const uint32_t bucket = it->first;
+        collatz_server::WantTune w = it->second;
+        lk.unlock();
+        const int st = tune_step(S->sv, w.limbs, w.max_hi, w.st);
+        lk.lock();
+        if (st > 0) S->want_tune.erase(bucket);
+        else S->want_tune[bucket] = w;
+        // Another caller is tuning this sieve right now: wait for work or a moment.
+        if (st < 0)
+          S->cv.wait_for(lk, std::chrono::milliseconds(2),
+                         [&] { return S->stop || !S->pending.empty(); });
+        continue;
+      }
       // Admission: oldest first, while the footprint budget and the in-flight cap allow.
       while (!S->pending.empty() && S->active.size() < S->opts.max_inflight) {
         Request* r = S->pending.front();
@@ -166,14 +193,22 @@ static void server_loop(collatz_server* S) {
       }
     }
     // The launch shape: the autotuned waves per CU and checks per refill of the sieve for
-    // this word width and magnitude (tuned on first use), and tiles for this launch's size
-    // (a small launch spreads over all resident waves); then each row's pieces and each
-    // slot's first tile.
+    // this word width and magnitude, and tiles for this launch's size (a small launch
+    // spreads over all resident waves); then each row's pieces and each slot's first tile.
+    // A bucket without a tuned shape runs the default one, and its sweep is queued for the
+    // server's idle moments: a sweep run here would hold every request of the launch back
+    // by some 80 timing launches.
     uint64_t est_total = 0;
     for (const uint64_t e : row_est) est_total += e;
     int width = launch_width(2, max_hi, S->opts.cutoff_log2);
+    bool want = false;
     const Tuning tn = lp.rows.empty() ? Tuning{4096, 32, 2}
-                                      : choose_tuning(S->sv, width, max_hi, est_total, 0, 0, 0);
+                                      : tuned_or_default(S->sv, width, max_hi, est_total, want);
+    if (want) {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->want_tune.emplace(tune_bucket(width, max_hi),
+                           collatz_server::WantTune{width, max_hi, TuneSweep{}});
+    }
     for (size_t i = 0; i < lp.rows.size(); ++i) {
       if (slot_first[i]) lp.part.push_back(lp.ntiles);
       uint64_t pieces = (uint64_t)((double)row_est[i] / tn.tile + 0.5);

@@ -139,12 +139,12 @@ struct SinkWave {
 };
 
 // Set up a candidate: v = M + q3 (q3 has L words; q3_big says q * 3^f needs more), n =
-// qb | r.  Returns false if the start value does not fit L words (then the candidate must
-// go to the three-limb pass).
+// qb | r, and its step count at steps0.  Returns false if the start value does not fit L
+// words (then the candidate must go to the three-limb pass).
 template <int L>
 __device__ __forceinline__ bool cand_start(Cand<L>& c, uint64_t M, const uint32_t* q3,
                                            bool q3_big, uint64_t qb_lo, uint64_t qb_hi,
-                                           uint64_t r) {
+                                           uint64_t r, uint32_t steps0 = 0) {
   uint32_t cy = 0;
   c.v.w[0] = addc(q3[0], (uint32_t)M, 0u, cy);
   c.v.w[1] = addc(q3[1], (uint32_t)(M >> 32), cy, cy);
@@ -154,7 +154,7 @@ __device__ __forceinline__ bool cand_start(Cand<L>& c, uint64_t M, const uint32_
   c.n_lo = qb_lo | r;
   c.n_hi = qb_hi;
   c.hash = 0;
-  c.steps = 0;
+  c.steps = steps0;
   c.ovf = 0;
   return fits;
 }
@@ -178,13 +178,12 @@ __device__ __forceinline__ void push_stall(const DeviceLists& lists, uint32_t sl
 }
 
 __device__ __forceinline__ void push_escalate(const DeviceLists& lists, uint64_t n_lo,
-                                              uint64_t n_hi, uint64_t res_index, uint32_t slot) {
+                                              uint64_t n_hi, uint32_t slot) {
   const unsigned long long i = atomicAdd(lists.esc_count, 1ull);
   if (i < lists.esc_cap) {
     EscEntry e;
     e.n_lo = n_lo;
     e.n_hi = n_hi;
-    e.res_index = res_index;
     e.slot = slot;
     e.pad = 0;
     lists.esc[i] = e;
@@ -312,11 +311,14 @@ __device__ __forceinline__ uint64_t esc_mask(const Num<L>& v) {
 // Lanes with number < n (lt_base).
 template <int L>
 __device__ __forceinline__ uint64_t lt_base_mask(const Num<L>& v, uint64_t n_lo, uint64_t n_hi) {
-  const uint64_t lo = ballot(pair(v.w[0], v.w[1]) < n_lo);
   if constexpr (L == 3) {
+    const uint64_t lo = ballot(pair(v.w[0], v.w[1]) < n_lo);
     const uint32_t nh = (uint32_t)n_hi;
+    // Starts below 2^64 (n's top word 0, uniform over a tile) need two compares.
+    if (nh == 0u) return ballot(v.w[2] == 0u) & lo;
     return ballot(v.w[2] < nh) | (ballot(v.w[2] == nh) & lo);
   } else {
+    const uint64_t lo = ballot(pair(v.w[0], v.w[1]) < n_lo);
     const uint64_t vh = pair(v.w[2], v.w[3]);
     const uint64_t m = ballot(vh < n_hi) | (ballot(vh == n_hi) & lo);
     if constexpr (L > 4) return m & ballot((v.w[4] | v.w[5]) == 0u);
@@ -416,7 +418,7 @@ __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowD
           have = true;
           nchk = 0;
         } else {
-          push_escalate(lists, c.n_lo, c.n_hi, cur_i, row.slot);
+          push_escalate(lists, c.n_lo, c.n_hi, row.slot);
         }
       }
     }
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowD
       one_check<L, LOWCUT>(c, c.n_hi, sink, lim, lists, p3s, esc, done);
       acc.chk += 1;
       if (esc) {
-        push_escalate(lists, c.n_lo, c.n_hi, cur_i, row.slot);
+        push_escalate(lists, c.n_lo, c.n_hi, row.slot);
         have = false;
       } else if (done) {
         finish<L>(c, acc, sv.k);
@@ -468,11 +470,21 @@ __global__ __launch_bounds__(256) void k_escalated(SieveArrays sv, const EscEntr
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const EscEntry e = esc[t];
-  const uint4 sr = sv.rec[e.res_index];
-  const uint64_t r = rec_r(sr);
-  const uint64_t M = rec_M(sr);
-  const uint32_t f = rec_f(sr);
   const uint32_t k = sv.k;
+  // The survivor record of r = n mod 2^k is (T^k(r) + 1, the odd steps f among those k),
+  // recomputed here: k < 40 steps of T from r < 2^k stay below 2^k (3/2)^k < 2^64.
+  const uint64_t r = e.n_lo & ((1ull << k) - 1);
+  uint64_t x = r;
+  uint32_t f = 0;
+  for (uint32_t i = 0; i < k; ++i) {
+    if (x & 1u) {
+      x += (x >> 1) + 1;  // (3x + 1) / 2
+      ++f;
+    } else {
+      x >>= 1;
+    }
+  }
+  const uint64_t M = x + 1;
   const uint64_t q_lo = k >= 64 ? (e.n_hi >> (k - 64)) : ((e.n_lo >> k) | (k ? e.n_hi << (64 - k) : 0));
   const uint64_t q_hi = k >= 64 ? 0 : (e.n_hi >> k);
   uint32_t q3[6];
@@ -503,21 +515,21 @@ constexpr int kWavesPerBlock = 4;
 
 // q * 3^f for f < 41 on L words.  Where the product does not fit L words the row is all
 // ones: adding any start value (>= 1) then carries out, which marks the candidate for the
-// three-limb pass without a separate flag.
+// three-limb pass without a separate flag.  On three words a row at or above the start
+// bound 2^86 holds 2^95 instead (see the tile set-up).
 template <int L, bool LA>
 struct WaveTables {
   uint32_t q3[41][L];
   uint32_t q3lo[LA ? 41 : 1];  // low word of q * 3^f (the look-ahead index), also where q3
-                               // is all ones
+                               // holds a marker
 };
 
 // Look-ahead form: a wave reads its tile 64 survivors at a time (one coalesced window,
 // loaded one window ahead), tests them against the look-ahead table and keeps the passing
-// records and their offsets, in order, in LDS; lanes that need a candidate take the next
+// records, in order, in LDS; lanes that need a candidate take the next
 // one from there.
 struct LaStage {
   uint4 rec[64];
-  uint32_t off[64];
 };
 
 // BOUNDED: the work loop carries the host-set iteration bound (Limits::max_loop).  The
@@ -563,8 +575,19 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
       uint32_t big = 0;
 #pragma unroll
       for (int i = L; i < 6; ++i) big |= w6[i];
+      if constexpr (L == 3) {
+        // On three words a row whose q 3^f reaches the start bound 2^86 holds 2^95 instead:
+        // a start value M + q 3^f (M < 2^64) then stays below 2^96 without a carry out
+        // (q 3^f < 2^86 leaves 2^96 - 2^86 of room; 2^95 + M < 2^96) and fails the start
+        // test exactly when it should, so the start needs no carry-out test.
+        big |= (uint32_t)(w6[2] >= (1u << (kNarrowCheckLog2 - 64)));
+        T.q3[lane][0] = big ? 0u : w6[0];
+        T.q3[lane][1] = big ? 0u : w6[1];
+        T.q3[lane][2] = big ? 0x80000000u : w6[2];
+      } else {
 #pragma unroll
-      for (int i = 0; i < L; ++i) T.q3[lane][i] = big ? 0xFFFFFFFFu : w6[i];
+        for (int i = 0; i < L; ++i) T.q3[lane][i] = big ? 0xFFFFFFFFu : w6[i];
+      }
       if constexpr (LA) T.q3lo[lane] = w6[0];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -586,9 +609,10 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
     // Lane masks (uniform): lanes with a running candidate, lanes holding a prefetched
     // record.  Per-lane conditions are read from them with in_mask().
     uint64_t actm = 0, pfm = 0;
-    uint32_t cidx = 0;  // offset of the running candidate in the tile
     uint4 pf_rec = make_uint4(0, 0, 0, 0);
-    uint32_t pf_off = 0;
+    // A candidate's step count starts at k, so that it is J (k + the T-steps run) when the
+    // candidate finishes; the step bound is moved by k with it.
+    const uint32_t step_lim = lim.max_steps + sv.k;
     const uint4* tile_rec = sv.rec + first;
     // Look-ahead stage: entries [shead, scnt) are kept survivors not yet taken; nx holds the
     // records of the next window (survivors [cur, cur + 64)), loaded one window ahead.
@@ -613,15 +637,16 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
         if (in_mask(S)) {
           const uint32_t f = rec_f(pf_rec);
           const bool fits = cand_start<L>(c, rec_M(pf_rec), T.q3[f], false, qb_lo, qb_hi,
-                                          rec_r(pf_rec));
+                                          rec_r(pf_rec), sv.k);
           c.slot = slot;  // (overflow reports carry it)
-          cidx = pf_off;
-          if (!fits) c.v.w[L - 1] = 0xFFFFFFFFu;
+          if constexpr (L != 3) {
+            if (!fits) c.v.w[L - 1] = 0xFFFFFFFFu;
+          }
         }
         const uint64_t ok = start_ok_mask<L, LOWCUT>(c.v) & S;
         const uint64_t bad = S & ~ok;
         if (__builtin_expect(bad != 0, 0)) {
-          if (in_mask(bad)) push_escalate(lists, c.n_lo, qb_hi, first + cidx, slot);
+          if (in_mask(bad)) push_escalate(lists, c.n_lo, qb_hi, slot);
         }
         actm |= ok;
         pfm &= ~S;
@@ -649,10 +674,7 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
             const uint32_t avail = scnt - shead;
             const uint32_t take = avail < want ? avail : want;
             const uint64_t tk = ballot(rank < take) & open;
-            if (in_mask(tk)) {
-              pf_rec = S.rec[shead + rank];
-              pf_off = S.off[shead + rank];
-            }
+            if (in_mask(tk)) pf_rec = S.rec[shead + rank];
             pfm |= tk;
             open &= ~tk;
             rank -= take;  // (lanes that took, or need nothing, no longer look at it)
@@ -666,11 +688,7 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
               keep = la_keep(sv, f, nx.z + T.q3lo[f]);
             }
             const uint64_t km = __ballot(keep);
-            if (keep) {
-              const uint32_t p = lane_rank(km);
-              S.rec[p] = nx;
-              S.off[p] = o;
-            }
+            if (keep) S.rec[lane_rank(km)] = nx;
             // A wave's LDS operations execute in order, so other lanes' reads after this
             // point see the writes; only the compiler must keep them in order.  (A memory
             // fence would also wait for loads in flight.)
@@ -688,8 +706,7 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
         const uint32_t rank = lane_rank(need);
         const uint64_t tk = ballot(rank < take) & need;
         if (in_mask(tk)) {
-          pf_off = cur + rank;
-          pf_rec = tile_rec[pf_off];
+          pf_rec = tile_rec[cur + rank];
         }
         pfm |= tk;
         cur += take;
@@ -699,10 +716,10 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
       // to U checks beyond the bound first).
       uint64_t stalled;
       if constexpr (BOUNDED) {
-        const uint32_t pen = iters > lim.max_loop ? lim.max_steps : 0u;
-        stalled = ballot(c.steps + pen >= lim.max_steps) & actm;
+        const uint32_t pen = iters > lim.max_loop ? step_lim : 0u;
+        stalled = ballot(c.steps + pen >= step_lim) & actm;
       } else {
-        stalled = ballot(c.steps >= lim.max_steps) & actm;
+        stalled = ballot(c.steps >= step_lim) & actm;
       }
       if (__builtin_expect(stalled != 0, 0)) {
         if (BOUNDED && __builtin_amdgcn_readfirstlane(iters) > lim.max_loop) {
@@ -728,16 +745,10 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
         const uint64_t escm = esc_mask<L, LOWCUT>(c.v) & actm;
         const uint64_t donem = lt_base_mask<L>(c.v, c.n_lo, qb_hi) & actm & ~escm;
         if (__builtin_expect(escm != 0, 0)) {
-          if (in_mask(escm)) {
-            // (The empty asm keeps the index sum in this rare branch: hoisted, it would
-            // cost two instructions on every refill.)
-            uint32_t off = cidx;
-            asm volatile("" : "+v"(off));
-            push_escalate(lists, c.n_lo, qb_hi, first + off, slot);
-          }
+          if (in_mask(escm)) push_escalate(lists, c.n_lo, qb_hi, slot);
         }
         if (donem) {
-          if (in_mask(donem)) finish<L>(c, acc, sv.k);
+          if (in_mask(donem)) finish<L>(c, acc, 0);  // (the step count includes k)
         }
         actm &= ~(escm | donem);
         if (u + 1 < U) {

@@ -119,3 +119,33 @@ def test_two_servers_and_direct_calls_share_a_sieve(s20):
     assert not errors, errors
     a.close()
     b.close()
+
+
+def test_server_tunes_in_idle_time_only():
+    """A launch whose (word width, magnitude) bucket has no tuned shape runs the default
+    shape and queues the bucket's sweep, which the server runs while idle, one timing launch
+    at a time: results stay those of the range alone, and a request that arrives during the
+    sweep waits at most for one timing launch (a few ms), not for the whole sweep (some 80
+    launches)."""
+    import collatz_amd as cz
+    sv = cz.Sieve(20)  # fresh: nothing tuned yet
+    srv = cz.Server(sv, chunk_log2=24)
+    # About 7.8e6 candidates: above the size the tuner tunes for (2^22).
+    big = ((1 << 50) + 7, (1 << 50) + (1 << 29))
+    total, chunks, _ = srv.wait(srv.submit(*big), 60000)
+    assert_chunks_equal(chunks, sv.verify_range(*big, 24).chunks, "big request")
+    lat = []
+    for i in range(12):
+        time.sleep(0.02)  # idle: the queued sweep advances
+        lo = (1 << 44) + i * (1 << 22)
+        total, chunks, (ts, ta, td) = srv.wait(srv.submit(lo, lo + (1 << 22)), 60000)
+        assert_chunks_equal(chunks, sv.verify_range(lo, lo + (1 << 22), 24).chunks,
+                            f"request {i}")
+        lat.append(td - ts)
+    # Once more in the same bucket, after the sweep has had time to finish.
+    time.sleep(1.0)
+    total, chunks, _ = srv.wait(srv.submit(*big), 60000)
+    assert_chunks_equal(chunks, sv.verify_range(*big, 24).chunks, "big request, tuned")
+    srv.close()
+    assert max(lat) < 150.0, lat
+    sv.close()
