@@ -311,14 +311,13 @@ __device__ __forceinline__ uint64_t esc_mask(const Num<L>& v) {
 // Lanes with number < n (lt_base).
 template <int L>
 __device__ __forceinline__ uint64_t lt_base_mask(const Num<L>& v, uint64_t n_lo, uint64_t n_hi) {
+  const uint64_t lo = ballot(pair(v.w[0], v.w[1]) < n_lo);
   if constexpr (L == 3) {
-    const uint64_t lo = ballot(pair(v.w[0], v.w[1]) < n_lo);
+    // (A uniform branch on n's top word, to skip the < compare below 2^64, moved the tile
+    // cursor and the check count of the persistent kernel into vector registers: slower.)
     const uint32_t nh = (uint32_t)n_hi;
-    // Starts below 2^64 (n's top word 0, uniform over a tile) need two compares.
-    if (nh == 0u) return ballot(v.w[2] == 0u) & lo;
     return ballot(v.w[2] < nh) | (ballot(v.w[2] == nh) & lo);
   } else {
-    const uint64_t lo = ballot(pair(v.w[0], v.w[1]) < n_lo);
     const uint64_t vh = pair(v.w[2], v.w[3]);
     const uint64_t m = ballot(vh < n_hi) | (ballot(vh == n_hi) & lo);
     if constexpr (L > 4) return m & ballot((v.w[4] | v.w[5]) == 0u);
@@ -541,7 +540,10 @@ struct LaStage {
 // prefetched records of idle lanes (clearing their prefetch bits), takes survivors (the
 // tile cursor advances), or leaves the loop; the masks, the cursor and the exit test are
 // uniform (scalar registers, scalar branches).
-template <int L, bool LOWCUT, bool LA, bool BOUNDED = LA>
+// UC: the checks per refill as a compile-time constant (1, 2 or 3), or 0 to take them from
+// U.  With a constant the check loop unrolls, and with one check per refill the record set-up
+// for starts between checks (hoisted ahead of the loop) goes away.
+template <int L, bool LOWCUT, bool LA, bool BOUNDED = LA, int UC = 0>
 __global__ __launch_bounds__(256) void k_verify_persistent(
     SieveArrays sv, const RowDesc* rows, const TileDesc* tiles, Ctl* ctl, Limits lim,
     const uint64_t* pow3, DevRec* tile_recs, DeviceLists lists, uint32_t U) {
@@ -736,7 +738,8 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
       //     Every lane runs the check (the SIMD issues it for all lanes either way); a lane
       //     without a candidate computes on stale values and its results are dropped, so
       //     the check's offers run in uniform control flow.
-      for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t Un = UC > 0 ? (uint32_t)UC : U;  // (a constant when UC is set)
+      for (uint32_t u = 0; u < Un; ++u) {
         sink.live = actm;
         wave_chk += (uint32_t)__popcll(actm);
         bool esc_lane, done_lane;  // (unused here: the masks below are taken instead)
@@ -751,7 +754,7 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
           if (in_mask(donem)) finish<L>(c, acc, 0);  // (the step count includes k)
         }
         actm &= ~(escm | donem);
-        if (u + 1 < U) {
+        if (u + 1 < Un) {
           const uint64_t S = ~actm & pfm;
           if (S) start_prefetched(S);
         }
@@ -855,7 +858,18 @@ static void launch_persistent_lb(int limbs, const SieveArrays& sv, const RowDesc
                                  const Limits& lim, const uint64_t* pow3, DevRec* tile_recs,
                                  DeviceLists lists, hipStream_t s) {
   const dim3 grid(nblocks), block(256);
-  if (limbs == 1)
+  if (limbs == 1 && !LA && !BOUNDED && U >= 1 && U <= 3) {
+    // The hot form (two-limb work below 2^68): the tuner's checks per refill compiled in.
+    if (U == 1)
+      hipLaunchKernelGGL((k_verify_persistent<3, false, false, false, 1>), grid, block, 0, s,
+                         sv, rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
+    else if (U == 2)
+      hipLaunchKernelGGL((k_verify_persistent<3, false, false, false, 2>), grid, block, 0, s,
+                         sv, rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
+    else
+      hipLaunchKernelGGL((k_verify_persistent<3, false, false, false, 3>), grid, block, 0, s,
+                         sv, rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
+  } else if (limbs == 1)
     hipLaunchKernelGGL((k_verify_persistent<3, false, LA, BOUNDED>), grid, block, 0, s, sv,
                        rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
   else if (limbs == 2 && lim.cutoff >= kWideLog2)
