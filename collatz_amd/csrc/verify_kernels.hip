@@ -540,7 +540,7 @@ struct LaStage {
 // prefetched records of idle lanes (clearing their prefetch bits), takes survivors (the
 // tile cursor advances), or leaves the loop; the masks, the cursor and the exit test are
 // uniform (scalar registers, scalar branches).
-// UC: the checks per refill as a compile-time constant (1, 2 or 3), or 0 to take them from
+// UC: the checks per refill as a compile-time constant (1 to 4), or 0 to take them from
 // U.  With a constant the check loop unrolls, and with one check per refill the record set-up
 // for starts between checks (hoisted ahead of the loop) goes away.
 template <int L, bool LOWCUT, bool LA, bool BOUNDED = LA, int UC = 0>
@@ -858,17 +858,25 @@ static void launch_persistent_lb(int limbs, const SieveArrays& sv, const RowDesc
                                  const Limits& lim, const uint64_t* pow3, DevRec* tile_recs,
                                  DeviceLists lists, hipStream_t s) {
   const dim3 grid(nblocks), block(256);
-  if (limbs == 1 && !LA && !BOUNDED && U >= 1 && U <= 3) {
-    // The hot form (two-limb work below 2^68): the tuner's checks per refill compiled in.
-    if (U == 1)
-      hipLaunchKernelGGL((k_verify_persistent<3, false, false, false, 1>), grid, block, 0, s,
+  // The hot forms (two-limb work below 2^68, with or without the look-ahead; without it only
+  // where no loop bound is set): the tuner's checks per refill compiled in (its grid is 1..3,
+  // and 1..4 with the look-ahead).
+  constexpr bool kHot = LA || !BOUNDED;
+  constexpr uint32_t kMaxUC = LA ? 4 : 3;
+  if (kHot && limbs == 1 && U >= 1 && U <= kMaxUC) {
+    if constexpr (!kHot) return;  // (no instantiations for the forms that are not hot)
+    else if (U == 1)
+      hipLaunchKernelGGL((k_verify_persistent<3, false, LA, BOUNDED, 1>), grid, block, 0, s,
                          sv, rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
     else if (U == 2)
-      hipLaunchKernelGGL((k_verify_persistent<3, false, false, false, 2>), grid, block, 0, s,
+      hipLaunchKernelGGL((k_verify_persistent<3, false, LA, BOUNDED, 2>), grid, block, 0, s,
+                         sv, rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
+    else if (U == 3)
+      hipLaunchKernelGGL((k_verify_persistent<3, false, LA, BOUNDED, 3>), grid, block, 0, s,
                          sv, rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
     else
-      hipLaunchKernelGGL((k_verify_persistent<3, false, false, false, 3>), grid, block, 0, s,
-                         sv, rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
+      hipLaunchKernelGGL((k_verify_persistent<3, false, LA, BOUNDED, kMaxUC>), grid, block, 0,
+                         s, sv, rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
   } else if (limbs == 1)
     hipLaunchKernelGGL((k_verify_persistent<3, false, LA, BOUNDED>), grid, block, 0, s, sv,
                        rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
@@ -891,12 +899,18 @@ static void launch_persistent_la(int limbs, const SieveArrays& sv, const RowDesc
                                  const TileDesc* tiles, Ctl* ctl, uint32_t nblocks, uint32_t U,
                                  const Limits& lim, const uint64_t* pow3, DevRec* tile_recs,
                                  DeviceLists lists, hipStream_t s) {
-  if (LA || lim.max_loop < kMaxLoop)
-    launch_persistent_lb<LA, true>(limbs, sv, rows, tiles, ctl, nblocks, U, lim, pow3,
-                                   tile_recs, lists, s);
-  else
-    launch_persistent_lb<LA, false>(limbs, sv, rows, tiles, ctl, nblocks, U, lim, pow3,
-                                    tile_recs, lists, s);
+  // (The look-ahead form always carries the bound.)
+  if constexpr (LA) {
+    launch_persistent_lb<true, true>(limbs, sv, rows, tiles, ctl, nblocks, U, lim, pow3,
+                                     tile_recs, lists, s);
+  } else {
+    if (lim.max_loop < kMaxLoop)
+      launch_persistent_lb<false, true>(limbs, sv, rows, tiles, ctl, nblocks, U, lim, pow3,
+                                        tile_recs, lists, s);
+    else
+      launch_persistent_lb<false, false>(limbs, sv, rows, tiles, ctl, nblocks, U, lim, pow3,
+                                         tile_recs, lists, s);
+  }
 }
 
 hipError_t launch_verify_persistent(int limbs, const SieveArrays& sv, const RowDesc* rows,
