@@ -45,6 +45,51 @@ def test_bit_vectors_cuda_differ_only_by_unsigned_compare(bv_cuda_n45, bv_opencl
     assert np.array_equal(bc[15], bo[15])
 
 
+def test_bit_vectors_cuda_differ_only_where_the_unsigned_compare_flips(bv_cuda_n45,
+                                                                      bv_opencl_n45, oracle):
+    """The two programs' precompute() differ in one compare: the path-merge test
+    curval + (DENOMINATOR - NUMERATOR) > maxval, signed in collatz_cudav7.cu:461 and unsigned
+    in collatz_openclv2.cpp:331 (the odd-even-even test beside it never assigns in the C++
+    programs, whose doubleeven() is always false).  Unsigned, a negative curval + 179
+    compares above any maxval >= 0 and becomes maxval, which can only lower it; so every bit
+    the OpenCL vectors have beyond the CUDA ones must belong to a residue n0 whose 24-step
+    parity word reaches a negative path-merge term, and the CUDA vectors have no bit the
+    OpenCL ones lack.  Bit j of BV_a holds residue n0 = j 3^(TPO + a) mod 2^24 (its index is
+    n0 times the magic number 3^-(TPO + a)).  The parity word and running value here follow
+    the definitions (curval += 306 per even step, -= 179 per odd one), not the oracle's
+    code."""
+    _, bc, _ = bv_cuda_n45
+    _, bo, _ = bv_opencl_n45
+    B, mask = 24, (1 << 24) - 1
+    tpo = oracle.tpo(45 - 10)
+
+    def negative_merge_term(n0):
+        n, cur, evens = n0, 0, 0
+        for _ in range(B):
+            if n % 2 == 0:  # an "up" step of the shifted map: n -> 3n/2
+                n += n // 2
+                cur -= 179
+                evens = 0
+            else:
+                n = n // 2 + 1
+                cur += 306
+                evens += 1
+                if evens % 2 == 0 and cur + 179 < 0:
+                    return True
+        return False
+
+    extra = 0
+    for a in range(16):
+        wc, wo = bc[a].view(np.uint32), bo[a].view(np.uint32)
+        assert not np.any(wc & ~wo), a
+        idx = np.nonzero(np.unpackbits((wo & ~wc).view(np.uint8), bitorder="little"))[0]
+        extra += len(idx)
+        p = pow(3, tpo + a, 1 << B)
+        for j in idx.tolist():
+            assert negative_merge_term((j * p) & mask), (a, j)
+    assert extra > 0
+
+
 def test_mod9_rows(golden, oracle):
     rows = oracle.mod9_rows(35, 10)
     assert rows.view(np.uint32).reshape(9, -1).tolist() == golden["mod9"]
