@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <unordered_map>
 
 namespace cz {
 
@@ -66,6 +67,72 @@ int plan_interval(const collatz_sieve* sv, u128 lo, u128 hi, uint32_t C, uint64_
   return COLLATZ_OK;
 }
 
+// Rows of a group interleaved at a time (a group of [2^60, 2^60 + 2^44) with k = 37 is its
+// 128 q; a batch of 4096 intervals has groups of thousands of rows, and interleaving all of
+// them at once put every wave of an XCD on the same records: 328 -> 339 ms per launch).
+constexpr uint32_t kTileBand = 128;
+// Tiles per resident wave from which a launch uses the per-XCD shares of the tile order.
+constexpr uint32_t kSharesMinTilesPerWave = 16;
+
+uint64_t tile_order(const LaunchPlan& lp, uint32_t* order, uint32_t band) {
+  const size_t n = lp.rows.size();
+  // A row's survivor window: its class and its two bounds, each either a class boundary
+  // (an index) or a residue key resolved on the device (its value); tagged apart.
+  struct Key {
+    uint64_t lo, hi;
+    uint32_t cls;
+    bool operator==(const Key& o) const { return lo == o.lo && hi == o.hi && cls == o.cls; }
+  };
+  struct KeyHash {
+    size_t operator()(const Key& k) const {
+      uint64_t h = k.lo * 0x9E3779B97F4A7C15ull;
+      h ^= (k.hi + 0x632BE59BD9B4E019ull) * 0xBF58476D1CE4E5B9ull;
+      h ^= (uint64_t)k.cls * 0x94D049BB133111EBull;
+      return (size_t)(h ^ (h >> 31));
+    }
+  };
+  auto bound = [&](uint64_t b) -> uint64_t {
+    return (b & kResolveFlag) ? (lp.keys[b & ~kResolveFlag] << 1) | 1u : b << 1;
+  };
+  std::unordered_map<Key, uint32_t, KeyHash> gid;
+  gid.reserve(n);
+  std::vector<uint32_t> grp(n), gsize, gpieces;
+  for (size_t r = 0; r < n; ++r) {
+    const RowDesc& row = lp.rows[r];
+    const Key k{bound(row.i_lo), bound(row.i_hi), row.cls};
+    auto it = gid.emplace(k, (uint32_t)gsize.size()).first;
+    if (it->second == gsize.size()) {
+      gsize.push_back(0);
+      gpieces.push_back(0);
+    }
+    grp[r] = it->second;
+    gsize[it->second]++;
+    const uint64_t e = r + 1 < n ? lp.rows[r + 1].block_start : lp.ntiles;
+    gpieces[it->second] =
+        std::max<uint32_t>(gpieces[it->second], (uint32_t)(e - lp.rows[r].block_start));
+  }
+  // The rows of each group, in row order (a counting sort by group).
+  std::vector<uint32_t> gstart(gsize.size() + 1, 0), members(n);
+  for (size_t g = 0; g < gsize.size(); ++g) gstart[g + 1] = gstart[g] + gsize[g];
+  std::vector<uint32_t> fill(gstart.begin(), gstart.end() - 1);
+  for (size_t r = 0; r < n; ++r) members[fill[grp[r]]++] = (uint32_t)r;
+  uint64_t pos = 0;
+  for (size_t g = 0; g < gsize.size(); ++g) {
+    for (uint64_t m0 = gstart[g]; m0 < gstart[g + 1]; m0 += band) {
+      const uint64_t m1 = std::min<uint64_t>(gstart[g + 1], m0 + band);
+      for (uint32_t j = 0; j < gpieces[g]; ++j) {
+        for (uint64_t m = m0; m < m1; ++m) {
+          const uint32_t r = members[m];
+          const uint64_t b = lp.rows[r].block_start;
+          const uint64_t e = r + 1 < n ? lp.rows[r + 1].block_start : lp.ntiles;
+          if (j < e - b) order[pos++] = (uint32_t)(b + j);
+        }
+      }
+    }
+  }
+  return pos;
+}
+
 bool Engine::GraphKey::operator==(const GraphKey& o) const {
   return std::memcmp(bufs, o.bufs, sizeof(bufs)) == 0 && limbs == o.limbs &&
          cutoff == o.cutoff && nblocks == o.nblocks && U == o.U && max_steps == o.max_steps &&
@@ -86,6 +153,8 @@ int Engine::ensure(const LaunchPlan& lp, uint64_t ovf_cap) {
   CZ_HIP(rows_.reserve(std::max<size_t>(1, lp.rows.size())));
   CZ_HIP(keys_.reserve(std::max<size_t>(1, lp.keys.size())));
   CZ_HIP(tiles_.reserve(std::max<size_t>(1, lp.ntiles)));
+  CZ_HIP(order_.reserve(std::max<size_t>(1, lp.ntiles)));
+  CZ_HIP(h_order_.reserve(std::max<size_t>(1, lp.ntiles)));
   CZ_HIP(part_.reserve(lp.part.size()));
   CZ_HIP(tile_recs_.reserve(std::max<size_t>(1, lp.ntiles)));
   CZ_HIP(h_rows_.reserve(std::max<size_t>(1, lp.rows.size())));
@@ -115,7 +184,8 @@ hipError_t Engine::enqueue(const GraphKey& k, const Limits& lim, DeviceLists lis
   if (e != hipSuccess) return e;
   e = launch_make_tiles(rows_.p, ctl_.p, tiles_.p, s);
   if (e != hipSuccess) return e;
-  e = launch_verify_persistent(k.limbs, sv_->arrays(), rows_.p, tiles_.p, ctl_.p, k.nblocks,
+  e = launch_verify_persistent(k.limbs, sv_->arrays(), rows_.p, tiles_.p, order_.p, ctl_.p,
+                               k.nblocks,
                                k.U, lim, sv_->pow3.p, tile_recs_.p, lists, s);
   if (e != hipSuccess) return e;
   return launch_reduce_slots(tile_recs_.p, part_.p, ctl_.p, slot_recs_.p, s);
@@ -136,13 +206,16 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   std::memcpy(h_rows_.p, lp.rows.data(), lp.rows.size() * sizeof(RowDesc));
   std::memcpy(h_keys_.p, lp.keys.data(), lp.keys.size() * 8);
   std::memcpy(h_part_.p, lp.part.data(), lp.part.size() * 8);
+  if (tile_order(lp, h_order_.p, kTileBand) != lp.ntiles)
+    return fail(COLLATZ_EDEVICE, "internal: the tile order does not list every tile once");
   if (!lp.rows.empty())
     CZ_HIP(hipMemcpyAsync(rows_.p, h_rows_.p, lp.rows.size() * sizeof(RowDesc),
                           hipMemcpyHostToDevice, s));
   if (!lp.keys.empty())
     CZ_HIP(hipMemcpyAsync(keys_.p, h_keys_.p, lp.keys.size() * 8, hipMemcpyHostToDevice, s));
   CZ_HIP(hipMemcpyAsync(part_.p, h_part_.p, lp.part.size() * 8, hipMemcpyHostToDevice, s));
-  CZ_HIP(hipMemcpyAsync(ctl_.p, &ctl, sizeof(Ctl), hipMemcpyHostToDevice, s));
+  if (lp.ntiles)
+    CZ_HIP(hipMemcpyAsync(order_.p, h_order_.p, lp.ntiles * 4, hipMemcpyHostToDevice, s));
 
   DeviceLists lists;
   lists.esc = esc_.p;
@@ -166,12 +239,21 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   key.bufs[5] = slot_recs_.p;
   key.bufs[6] = ovf_.p;
   key.bufs[7] = (const void*)(uintptr_t)ovf_cap;
+  key.bufs[8] = order_.p;
   key.limbs = limbs;
   key.cutoff = cutoff;
   const uint32_t waves =
       std::min<uint32_t>(tn.waves, persistent_waves_fit(limbs, cutoff, sv_->la_bits != 0));
   key.nblocks =
       (uint32_t)std::max<uint64_t>(1, (uint64_t)device_cu_count(sv_->device) * waves / 4);
+  // Per-XCD shares of the tile order only for launches that give every resident wave many
+  // tiles: each share's waves then read their groups' records from their own L2 (on
+  // [2^60, 2^60 + 2^44) with k = 37, 16 -> 1.5 B per candidate), while in a short launch the
+  // extra turns of the exhausted shares' waves at the end cost more than the reads save
+  // ([2^40, 2^40 + 2^36): 1.55 -> 2.17 ms; a served launch of a few requests: p50 0.26 ->
+  // 0.95 ms).
+  ctl.nshares = lp.ntiles >= (uint64_t)kSharesMinTilesPerWave * key.nblocks * 4 ? kXcds : 1;
+  CZ_HIP(hipMemcpyAsync(ctl_.p, &ctl, sizeof(Ctl), hipMemcpyHostToDevice, s));
   key.U = tn.U;
   key.max_steps = lim.max_steps;
   key.max_loop = lim.max_loop;

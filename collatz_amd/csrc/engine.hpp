@@ -66,6 +66,13 @@ struct LaunchResult {
   double device_ms = 0;
 };
 
+// The order in which a launch's tiles are taken (tile indices, lp.ntiles of them): rows that
+// read the same survivors (same residue class and residue bounds, so they differ only in q)
+// form a group, and a group's tiles are listed piece by piece across `band` of its rows at a
+// time, so that the waves that take consecutive tiles read the same records.  Returns the
+// number of tiles listed (lp.ntiles, each once).
+uint64_t tile_order(const LaunchPlan& lp, uint32_t* order, uint32_t band);
+
 // Appends the rows of [lo, hi) (chunk slots slot_base..) and their tiles to the plan.
 int plan_interval(const collatz_sieve* sv, u128 lo, u128 hi, uint32_t C, uint64_t slot_base,
                   uint32_t tile, LaunchPlan& lp, std::vector<uint64_t>& n_checked,
@@ -88,6 +95,7 @@ class Engine {
   DevBuf<RowDesc> rows_;
   DevBuf<uint64_t> keys_, part_, ovf_;
   DevBuf<TileDesc> tiles_;
+  DevBuf<uint32_t> order_;        // tile indices in the order waves take them
   DevBuf<DevRec> tile_recs_, slot_recs_, esc_recs_;
   DevBuf<EscEntry> esc_;
   DevBuf<unsigned long long> cnt_;
@@ -96,6 +104,7 @@ class Engine {
   HostBuf<DevRec> h_recs_;
   HostBuf<RowDesc> h_rows_;
   HostBuf<uint64_t> h_keys_, h_part_;
+  HostBuf<uint32_t> h_order_;
   HostBuf<unsigned long long> h_cnt_;
   HostBuf<Ctl> h_ctl_;
   hipEvent_t e0_ = nullptr, e1_ = nullptr, e2_ = nullptr, e3_ = nullptr;
@@ -103,7 +112,7 @@ class Engine {
   hipGraphExec_t gexec_ = nullptr;
   hipGraph_t graph_ = nullptr;
   struct GraphKey {
-    const void* bufs[8];
+    const void* bufs[9];
     int limbs;
     uint32_t cutoff, nblocks, U, max_steps, max_loop;
     bool operator==(const GraphKey& o) const;

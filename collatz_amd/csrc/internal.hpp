@@ -58,12 +58,18 @@ constexpr uint64_t kResolveFlag = 1ull << 63;
 
 // Sizes of one persistent launch, read by the kernels from device memory so that one
 // captured graph serves launches of any size.
+// XCDs of an MI355X, each with its own L2.  The workgroups of a launch are dispatched to
+// them round-robin: workgroup w runs on XCD w mod kXcds.
+constexpr uint32_t kXcds = 8;
+
 struct Ctl {
   uint64_t nrows;
   uint64_t nslots;
   uint32_t ntiles;
-  uint32_t tile_counter;
-  uint64_t pad;
+  uint32_t nshares;  // the tile order is cut into this many equal shares (1 or kXcds)
+  // Share x's next position (the waves of XCD x mod nshares take from share x first, then
+  // from the others).
+  uint32_t share_next[kXcds];
 };
 
 // Partial or final record of one chunk, as the kernels produce it.  maxnum is the max of
@@ -248,11 +254,14 @@ hipError_t launch_verify_plain(int limbs, const SieveArrays& sv, const RowDesc* 
                                uint32_t nrows, uint64_t nblocks, uint32_t R, uint32_t mod9,
                                const Limits& lim, const uint64_t* pow3, DevRec* block_recs,
                                DeviceLists lists, hipStream_t s);
+// `order` lists the ctl->ntiles tile indices in the order they are to be taken (tile t's
+// record goes to tile_recs[t] whatever its position).
 hipError_t launch_verify_persistent(int limbs, const SieveArrays& sv, const RowDesc* rows,
-                                    const TileDesc* tiles, Ctl* ctl, uint32_t nblocks,
-                                    uint32_t U, const Limits& lim, const uint64_t* pow3,
-                                    DevRec* tile_recs, DeviceLists lists, hipStream_t s);
-// Clears the tile counter and the list counters with a kernel (no memset nodes, so the
+                                    const TileDesc* tiles, const uint32_t* order, Ctl* ctl,
+                                    uint32_t nblocks, uint32_t U, const Limits& lim,
+                                    const uint64_t* pow3, DevRec* tile_recs, DeviceLists lists,
+                                    hipStream_t s);
+// Clears the tile counters and the list counters with a kernel (no memset nodes, so the
 // sequence can be graph-captured).
 // Waves per CU that stay resident for the persistent kernel of this width and cutoff.
 uint32_t persistent_waves_fit(int limbs, uint32_t cutoff, bool la);

@@ -545,8 +545,9 @@ struct LaStage {
 // for starts between checks (hoisted ahead of the loop) goes away.
 template <int L, bool LOWCUT, bool LA, bool BOUNDED = LA, int UC = 0>
 __global__ __launch_bounds__(256) void k_verify_persistent(
-    SieveArrays sv, const RowDesc* rows, const TileDesc* tiles, Ctl* ctl, Limits lim,
-    const uint64_t* pow3, DevRec* tile_recs, DeviceLists lists, uint32_t U) {
+    SieveArrays sv, const RowDesc* rows, const TileDesc* tiles, const uint32_t* order,
+    Ctl* ctl, Limits lim, const uint64_t* pow3, DevRec* tile_recs, DeviceLists lists,
+    uint32_t U) {
   __shared__ WaveTables<L, LA> tabs[kWavesPerBlock];
   __shared__ LaStage stages[LA ? kWavesPerBlock : 1];
   __shared__ Pow3Lds p3s;
@@ -556,13 +557,25 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
   const uint32_t lane = threadIdx.x & 63;
   WaveTables<L, LA>& T = tabs[wv];
   const uint32_t ntiles = ctl->ntiles;
-  uint32_t* tile_counter = &ctl->tile_counter;
-
+  // The waves of this workgroup (on XCD blockIdx.x mod kXcds) take tiles from that XCD's
+  // share of the tile order, then from the other shares in turn: the order puts tiles that
+  // read the same survivor records next to each other, so the waves of one XCD read them
+  // from its L2.  Every share is left at most once, so the loop takes at most kXcds - 1
+  // empty turns besides its tiles.
+  const uint32_t nshares = ctl->nshares > 1u ? ctl->nshares : 1u;
+  uint32_t share = blockIdx.x % nshares, turns = 0;
   for (;;) {
-    uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(tile_counter, 1u);
-    t = __builtin_amdgcn_readfirstlane(t);
-    if (t >= ntiles) break;
+    const uint32_t s_lo = (uint32_t)((uint64_t)ntiles * share / nshares);
+    const uint32_t s_n = (uint32_t)((uint64_t)ntiles * (share + 1) / nshares) - s_lo;
+    uint32_t pos = 0;
+    if (lane == 0) pos = atomicAdd(&ctl->share_next[share], 1u);
+    pos = __builtin_amdgcn_readfirstlane(pos);
+    if (pos >= s_n) {
+      if (++turns == nshares) break;
+      share = (share + 1) % nshares;
+      continue;
+    }
+    const uint32_t t = order[s_lo + pos];
     const TileDesc td = tiles[t];
     const RowDesc rw = rows[td.row];
     const uint64_t len = rw.i_hi - rw.i_lo;
@@ -854,9 +867,10 @@ uint32_t persistent_waves_fit(int limbs, uint32_t cutoff, bool la) {
 
 template <bool LA, bool BOUNDED>
 static void launch_persistent_lb(int limbs, const SieveArrays& sv, const RowDesc* rows,
-                                 const TileDesc* tiles, Ctl* ctl, uint32_t nblocks, uint32_t U,
-                                 const Limits& lim, const uint64_t* pow3, DevRec* tile_recs,
-                                 DeviceLists lists, hipStream_t s) {
+                                 const TileDesc* tiles, const uint32_t* order, Ctl* ctl,
+                                 uint32_t nblocks, uint32_t U, const Limits& lim,
+                                 const uint64_t* pow3, DevRec* tile_recs, DeviceLists lists,
+                                 hipStream_t s) {
   const dim3 grid(nblocks), block(256);
   // The hot forms (two-limb work below 2^68, with or without the look-ahead; without it only
   // where no loop bound is set): the tuner's checks per refill compiled in (its grid is 1..3,
@@ -867,67 +881,69 @@ static void launch_persistent_lb(int limbs, const SieveArrays& sv, const RowDesc
     if constexpr (!kHot) return;  // (no instantiations for the forms that are not hot)
     else if (U == 1)
       hipLaunchKernelGGL((k_verify_persistent<3, false, LA, BOUNDED, 1>), grid, block, 0, s,
-                         sv, rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
+                         sv, rows, tiles, order, ctl, lim, pow3, tile_recs, lists, U);
     else if (U == 2)
       hipLaunchKernelGGL((k_verify_persistent<3, false, LA, BOUNDED, 2>), grid, block, 0, s,
-                         sv, rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
+                         sv, rows, tiles, order, ctl, lim, pow3, tile_recs, lists, U);
     else if (U == 3)
       hipLaunchKernelGGL((k_verify_persistent<3, false, LA, BOUNDED, 3>), grid, block, 0, s,
-                         sv, rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
+                         sv, rows, tiles, order, ctl, lim, pow3, tile_recs, lists, U);
     else
       hipLaunchKernelGGL((k_verify_persistent<3, false, LA, BOUNDED, kMaxUC>), grid, block, 0,
-                         s, sv, rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
+                         s, sv, rows, tiles, order, ctl, lim, pow3, tile_recs, lists, U);
   } else if (limbs == 1)
     hipLaunchKernelGGL((k_verify_persistent<3, false, LA, BOUNDED>), grid, block, 0, s, sv,
-                       rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
+                       rows, tiles, order, ctl, lim, pow3, tile_recs, lists, U);
   else if (limbs == 2 && lim.cutoff >= kWideLog2)
     hipLaunchKernelGGL((k_verify_persistent<4, false, LA, BOUNDED>), grid, block, 0, s, sv,
-                       rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
+                       rows, tiles, order, ctl, lim, pow3, tile_recs, lists, U);
   else if (limbs == 2)
     hipLaunchKernelGGL((k_verify_persistent<4, true, LA, BOUNDED>), grid, block, 0, s, sv,
-                       rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
+                       rows, tiles, order, ctl, lim, pow3, tile_recs, lists, U);
   else if (lim.cutoff >= 128)
     hipLaunchKernelGGL((k_verify_persistent<6, false, LA, BOUNDED>), grid, block, 0, s, sv,
-                       rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
+                       rows, tiles, order, ctl, lim, pow3, tile_recs, lists, U);
   else
     hipLaunchKernelGGL((k_verify_persistent<6, true, LA, BOUNDED>), grid, block, 0, s, sv,
-                       rows, tiles, ctl, lim, pow3, tile_recs, lists, U);
+                       rows, tiles, order, ctl, lim, pow3, tile_recs, lists, U);
 }
 
 template <bool LA>
 static void launch_persistent_la(int limbs, const SieveArrays& sv, const RowDesc* rows,
-                                 const TileDesc* tiles, Ctl* ctl, uint32_t nblocks, uint32_t U,
-                                 const Limits& lim, const uint64_t* pow3, DevRec* tile_recs,
-                                 DeviceLists lists, hipStream_t s) {
+                                 const TileDesc* tiles, const uint32_t* order, Ctl* ctl,
+                                 uint32_t nblocks, uint32_t U, const Limits& lim,
+                                 const uint64_t* pow3, DevRec* tile_recs, DeviceLists lists,
+                                 hipStream_t s) {
   // (The look-ahead form always carries the bound.)
   if constexpr (LA) {
-    launch_persistent_lb<true, true>(limbs, sv, rows, tiles, ctl, nblocks, U, lim, pow3,
-                                     tile_recs, lists, s);
+    launch_persistent_lb<true, true>(limbs, sv, rows, tiles, order, ctl, nblocks, U, lim,
+                                     pow3, tile_recs, lists, s);
   } else {
     if (lim.max_loop < kMaxLoop)
-      launch_persistent_lb<false, true>(limbs, sv, rows, tiles, ctl, nblocks, U, lim, pow3,
-                                        tile_recs, lists, s);
+      launch_persistent_lb<false, true>(limbs, sv, rows, tiles, order, ctl, nblocks, U, lim,
+                                        pow3, tile_recs, lists, s);
     else
-      launch_persistent_lb<false, false>(limbs, sv, rows, tiles, ctl, nblocks, U, lim, pow3,
-                                         tile_recs, lists, s);
+      launch_persistent_lb<false, false>(limbs, sv, rows, tiles, order, ctl, nblocks, U, lim,
+                                         pow3, tile_recs, lists, s);
   }
 }
 
 hipError_t launch_verify_persistent(int limbs, const SieveArrays& sv, const RowDesc* rows,
-                                    const TileDesc* tiles, Ctl* ctl, uint32_t nblocks,
-                                    uint32_t U, const Limits& lim, const uint64_t* pow3,
-                                    DevRec* tile_recs, DeviceLists lists, hipStream_t s) {
+                                    const TileDesc* tiles, const uint32_t* order, Ctl* ctl,
+                                    uint32_t nblocks, uint32_t U, const Limits& lim,
+                                    const uint64_t* pow3, DevRec* tile_recs, DeviceLists lists,
+                                    hipStream_t s) {
   if (sv.la)
-    launch_persistent_la<true>(limbs, sv, rows, tiles, ctl, nblocks, U, lim, pow3, tile_recs,
-                               lists, s);
+    launch_persistent_la<true>(limbs, sv, rows, tiles, order, ctl, nblocks, U, lim, pow3,
+                               tile_recs, lists, s);
   else
-    launch_persistent_la<false>(limbs, sv, rows, tiles, ctl, nblocks, U, lim, pow3, tile_recs,
-                                lists, s);
+    launch_persistent_la<false>(limbs, sv, rows, tiles, order, ctl, nblocks, U, lim, pow3,
+                                tile_recs, lists, s);
   return hipGetLastError();
 }
 
 __global__ void k_engine_reset(Ctl* ctl, unsigned long long* cnt, uint32_t ncnt) {
-  if (threadIdx.x == 0) ctl->tile_counter = 0;
+  if (threadIdx.x < kXcds) ctl->share_next[threadIdx.x] = 0;
   if (threadIdx.x < ncnt) cnt[threadIdx.x] = 0;
 }
 
