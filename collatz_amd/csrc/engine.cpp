@@ -252,7 +252,8 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   // extra turns of the exhausted shares' waves at the end cost more than the reads save
   // ([2^40, 2^40 + 2^36): 1.55 -> 2.17 ms; a served launch of a few requests: p50 0.26 ->
   // 0.95 ms).
-  ctl.nshares = lp.ntiles >= (uint64_t)kSharesMinTilesPerWave * key.nblocks * 4 ? kXcds : 1;
+  static_assert(kXcds == 8, "shares are 2^share_log2 with share_log2 = 3");
+  ctl.share_log2 = lp.ntiles >= (uint64_t)kSharesMinTilesPerWave * key.nblocks * 4 ? 3 : 0;
   CZ_HIP(hipMemcpyAsync(ctl_.p, &ctl, sizeof(Ctl), hipMemcpyHostToDevice, s));
   key.U = tn.U;
   key.max_steps = lim.max_steps;

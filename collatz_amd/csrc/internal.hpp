@@ -66,10 +66,15 @@ struct Ctl {
   uint64_t nrows;
   uint64_t nslots;
   uint32_t ntiles;
-  uint32_t nshares;  // the tile order is cut into this many equal shares (1 or kXcds)
-  // Share x's next position (the waves of XCD x mod nshares take from share x first, then
-  // from the others).
-  uint32_t share_next[kXcds];
+  // The tile order is cut into 2^share_log2 equal shares (1 or kXcds): share x is positions
+  // [ntiles x >> share_log2, ntiles (x + 1) >> share_log2).  The waves of XCD
+  // x mod 2^share_log2 take from share x first, then from the others.  (A power of two, so
+  // that the kernel needs no division: a 64-bit one is a long VALU routine whose registers
+  // count for the whole kernel.)
+  uint32_t share_log2;
+  // Next position of each share, on a cache line of their own: every wave's atomics hit
+  // them, and the fields above are read by the kernels.
+  alignas(64) uint32_t share_next[kXcds];
 };
 
 // Partial or final record of one chunk, as the kernels produce it.  maxnum is the max of

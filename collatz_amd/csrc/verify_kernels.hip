@@ -86,7 +86,10 @@ __device__ __forceinline__ void words_apart(uint32_t* x) {
 // code: a larger value wins, an equal one with a smaller start.  A lane-private max would
 // start from zero in every tile and take a divergent update path many times per tile; the
 // wave's max needs the merge about log(offers) times.
-template <int L>
+// KEYMAX: merge many reaching lanes through a cross-lane max of their top-64-bit key first
+// (the plain kernel, whose waves take short shares; in the persistent kernels its registers
+// would cost occupancy).
+template <int L, bool KEYMAX = false>
 struct SinkWave {
   uint32_t bw[L];         // the wave's best value (0: none yet; every value is >= 2)
   uint64_t bs_lo, bs_hi;  // its start
@@ -117,7 +120,27 @@ struct SinkWave {
 #pragma unroll
       for (int i = 0; i < L; ++i) x[i] = v.w[i];
       words_apart<L>(x);
-      for (uint64_t pend = pend0; pend; pend &= pend - 1) {
+      uint64_t pend1 = pend0;
+      if constexpr (KEYMAX && L != 6) {
+        // Many reaching lanes (at the start of a tile, when the best is still low): only the
+        // lanes holding the largest top-64-bit key can hold the largest value, so a
+        // cross-lane max of the key (six shuffle rounds) cuts the one-at-a-time merge below
+        // to them, usually one lane.
+        if (__popcll(pend0) > 2) {
+          constexpr int k = L - 2;
+          const uint64_t own = pair(x[k], x[k + 1]);
+          uint64_t key = __builtin_amdgcn_inverse_ballot_w64(pend0) ? own : 0;
+#pragma unroll
+          for (int m = 32; m >= 1; m >>= 1) {
+            const uint64_t o = shfl_xor64(key, m);
+            key = o > key ? o : key;
+          }
+          const uint64_t kmax = pair(__builtin_amdgcn_readfirstlane((uint32_t)key),
+                                     __builtin_amdgcn_readfirstlane((uint32_t)(key >> 32)));
+          pend1 = __builtin_amdgcn_ballot_w64(own == kmax) & pend0;
+        }
+      }
+      for (uint64_t pend = pend1; pend; pend &= pend - 1) {
         const uint32_t l = (uint32_t)__builtin_ctzll(pend);
         uint32_t cw[L];
 #pragma unroll
@@ -348,22 +371,26 @@ __device__ __forceinline__ bool run_candidate(Cand<L>& c, Acc<L>& acc, const Lim
 
 // ---- plain kernel: a fixed share of one row per wave, lanes refilled --------------------
 // Block b covers up to 256 R consecutive survivors of one row (the row whose block range
-// holds b); each of its four waves takes a quarter of them.  A lane runs one candidate at
-// a time, one check per loop iteration.  A lane whose candidate has ended (descended,
-// escalated or abandoned at the check bound) starts the survivor record it loaded one
-// iteration earlier, and lanes without a loaded record take the next survivors of the
-// wave's share (consecutive indices by rank among them), so no lane waits for its wave's
-// slowest trajectory until the share is used up, and the record's load overlaps a check.
-// The loop ends when no lane holds a candidate or a record and the share is used up.  It
-// also ends at lim.max_loop iterations (counted in lists.loop_aborts: the host then fails
-// the call); a wave's loop normally runs about share / 64 x (checks per candidate)
-// iterations plus its tail, far below the default bound of 2^31.
+// holds b); each of its four waves takes a quarter of them.  No state outlives the launch
+// and no work moves between waves: this is the drop-in kernel of collatz_verify_range.  A
+// lane runs one candidate at a time, one check per loop iteration; a lane whose candidate
+// has ended (descended, escalated, or abandoned at the step bound) starts the survivor record
+// it loaded one iteration earlier, and the lanes without a loaded record take the next
+// survivors of the wave's share by rank among them, so no lane waits for its wave's slowest
+// trajectory until the share is used up.  As in the persistent kernel below, which lanes run
+// a candidate and which hold a record are two uniform masks (scalar registers), every lane
+// runs the check (a lane without a candidate on stale values, its effects dropped), the
+// wave's max excursion is kept in scalar registers, the step count starts at k, and on three
+// words a row at or above the start bound holds 2^95 in its q 3^f table.  The loop ends when
+// no lane holds a candidate or a record and the share is used up, or at lim.max_loop
+// iterations (counted in lists.loop_aborts: the host then fails the call).
 template <int L, bool LOWCUT>
 __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowDesc* rows,
                                                       uint32_t nrows, uint32_t R, uint32_t mod9,
                                                       Limits lim, const uint64_t* pow3,
                                                       DevRec* recs, DeviceLists lists) {
-  __shared__ uint32_t q3[41][6];
+  __shared__ uint32_t q3[41][L];
+  __shared__ uint32_t q3lo[41];  // low word of q 3^f (the look-ahead index)
   __shared__ DevRec lrec[4];
   __shared__ Pow3Lds p3s;
   stage_pow3(p3s);
@@ -375,25 +402,53 @@ __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowD
     else hi = mid;
   }
   const RowDesc row = rows[lo];
-  if (threadIdx.x < 41) q_times_pow3(row.q_lo, row.q_hi, pow3[threadIdx.x], q3[threadIdx.x]);
+  if (threadIdx.x < 41) {
+    uint32_t w6[6];
+    q_times_pow3(row.q_lo, row.q_hi, pow3[threadIdx.x], w6);
+    uint32_t big = 0;
+#pragma unroll
+    for (int i = L; i < 6; ++i) big |= w6[i];
+    if constexpr (L == 3) {
+      // (As in the persistent kernel: a start M + q 3^f then cannot carry out of three words
+      // and fails the start test exactly when q 3^f reaches 2^86.)
+      big |= (uint32_t)(w6[2] >= (1u << (kNarrowCheckLog2 - 64)));
+      q3[threadIdx.x][0] = big ? 0u : w6[0];
+      q3[threadIdx.x][1] = big ? 0u : w6[1];
+      q3[threadIdx.x][2] = big ? 0x80000000u : w6[2];
+    } else {
+#pragma unroll
+      for (int i = 0; i < L; ++i) q3[threadIdx.x][i] = big ? 0xFFFFFFFFu : w6[i];
+    }
+    q3lo[threadIdx.x] = w6[0];
+  }
   __syncthreads();
+  const uint32_t lane = threadIdx.x & 63u;
 
   Acc<L> acc;
   acc.init();
-  SinkOne<L> sink{acc};
+  SinkWave<L, true> sink;
+  sink.clear();
+  uint32_t wave_chk = 0;
   // The block's survivors [bfirst, bfirst + blen), split evenly over its four waves: the
-  // wave's share is [next, end), empty where a block holds fewer survivors than waves.
+  // wave's share is [next, end) (uniform), empty where a block holds fewer survivors than
+  // waves.
+  // (The wave index through readfirstlane: the compiler does not know threadIdx.x >> 6 is
+  // uniform in a wave, and would keep the share bounds, the masks and the cursor in vector
+  // registers.)
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint64_t bfirst = row.i_lo + (blk - row.block_start) * (256ull * R);
   const uint64_t blen = row.i_hi - bfirst < 256ull * R ? row.i_hi - bfirst : 256ull * R;
   const uint64_t share = (blen + 3) >> 2;
-  uint64_t next = bfirst + (threadIdx.x >> 6) * share;
+  uint64_t next = bfirst + wv * share;
   const uint64_t end = next + share < bfirst + blen ? next + share : bfirst + blen;
+  const uint64_t qb_lo = row.qb_lo, qb_hi = row.qb_hi;
+  const uint32_t slot = row.slot;
   Cand<L> c;
-  c.slot = row.slot;
-  uint64_t cur_i = 0, pf_i = 0;  // survivor index of the candidate and of the loaded record
+  c.n_hi = qb_hi;
+  c.slot = slot;
+  uint64_t actm = 0, pfm = 0;  // lanes with a running candidate, lanes with a loaded record
   uint4 pf = make_uint4(0, 0, 0, 0);
-  bool have = false, pf_ok = false;
-  uint32_t nchk = 0;  // checks of the lane's candidate so far
+  const uint32_t step_lim = lim.max_steps + sv.k;  // (the step count starts at k)
   uint32_t iters = 0;
   bool aborted = false;
   for (;;) {
@@ -401,61 +456,86 @@ __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowD
       aborted = true;
       break;
     }
-    // Start the loaded record.
-    if (!have && pf_ok) {
-      pf_ok = false;
-      const uint64_t r = rec_r(pf);
-      const uint64_t M = rec_M(pf);
-      const uint32_t f = rec_f(pf);
-      if (!sv.la || la_keep(sv, f, (uint32_t)M + q3[f][0])) {
-        uint32_t bigw = 0;
-#pragma unroll
-        for (int i = L; i < 6; ++i) bigw |= q3[f][i];
-        cur_i = pf_i;
-        if (cand_start<L>(c, M, q3[f], bigw != 0, row.qb_lo, row.qb_hi, r) &&
-            cand_enter<L, LOWCUT>(c)) {
-          have = true;
-          nchk = 0;
-        } else {
-          push_escalate(lists, c.n_lo, c.n_hi, row.slot);
+    // (1) Lanes without a candidate start their loaded record (a look-ahead sieve first
+    //     tests it; a record it drops is simply not started).
+    {
+      const uint64_t S = ~actm & pfm;
+      if (S) {
+        bool keep = true;
+        if (in_mask(S)) {
+          const uint32_t f = rec_f(pf);
+          const uint64_t M = rec_M(pf);
+          if (sv.la) keep = la_keep(sv, f, (uint32_t)M + q3lo[f]);
+          const bool fits = cand_start<L>(c, M, q3[f], false, qb_lo, qb_hi, rec_r(pf), sv.k);
+          if constexpr (L != 3) {
+            if (!fits) c.v.w[L - 1] = 0xFFFFFFFFu;
+          }
         }
+        const uint64_t kept = ballot(keep) & S;
+        const uint64_t ok = start_ok_mask<L, LOWCUT>(c.v) & kept;
+        const uint64_t bad = kept & ~ok;
+        if (__builtin_expect(bad != 0, 0)) {
+          if (in_mask(bad)) push_escalate(lists, c.n_lo, qb_hi, slot);
+        }
+        actm |= ok;
+        pfm &= ~S;
       }
     }
-    // Lanes without a record load the next survivors of the share.
-    const uint64_t want = ballot(!pf_ok);
-    if (want != 0 && next < end) {
-      const uint32_t rank = lane_rank(want);
-      const uint64_t avail = end - next;
-      const uint32_t nwant = (uint32_t)__popcll(want);
-      const uint32_t take = avail < nwant ? (uint32_t)avail : nwant;
-      if (!pf_ok && rank < take) {
-        pf_i = next + rank;
-        pf = sv.rec[pf_i];
-        pf_ok = true;
-      }
+    // (2) Lanes without a loaded record take the next survivors of the share.
+    const uint64_t need = ~pfm;
+    if (need && next < end) {
+      const uint32_t want = (uint32_t)__popcll(need);
+      const uint32_t avail = (uint32_t)(end - next);  // (a share is at most 2^18 survivors)
+      const uint32_t take = avail < want ? avail : want;
+      const uint32_t rank = lane_rank(need);
+      const uint64_t tk = ballot(rank < take) & need;
+      if (in_mask(tk)) pf = sv.rec[next + rank];
+      pfm |= tk;
       next += take;
     }
-    // One check.
-    if (have) {
-      bool esc, done;
-      one_check<L, LOWCUT>(c, c.n_hi, sink, lim, lists, p3s, esc, done);
-      acc.chk += 1;
-      if (esc) {
-        push_escalate(lists, c.n_lo, c.n_hi, row.slot);
-        have = false;
-      } else if (done) {
-        finish<L>(c, acc, sv.k);
-        have = false;
-      } else if (++nchk >= lim.max_checks) {
-        push_stall(lists, c.slot);
-        have = false;
-      }
+    // (Nothing running and nothing loaded means the share is used up: otherwise (2) loaded.)
+    if ((actm | pfm) == 0) break;
+    // A candidate past the step bound is abandoned.
+    const uint64_t stalled = ballot(c.steps >= step_lim) & actm;
+    if (__builtin_expect(stalled != 0, 0)) {
+      if (in_mask(stalled)) push_stall(lists, slot);
+      actm &= ~stalled;
     }
-    if (ballot(have) == 0 && ballot(pf_ok) == 0 && next >= end) break;
+    // (3) One check on every lane.
+    sink.live = actm;
+    wave_chk += (uint32_t)__popcll(actm);
+    bool esc_lane, done_lane;  // (unused here: the masks below are taken instead)
+    one_check<L, LOWCUT>(c, qb_hi, sink, lim, lists, p3s, esc_lane, done_lane, in_mask(actm));
+    const uint64_t escm = esc_mask<L, LOWCUT>(c.v) & actm;
+    const uint64_t donem = lt_base_mask<L>(c.v, c.n_lo, qb_hi) & actm & ~escm;
+    if (__builtin_expect(escm != 0, 0)) {
+      if (in_mask(escm)) push_escalate(lists, c.n_lo, qb_hi, slot);
+    }
+    if (donem) {
+      if (in_mask(donem)) finish<L>(c, acc, 0);  // (the step count includes k)
+    }
+    actm &= ~(escm | donem);
   }
-  if (aborted && (threadIdx.x & 63u) == 0) atomicAdd(lists.loop_aborts, 1ull);
-  DevRec rec = block_merge<4>(acc_to_rec<L>(acc), lrec);
-  if (threadIdx.x == 0) recs[blk] = rec;
+  if (aborted && lane == 0) atomicAdd(lists.loop_aborts, 1ull);
+  // The wave's record (counts merged across its lanes, the max from its scalar best), then
+  // the block's.
+  DevRec r = acc_to_rec<L>(acc);
+  wave_merge(r);
+  r.checks = wave_chk;
+  if (r.count) {
+    r.maxnum[0] = pair(sink.bw[0], sink.bw[1]);
+    if constexpr (L == 3) r.maxnum[1] = sink.bw[2];
+    else r.maxnum[1] = pair(sink.bw[2], sink.bw[3]);
+    if constexpr (L > 4) r.maxnum[2] = pair(sink.bw[4], sink.bw[5]);
+    r.max_start[0] = sink.bs_lo;
+    r.max_start[1] = sink.bs_hi;
+  }
+  if (lane == 0) lrec[wv] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < 4; ++i) rec_merge(r, lrec[i]);
+    recs[blk] = r;
+  }
 }
 
 // ---- escalated candidates on three limbs, one thread each -----------------------------
@@ -556,23 +636,24 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
   const uint32_t wv = threadIdx.x >> 6;
   const uint32_t lane = threadIdx.x & 63;
   WaveTables<L, LA>& T = tabs[wv];
-  const uint32_t ntiles = ctl->ntiles;
   // The waves of this workgroup (on XCD blockIdx.x mod kXcds) take tiles from that XCD's
   // share of the tile order, then from the other shares in turn: the order puts tiles that
   // read the same survivor records next to each other, so the waves of one XCD read them
   // from its L2.  Every share is left at most once, so the loop takes at most kXcds - 1
   // empty turns besides its tiles.
-  const uint32_t nshares = ctl->nshares > 1u ? ctl->nshares : 1u;
-  uint32_t share = blockIdx.x % nshares, turns = 0;
+  const uint32_t ntiles = ctl->ntiles;
+  const uint32_t lg = ctl->share_log2 ? 3u : 0u;  // (kXcds = 2^3 shares or one)
+  const uint32_t smask = (1u << lg) - 1u;
+  uint32_t share = blockIdx.x & smask, turns = 0;
   for (;;) {
-    const uint32_t s_lo = (uint32_t)((uint64_t)ntiles * share / nshares);
-    const uint32_t s_n = (uint32_t)((uint64_t)ntiles * (share + 1) / nshares) - s_lo;
+    const uint32_t s_lo = (uint32_t)(((uint64_t)ntiles * share) >> lg);
+    const uint32_t s_n = (uint32_t)(((uint64_t)ntiles * (share + 1)) >> lg) - s_lo;
     uint32_t pos = 0;
     if (lane == 0) pos = atomicAdd(&ctl->share_next[share], 1u);
     pos = __builtin_amdgcn_readfirstlane(pos);
     if (pos >= s_n) {
-      if (++turns == nshares) break;
-      share = (share + 1) % nshares;
+      if (turns++ == smask) break;
+      share = (share + 1) & smask;
       continue;
     }
     const uint32_t t = order[s_lo + pos];
