@@ -72,10 +72,10 @@ RATES = (50, 100, 200, 500)
 FULL_RATES = (200, 500)
 # HBM traffic of the batched launch, measured with rocprofv3 on this workload
 # (profiles/k_verify_persistent_batch4096_k32.md): FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0,
-# the factors a 1 GiB copy kernel gives.  Reads are the 16-byte survivor records (13.2 B per
+# the factors a 1 GiB copy kernel gives.  Reads are the 16-byte survivor records (13.19 B per
 # candidate measured: the records of the rows that share a survivor window, taken together
 # by the tile order, come partly from L2); writes are the per-tile records.
-READ_BYTES_PER_CANDIDATE = 13.23
+READ_BYTES_PER_CANDIDATE = 13.19
 WRITE_BYTES_PER_LAUNCH = 0.078e9
 
 
