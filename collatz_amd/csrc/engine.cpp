@@ -71,8 +71,10 @@ int plan_interval(const collatz_sieve* sv, u128 lo, u128 hi, uint32_t C, uint64_
 // 128 q; a batch of 4096 intervals has groups of thousands of rows, and interleaving all of
 // them at once put every wave of an XCD on the same records: 328 -> 339 ms per launch).
 constexpr uint32_t kTileBand = 128;
-// Tiles per resident wave from which a launch uses the per-XCD shares of the tile order.
-constexpr uint32_t kSharesMinTilesPerWave = 16;
+// Tiles per resident wave from which a launch uses the per-XCD shares of the tile order
+// (the 4096-interval batch has 65, [2^60, 2^60 + 2^44) at k = 37 has 38; a saturated served
+// launch of 2^30 candidates has 8, and ran 5.3e13 n/s with shares against 5.8e13 without).
+constexpr uint32_t kSharesMinTilesPerWave = 32;
 
 uint64_t tile_order(const LaunchPlan& lp, uint32_t* order, uint32_t band) {
   const size_t n = lp.rows.size();

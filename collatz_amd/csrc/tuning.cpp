@@ -31,20 +31,22 @@ static uint32_t bitlen(u128 x) {
 // SIMD).  Checks between refills: 2.
 static const uint32_t kDefaultWaves = 32, kDefaultU = 2;
 
-// Survivors per tile: the largest power of two in [4096, 131072] that still gives every
-// resident wave about 16 tiles.  Each tile ends with a tail in which lanes fall idle while
-// its last trajectories finish, so larger tiles waste less; but the tiles of a launch that
-// mixes ranges of very different sizes differ in size too (a row shorter than the tile is
-// a tile of its own), and with few tiles per wave the largest ones set the launch's end.
-// (Measured: one tile per wave ran [2^40, 2^40 + 2^36) in 2.04 ms against 2.16 ms, but
-// cut the saturated served stream from 3.7e13 to 3.2e13 n/s.)  A look-ahead keeps about a
-// third of the survivors, so tiles of such a sieve may be twice as large.  A launch too
-// small to give every resident wave a tile of 4096 takes smaller tiles (down to 256) until
-// it does: its duration is about one tile per wave, so this is its latency.
+// Survivors per tile: the largest power of two in [8192, 131072] that still gives every
+// resident wave about 8 tiles.  Each tile costs its wave a set-up (its row, the q 3^f table,
+// the first records in flight) and ends with a tail in which lanes fall idle while its last
+// trajectories finish, so larger tiles waste less; but the tiles of a launch that mixes
+// ranges of very different sizes differ in size too (a row shorter than the tile is a tile
+// of its own), and with few tiles per wave the largest ones set the launch's end.
+// (Measured against 16 tiles per wave and at least 4096: [2^40, 2^40 + 2^36) 1.50 -> 1.37
+// ms, the saturated served stream 5.3e13 -> 5.8e13 n/s with the per-XCD shares kept for
+// launches of 32 tiles per wave; 4 tiles per wave ran the stream at 5.7e13.)  A look-ahead
+// keeps about a third of the survivors, so tiles of such a sieve may be twice as large.  A
+// launch too small to give every resident wave a tile of 8192 takes smaller tiles (down to
+// 256) until it does: its duration is about one tile per wave, so this is its latency.
 uint32_t adaptive_tile(const collatz_sieve* sv, uint64_t candidates, uint32_t waves) {
   const uint64_t nwaves = (uint64_t)device_cu_count(sv->device) * waves;
   uint32_t t = sv->la_bits ? 262144 : 131072;
-  while (t > 4096 && candidates / t < 16 * nwaves) t >>= 1;
+  while (t > 8192 && candidates / t < 8 * nwaves) t >>= 1;
   while (t > 256 && candidates / t < nwaves) t >>= 1;
   return t;
 }

@@ -142,9 +142,15 @@ struct SinkWave {
       }
       for (uint64_t pend = pend1; pend; pend &= pend - 1) {
         const uint32_t l = (uint32_t)__builtin_ctzll(pend);
+        // The lane's top two words first: below the best's, the lane cannot hold it.  When
+        // every lane reaches (a tile's first check, the best still zero) only the running
+        // maxima in lane order, about ln 64 of them, go on to the whole compare.
         uint32_t cw[L];
+        cw[L - 1] = __builtin_amdgcn_readlane(x[L - 1], l);
+        cw[L - 2] = __builtin_amdgcn_readlane(x[L - 2], l);
+        if (pair(cw[L - 2], cw[L - 1]) < pair(bw[L - 2], bw[L - 1])) continue;
 #pragma unroll
-        for (int i = 0; i < L; ++i) cw[i] = __builtin_amdgcn_readlane(x[i], l);
+        for (int i = 0; i < L - 2; ++i) cw[i] = __builtin_amdgcn_readlane(x[i], l);
         const uint64_t c_lo = readlane64(n_lo, l), c_hi = readlane64(n_hi, l);
         int cmp = 0;
 #pragma unroll
