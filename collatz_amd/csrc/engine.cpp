@@ -138,7 +138,7 @@ uint64_t tile_order(const LaunchPlan& lp, uint32_t* order, uint32_t band) {
 bool Engine::GraphKey::operator==(const GraphKey& o) const {
   return std::memcmp(bufs, o.bufs, sizeof(bufs)) == 0 && limbs == o.limbs &&
          cutoff == o.cutoff && nblocks == o.nblocks && U == o.U && max_steps == o.max_steps &&
-         max_loop == o.max_loop;
+         max_loop == o.max_loop && lowkey == o.lowkey;
 }
 
 Engine::~Engine() {
@@ -187,8 +187,8 @@ hipError_t Engine::enqueue(const GraphKey& k, const Limits& lim, DeviceLists lis
   e = launch_make_tiles(rows_.p, ctl_.p, tiles_.p, s);
   if (e != hipSuccess) return e;
   e = launch_verify_persistent(k.limbs, sv_->arrays(), rows_.p, tiles_.p, order_.p, ctl_.p,
-                               k.nblocks,
-                               k.U, lim, sv_->pow3.p, tile_recs_.p, lists, s);
+                               k.nblocks, k.U, k.lowkey, lim, sv_->pow3.p, tile_recs_.p, lists,
+                               s);
   if (e != hipSuccess) return e;
   return launch_reduce_slots(tile_recs_.p, part_.p, ctl_.p, slot_recs_.p, s);
 }
@@ -260,6 +260,16 @@ int Engine::run(const LaunchPlan& lp, const Tuning& tn, int limbs, uint32_t cuto
   key.U = tn.U;
   key.max_steps = lim.max_steps;
   key.max_loop = lim.max_loop;
+  // Every start below 2^kLowKeyLog2 (a row's starts are below (q + 1) 2^k): the low-key
+  // test of the wave max.
+  key.lowkey = true;
+  for (const RowDesc& r : lp.rows) {
+    const uint64_t bound = 1ull << kLowKeyLog2;
+    if (r.qb_hi || r.qb_lo >= bound || bound - r.qb_lo < (1ull << sv_->k)) {
+      key.lowkey = false;
+      break;
+    }
+  }
 
   CZ_HIP(hipEventRecord(e0_, s));
   if (graph) {

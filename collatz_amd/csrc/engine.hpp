@@ -115,6 +115,7 @@ class Engine {
     const void* bufs[9];
     int limbs;
     uint32_t cutoff, nblocks, U, max_steps, max_loop;
+    bool lowkey;
     bool operator==(const GraphKey& o) const;
   } gkey_{};
   hipError_t enqueue(const GraphKey& k, const Limits& lim, DeviceLists lists, hipStream_t s);

@@ -161,8 +161,8 @@ struct PlainView {
   uint32_t* key_cls = nullptr;
   uint64_t* key_idx = nullptr;   // their indices in the sieve
   RowDesc* rows = nullptr;
-  uint64_t* part = nullptr;      // first block of each chunk (nchunks + 1)
-  DevRec* block = nullptr;       // one record per block
+  uint64_t* part = nullptr;      // first share of each chunk (nchunks + 1)
+  DevRec* block = nullptr;       // one record per share
   DevRec* chunk = nullptr;       // one record per chunk
   EscEntry* esc = nullptr;       // escalation list
   uint64_t esc_cap = 0;
@@ -181,11 +181,11 @@ class PlainBuffers {
   virtual ~PlainBuffers() = default;
   virtual uint64_t max_rows() const = 0;
   virtual uint64_t max_chunks() const = 0;
-  // Blocks the launch aims at (residues per lane follow from it).
-  virtual uint64_t target_blocks() const = 0;
+  // Share records a launch may use, at most (the plain kernel's shares follow from it).
+  virtual uint64_t max_shares() const = 0;
   virtual int keys(uint64_t n, PlainView& v) = 0;
   // Everything a launch of these sizes needs; ovf_cap is the overflow reports wanted.
-  virtual int launch(uint64_t nrows, uint64_t nblocks, uint64_t nchunks, uint64_t ovf_cap,
+  virtual int launch(uint64_t nrows, uint64_t nshares, uint64_t nchunks, uint64_t ovf_cap,
                      bool esc, PlainView& v) = 0;
   virtual int esc_recs(uint64_t n, PlainView& v) = 0;
 };

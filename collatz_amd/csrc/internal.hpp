@@ -39,7 +39,7 @@ struct RowDesc {
   uint64_t q_lo, q_hi;        // q
   uint64_t qb_lo, qb_hi;      // q << k (low k bits zero)
   uint64_t i_lo, i_hi;        // sieve index range
-  uint64_t block_start;       // first plain-kernel block of this row
+  uint64_t block_start;       // first plain-kernel share (or persistent tile) of the row
   uint32_t slot;              // output record slot (chunk of an interval)
   uint32_t cls;               // residue class r mod 9 of this row's survivors
 };
@@ -255,17 +255,22 @@ hipError_t launch_class_partition(const uint64_t* res, const uint64_t* num, cons
 
 // Range kernels: `limbs` is the word width, 1 = three 32-bit words (96 bits, escalating at
 // 2^91), 2 = two u64 limbs (escalating at 2^123), 3 = three u64 limbs.
+// The plain kernel runs one wave per share: row r owns shares [rows[r].block_start,
+// rows[r + 1].block_start) (equal pieces of its survivors), and share g's record goes to
+// share_recs[g].  `lowkey`: the range ends at or below 2^kLowKeyLog2 (the wave max's filter
+// then tests the values' low words; three- and four-word launches).
+constexpr unsigned kLowKeyLog2 = 35;
 hipError_t launch_verify_plain(int limbs, const SieveArrays& sv, const RowDesc* rows,
-                               uint32_t nrows, uint64_t nblocks, uint32_t R, uint32_t mod9,
-                               const Limits& lim, const uint64_t* pow3, DevRec* block_recs,
+                               uint32_t nrows, uint64_t nshares, uint32_t mod9, bool lowkey,
+                               const Limits& lim, const uint64_t* pow3, DevRec* share_recs,
                                DeviceLists lists, hipStream_t s);
 // `order` lists the ctl->ntiles tile indices in the order they are to be taken (tile t's
-// record goes to tile_recs[t] whatever its position).
+// record goes to tile_recs[t] whatever its position).  `lowkey` as for the plain kernel.
 hipError_t launch_verify_persistent(int limbs, const SieveArrays& sv, const RowDesc* rows,
                                     const TileDesc* tiles, const uint32_t* order, Ctl* ctl,
-                                    uint32_t nblocks, uint32_t U, const Limits& lim,
-                                    const uint64_t* pow3, DevRec* tile_recs, DeviceLists lists,
-                                    hipStream_t s);
+                                    uint32_t nblocks, uint32_t U, bool lowkey,
+                                    const Limits& lim, const uint64_t* pow3, DevRec* tile_recs,
+                                    DeviceLists lists, hipStream_t s);
 // Clears the tile counters and the list counters with a kernel (no memset nodes, so the
 // sequence can be graph-captured).
 // Waves per CU that stay resident for the persistent kernel of this width and cutoff.

@@ -2,7 +2,7 @@
 //
 // For callers that cannot allocate device memory per call (tight serving loops, harnesses
 // that forbid hipMalloc): the same computation as collatz_verify_range's plain form, with
-// every buffer the kernels need (row descriptors, bound lookups, block and chunk records,
+// every buffer the kernels need (row descriptors, bound lookups, share and chunk records,
 // escalation and overflow lists, counters) carved out of one file-scope device array of
 // fixed size.  Each device holds its own copy of the array (about 20 MiB) from the moment
 // the library is loaded until the process ends.  Host code never uses the array's name as
@@ -25,10 +25,9 @@ namespace {
 constexpr uint64_t kNaRows = 32768;               // row descriptors (q, chunk, residue class)
 constexpr uint64_t kNaKeys = 2 * kNaRows;         // residue bounds to look up
 constexpr uint64_t kNaChunks = 4096;              // chunk records
-constexpr uint64_t kNaTargetBlocks = 16384;       // blocks the launch aims at
-constexpr uint64_t kNaBlocks = 98304;             // blocks of one launch, at most
-// A launch aims at kNaTargetBlocks blocks and each row adds at most one more.
-static_assert(kNaBlocks >= kNaTargetBlocks + kNaRows, "block records too few");
+// Share records (one per wave of the plain kernel): a launch sizes its shares to fit them.
+constexpr uint64_t kNaShares = 98304;
+static_assert(kNaShares > kNaRows, "share records too few");
 constexpr uint64_t kNaEsc = 1ull << 16;           // escalated candidates (more: one width up)
 constexpr uint64_t kNaOvf = 4096;                 // overflow starts listed
 
@@ -41,7 +40,7 @@ constexpr uint64_t kOffKeyIdx = align256(kOffKeyCls + kNaKeys * 4);
 constexpr uint64_t kOffRows = align256(kOffKeyIdx + kNaKeys * 8);
 constexpr uint64_t kOffPart = align256(kOffRows + kNaRows * sizeof(RowDesc));
 constexpr uint64_t kOffBlock = align256(kOffPart + (kNaChunks + 1) * 8);
-constexpr uint64_t kOffChunk = align256(kOffBlock + kNaBlocks * sizeof(DevRec));
+constexpr uint64_t kOffChunk = align256(kOffBlock + kNaShares * sizeof(DevRec));
 constexpr uint64_t kOffEsc = align256(kOffChunk + kNaChunks * sizeof(DevRec));
 constexpr uint64_t kOffEscRecs = align256(kOffEsc + kNaEsc * sizeof(EscEntry));
 constexpr uint64_t kOffOvf = align256(kOffEscRecs + kNaEsc * sizeof(DevRec));
@@ -89,7 +88,7 @@ class StaticBuffers final : public PlainBuffers {
   explicit StaticBuffers(const DeviceScratch& d) : d_(d) {}
   uint64_t max_rows() const override { return kNaRows; }
   uint64_t max_chunks() const override { return kNaChunks; }
-  uint64_t target_blocks() const override { return kNaTargetBlocks; }
+  uint64_t max_shares() const override { return kNaShares; }
   int keys(uint64_t n, PlainView& v) override {
     if (n > kNaKeys) return fail(COLLATZ_EINVAL, "noalloc: too many row bounds");
     v.keys = at<uint64_t>(kOffKeys);
@@ -97,13 +96,13 @@ class StaticBuffers final : public PlainBuffers {
     v.key_idx = at<uint64_t>(kOffKeyIdx);
     return COLLATZ_OK;
   }
-  int launch(uint64_t nrows, uint64_t nblocks, uint64_t nchunks, uint64_t, bool esc,
+  int launch(uint64_t nrows, uint64_t nshares, uint64_t nchunks, uint64_t, bool esc,
              PlainView& v) override {
     if (nrows > kNaRows || nchunks > kNaChunks)
       return fail(COLLATZ_EINVAL, "noalloc: range has more rows or chunks than the limit");
-    if (nblocks > kNaBlocks)
-      return fail(COLLATZ_EINVAL, "noalloc: range needs more blocks than the limit (" +
-                                      std::to_string(kNaBlocks) + ")");
+    if (nshares > kNaShares)
+      return fail(COLLATZ_EINVAL, "noalloc: range needs more shares than the limit (" +
+                                      std::to_string(kNaShares) + ")");
     v.rows = at<RowDesc>(kOffRows);
     v.part = at<uint64_t>(kOffPart);
     v.block = at<DevRec>(kOffBlock);

@@ -86,27 +86,33 @@ int verify_plain_with(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs
   if (plan.nchunks > B.max_chunks()) return fail(COLLATZ_EINVAL, "range has too many chunks");
   rc = resolve_rows(sv, plan, B, v, s);
   if (rc) return rc;
-  // Blocks of 256 lanes; each lane takes R residues of its row in turn.
+  // One wave per share: each row is cut into equal pieces of at most S survivors.  A share
+  // costs its wave a set-up (the row, its q 3^f table, the first records in flight), a first
+  // check at which every lane offers its value for the max, and an end at which lanes idle
+  // while the share's last trajectories finish, together about the work of a few hundred
+  // survivors; so S is at least 16384 (most rows of a 2^24-wide chunk at k = 32 are one
+  // share), but small enough that every resident wave has about four shares to take, and
+  // large enough that the shares fit the record buffer.  ([2^40, 2^40 + 2^36), k = 32: each
+  // row split over the four waves of a block ran 1.55 ms.)
   uint64_t slots = 0;
   for (const RowDesc& r : plan.rows) slots += r.i_hi - r.i_lo;
-  const uint64_t target_blocks = B.target_blocks();
-  // Survivors per wave: 64 R.  A wave's share ends with a tail in which its lanes idle
-  // while the share's last trajectories finish, so shares are large (R up to 4096).
-  const uint32_t R = (uint32_t)std::min<uint64_t>(
-      4096, std::max<uint64_t>(1, (slots + 256 * target_blocks - 1) / (256 * target_blocks)));
-  const uint64_t span = 256ull * R;
-  uint64_t nblocks = 0;
+  const uint64_t resident = (uint64_t)device_cu_count(sv->device) * 32;
+  uint64_t S = std::max<uint64_t>(slots / (4 * resident),
+                                  std::min<uint64_t>(16384, std::max<uint64_t>(1, slots / resident)));
+  const uint64_t cap = B.max_shares() > plan.rows.size() ? B.max_shares() - plan.rows.size() : 1;
+  S = std::max<uint64_t>({S, slots ? (slots - 1) / cap + 1 : 1, 1});
+  uint64_t nshares = 0;
   for (RowDesc& r : plan.rows) {
-    r.block_start = nblocks;
-    nblocks += (r.i_hi - r.i_lo + span - 1) / span;
+    r.block_start = nshares;
+    nshares += (r.i_hi - r.i_lo + S - 1) / S;
   }
   std::vector<uint64_t> part_start(plan.nchunks + 1);
   for (uint64_t c = 0; c <= plan.nchunks; ++c) {
     const uint64_t row = plan.chunk_first_row[c];
-    part_start[c] = row < plan.rows.size() ? plan.rows[row].block_start : nblocks;
+    part_start[c] = row < plan.rows.size() ? plan.rows[row].block_start : nshares;
   }
   const uint64_t ovf_cap = std::max<uint64_t>(out->overflow_cap, 1024);
-  rc = B.launch(plan.rows.size(), nblocks, plan.nchunks, ovf_cap, a.width < 3, v);
+  rc = B.launch(plan.rows.size(), nshares, plan.nchunks, ovf_cap, a.width < 3, v);
   if (rc) return rc;
   CZ_HIP(hipMemcpyAsync(v.rows, plan.rows.data(), plan.rows.size() * sizeof(RowDesc),
                         hipMemcpyHostToDevice, s));
@@ -127,8 +133,9 @@ int verify_plain_with(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs
   const Limits lim = make_limits((int)a.width, a.cutoff);
   // Device time: the verification and reduction kernels, then the three-limb pass.
   CZ_HIP(hipEventRecord(v.ev[0], s));
+  const bool lowkey = hi <= ((u128)1 << kLowKeyLog2);
   CZ_HIP(launch_verify_plain((int)a.width, sv->arrays(), v.rows, (uint32_t)plan.rows.size(),
-                             nblocks, R, sv->mod9, lim, sv->pow3.p, v.block, lists, s));
+                             nshares, sv->mod9, lowkey, lim, sv->pow3.p, v.block, lists, s));
   CZ_HIP(launch_reduce_chunks(v.block, v.part, (uint32_t)plan.nchunks, v.chunk, s));
   CZ_HIP(hipEventRecord(v.ev[1], s));
   std::vector<DevRec> recs(plan.nchunks);
@@ -212,7 +219,7 @@ class ScratchBuffers final : public PlainBuffers {
   explicit ScratchBuffers(PlainScratch& s) : scr_(s) {}
   uint64_t max_rows() const override { return 1ull << 26; }
   uint64_t max_chunks() const override { return 0x7FFFFFFFull; }
-  uint64_t target_blocks() const override { return 1ull << 14; }
+  uint64_t max_shares() const override { return ~0ull; }
   int keys(uint64_t n, PlainView& v) override {
     CZ_HIP(scr_.keys.reserve(n));
     CZ_HIP(scr_.key_cls.reserve(n));
@@ -222,12 +229,12 @@ class ScratchBuffers final : public PlainBuffers {
     v.key_idx = scr_.key_idx.p;
     return COLLATZ_OK;
   }
-  int launch(uint64_t nrows, uint64_t nblocks, uint64_t nchunks, uint64_t ovf_cap, bool esc,
+  int launch(uint64_t nrows, uint64_t nshares, uint64_t nchunks, uint64_t ovf_cap, bool esc,
              PlainView& v) override {
     const uint64_t esc_cap = 1ull << 20;
     CZ_HIP(scr_.rows.reserve(std::max<uint64_t>(1, nrows)));
     CZ_HIP(scr_.part.reserve(nchunks + 1));
-    CZ_HIP(scr_.block.reserve(std::max<uint64_t>(1, nblocks)));
+    CZ_HIP(scr_.block.reserve(std::max<uint64_t>(1, nshares)));
     CZ_HIP(scr_.chunk.reserve(std::max<uint64_t>(1, nchunks)));
     CZ_HIP(scr_.esc.reserve(esc ? esc_cap : 1));
     CZ_HIP(scr_.ovf.reserve(4 * ovf_cap));

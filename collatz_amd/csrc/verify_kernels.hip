@@ -38,6 +38,9 @@ __device__ __forceinline__ bool la_keep(const SieveArrays& sv, uint32_t f, uint3
   return fe >= ((sv.la[idx >> 3] >> ((idx & 7u) << 2)) & 15u);
 }
 
+// Waves of a 256-lane workgroup.
+constexpr int kWavesPerBlock = 4;
+
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
@@ -87,17 +90,31 @@ __device__ __forceinline__ void words_apart(uint32_t* x) {
 // start from zero in every tile and take a divergent update path many times per tile; the
 // wave's max needs the merge about log(offers) times.
 // KEYMAX: merge many reaching lanes through a cross-lane max of their top-64-bit key first
-// (the plain kernel, whose waves take short shares; in the persistent kernels its registers
-// would cost occupancy).
-template <int L, bool KEYMAX = false>
+// (the plain kernel; in the persistent kernel this code, though it rarely runs, slowed the
+// check loop of long launches by 2-3%).
+// LOWKEY (three or four words): for launches whose values stay small.  The key of the test
+// above is the value's top two words, the value over 2^32 (over 2^64 on four words); while
+// the best stays below about 2^36, most values share its key, nearly every offer reaches,
+// and the wave merges its lanes one at a time twice per check ([1, 2^32) at k = 20 ran 1.3
+// ms where [2^50, 2^50 + 2^32) ran 0.27).  LOWKEY tests the low two words and whether the
+// upper words are nonzero: reach = upper != 0 or low pair >= thr, with thr the best's low
+// pair while its upper words are zero, else all ones.  That is exact too, but costs time
+// where ties are rare (one more VALU instruction per offer; the persistent kernel ran
+// [1, 2^36) 15% slower with it, the plain one 4%), so launches take it only for ranges that
+// end at or below 2^35 (kLowKeyLog2): [1, 2^32) then runs 0.28 ms (plain) and 0.44 ms
+// (persistent) instead of 1.2 and 1.4, [1, 2^34) 1.03 and 1.34 instead of 1.31 and 1.44.
+template <int L, bool KEYMAX = false, bool LOWKEY = false>
 struct SinkWave {
+  static_assert(!LOWKEY || L == 3 || L == 4, "the low key is for three or four words");
   uint32_t bw[L];         // the wave's best value (0: none yet; every value is >= 2)
   uint64_t bs_lo, bs_hi;  // its start
   uint64_t live;          // lanes with a candidate
+  uint64_t thr;           // (LOWKEY) see above
   __device__ __forceinline__ void clear() {
 #pragma unroll
     for (int i = 0; i < L; ++i) bw[i] = 0;
     bs_lo = bs_hi = 0;
+    thr = 0;
   }
   __device__ __forceinline__ void offer(Num<L>& v, uint64_t n_lo, uint64_t n_hi, bool) {
     // Six words: the top pair is zero below 2^128, so the test there is the whole compare.
@@ -106,6 +123,11 @@ struct SinkWave {
     bool reach;
     if constexpr (L == 6) {
       reach = ge_words<L>(v.w, bw);
+    } else if constexpr (LOWKEY) {
+      uint32_t up = v.w[2];
+#pragma unroll
+      for (int i = 3; i < L; ++i) up |= v.w[i];
+      reach = (up != 0u) | (pair(v.w[0], v.w[1]) >= thr);
     } else {
       constexpr int k = L - 2;
       uint32_t lo = v.w[k], hi = v.w[k + 1];
@@ -123,12 +145,19 @@ struct SinkWave {
       uint64_t pend1 = pend0;
       if constexpr (KEYMAX && L != 6) {
         // Many reaching lanes (at the start of a tile, when the best is still low): only the
-        // lanes holding the largest top-64-bit key can hold the largest value, so a
-        // cross-lane max of the key (six shuffle rounds) cuts the one-at-a-time merge below
-        // to them, usually one lane.
+        // lanes holding the largest key can hold the largest value, so a cross-lane max of
+        // the key (six shuffle rounds) cuts the one-at-a-time merge below to them, usually
+        // one lane.  (The low key: a lane with nonzero upper words takes the largest key.)
         if (__popcll(pend0) > 2) {
-          constexpr int k = L - 2;
-          const uint64_t own = pair(x[k], x[k + 1]);
+          uint64_t own;
+          if constexpr (LOWKEY) {
+            uint32_t up = x[2];
+#pragma unroll
+            for (int i = 3; i < L; ++i) up |= x[i];
+            own = up ? ~0ull : pair(x[0], x[1]);
+          } else {
+            own = pair(x[L - 2], x[L - 1]);
+          }
           uint64_t key = __builtin_amdgcn_inverse_ballot_w64(pend0) ? own : 0;
 #pragma unroll
           for (int m = 32; m >= 1; m >>= 1) {
@@ -142,15 +171,20 @@ struct SinkWave {
       }
       for (uint64_t pend = pend1; pend; pend &= pend - 1) {
         const uint32_t l = (uint32_t)__builtin_ctzll(pend);
-        // The lane's top two words first: below the best's, the lane cannot hold it.  When
-        // every lane reaches (a tile's first check, the best still zero) only the running
-        // maxima in lane order, about ln 64 of them, go on to the whole compare.
         uint32_t cw[L];
-        cw[L - 1] = __builtin_amdgcn_readlane(x[L - 1], l);
-        cw[L - 2] = __builtin_amdgcn_readlane(x[L - 2], l);
-        if (pair(cw[L - 2], cw[L - 1]) < pair(bw[L - 2], bw[L - 1])) continue;
+        if constexpr (LOWKEY) {
 #pragma unroll
-        for (int i = 0; i < L - 2; ++i) cw[i] = __builtin_amdgcn_readlane(x[i], l);
+          for (int i = 0; i < L; ++i) cw[i] = __builtin_amdgcn_readlane(x[i], l);
+        } else {
+          // The lane's top two words first: below the best's, the lane cannot hold it.  When
+          // every lane reaches (a tile's first check, the best still zero) only the running
+          // maxima in lane order, about ln 64 of them, go on to the whole compare.
+          cw[L - 1] = __builtin_amdgcn_readlane(x[L - 1], l);
+          cw[L - 2] = __builtin_amdgcn_readlane(x[L - 2], l);
+          if (pair(cw[L - 2], cw[L - 1]) < pair(bw[L - 2], bw[L - 1])) continue;
+#pragma unroll
+          for (int i = 0; i < L - 2; ++i) cw[i] = __builtin_amdgcn_readlane(x[i], l);
+        }
         const uint64_t c_lo = readlane64(n_lo, l), c_hi = readlane64(n_hi, l);
         int cmp = 0;
 #pragma unroll
@@ -162,6 +196,12 @@ struct SinkWave {
           bs_lo = c_lo;
           bs_hi = c_hi;
         }
+      }
+      if constexpr (LOWKEY) {
+        uint32_t up = bw[2];
+#pragma unroll
+        for (int i = 3; i < L; ++i) up |= bw[i];
+        thr = up ? ~0ull : pair(bw[0], bw[1]);
       }
     }
   }
@@ -376,41 +416,56 @@ __device__ __forceinline__ bool run_candidate(Cand<L>& c, Acc<L>& acc, const Lim
 }
 
 // ---- plain kernel: a fixed share of one row per wave, lanes refilled --------------------
-// Block b covers up to 256 R consecutive survivors of one row (the row whose block range
-// holds b); each of its four waves takes a quarter of them.  No state outlives the launch
-// and no work moves between waves: this is the drop-in kernel of collatz_verify_range.  A
-// lane runs one candidate at a time, one check per loop iteration; a lane whose candidate
-// has ended (descended, escalated, or abandoned at the step bound) starts the survivor record
-// it loaded one iteration earlier, and the lanes without a loaded record take the next
-// survivors of the wave's share by rank among them, so no lane waits for its wave's slowest
-// trajectory until the share is used up.  As in the persistent kernel below, which lanes run
-// a candidate and which hold a record are two uniform masks (scalar registers), every lane
-// runs the check (a lane without a candidate on stale values, its effects dropped), the
-// wave's max excursion is kept in scalar registers, the step count starts at k, and on three
-// words a row at or above the start bound holds 2^95 in its q 3^f table.  The loop ends when
-// no lane holds a candidate or a record and the share is used up, or at lim.max_loop
-// iterations (counted in lists.loop_aborts: the host then fails the call).
-template <int L, bool LOWCUT>
+// Wave g (= 4 blockIdx.x + its index in the block) takes share g: piece g - s of the row
+// whose shares start at s (its block_start; a row is cut into equal pieces, most rows into
+// one).  No state outlives the launch and no work moves between waves: this is the drop-in
+// kernel of collatz_verify_range.  A lane runs one candidate at a time, two checks per loop
+// iteration; a lane whose candidate has ended (descended, escalated, or abandoned at the
+// step bound) starts the survivor record it loaded earlier, and the lanes
+// without a loaded record take the next survivors of the wave's share by rank among them,
+// so no lane waits for its wave's slowest trajectory until the share is used up.  As in the
+// persistent kernel below, which lanes run a candidate and which hold a record are two
+// uniform masks (scalar registers), every lane runs the check (a lane without a candidate
+// on stale values, its effects dropped), the wave's max excursion is kept in scalar
+// registers, the step count starts at k, and on three words a row at or above the start
+// bound holds 2^95 in its q 3^f table.  The loop ends when no lane holds a candidate or a
+// record and the share is used up, or at lim.max_loop iterations (counted in
+// lists.loop_aborts: the host then fails the call).  Each wave writes its share's record.
+template <int L, bool LOWCUT, bool LOWKEY>
 __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowDesc* rows,
-                                                      uint32_t nrows, uint32_t R, uint32_t mod9,
-                                                      Limits lim, const uint64_t* pow3,
-                                                      DevRec* recs, DeviceLists lists) {
-  __shared__ uint32_t q3[41][L];
-  __shared__ uint32_t q3lo[41];  // low word of q 3^f (the look-ahead index)
-  __shared__ DevRec lrec[4];
+                                                      uint32_t nrows, uint64_t nshares,
+                                                      uint32_t mod9, Limits lim,
+                                                      const uint64_t* pow3, DevRec* recs,
+                                                      DeviceLists lists) {
+  __shared__ uint32_t q3s[kWavesPerBlock][41][L];
+  __shared__ uint32_t q3los[kWavesPerBlock][41];  // low word of q 3^f (the look-ahead index)
   __shared__ Pow3Lds p3s;
   stage_pow3(p3s);
-  const uint64_t blk = blockIdx.x;
+  __syncthreads();
+  // (The wave index through readfirstlane: the compiler does not know threadIdx.x >> 6 is
+  // uniform in a wave, and would keep the share bounds, the masks and the cursor in vector
+  // registers.)
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t gw = (uint64_t)blockIdx.x * kWavesPerBlock + wv;
+  if (gw >= nshares) return;  // (the last block's spare waves; no barrier follows)
   uint32_t lo = 0, hi = nrows;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (rows[mid].block_start <= blk) lo = mid;
+    if (rows[mid].block_start <= gw) lo = mid;
     else hi = mid;
   }
   const RowDesc row = rows[lo];
-  if (threadIdx.x < 41) {
+  const uint64_t pieces = (lo + 1 < nrows ? rows[lo + 1].block_start : nshares) - row.block_start;
+  const uint64_t piece = gw - row.block_start;
+  const uint64_t len = row.i_hi - row.i_lo;  // (< 2^32: one residue class of the sieve)
+  uint64_t next = row.i_lo + len * piece / pieces;
+  const uint64_t end = row.i_lo + len * (piece + 1) / pieces;
+  uint32_t (*q3)[L] = q3s[wv];
+  uint32_t* q3lo = q3los[wv];
+  const uint32_t lane = threadIdx.x & 63u;
+  if (lane < 41) {
     uint32_t w6[6];
-    q_times_pow3(row.q_lo, row.q_hi, pow3[threadIdx.x], w6);
+    q_times_pow3(row.q_lo, row.q_hi, pow3[lane], w6);
     uint32_t big = 0;
 #pragma unroll
     for (int i = L; i < 6; ++i) big |= w6[i];
@@ -418,35 +473,24 @@ __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowD
       // (As in the persistent kernel: a start M + q 3^f then cannot carry out of three words
       // and fails the start test exactly when q 3^f reaches 2^86.)
       big |= (uint32_t)(w6[2] >= (1u << (kNarrowCheckLog2 - 64)));
-      q3[threadIdx.x][0] = big ? 0u : w6[0];
-      q3[threadIdx.x][1] = big ? 0u : w6[1];
-      q3[threadIdx.x][2] = big ? 0x80000000u : w6[2];
+      q3[lane][0] = big ? 0u : w6[0];
+      q3[lane][1] = big ? 0u : w6[1];
+      q3[lane][2] = big ? 0x80000000u : w6[2];
     } else {
 #pragma unroll
-      for (int i = 0; i < L; ++i) q3[threadIdx.x][i] = big ? 0xFFFFFFFFu : w6[i];
+      for (int i = 0; i < L; ++i) q3[lane][i] = big ? 0xFFFFFFFFu : w6[i];
     }
-    q3lo[threadIdx.x] = w6[0];
+    q3lo[lane] = w6[0];
   }
-  __syncthreads();
-  const uint32_t lane = threadIdx.x & 63u;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
   Acc<L> acc;
   acc.init();
-  SinkWave<L, true> sink;
+  SinkWave<L, true, LOWKEY> sink;
   sink.clear();
   uint32_t wave_chk = 0;
-  // The block's survivors [bfirst, bfirst + blen), split evenly over its four waves: the
-  // wave's share is [next, end) (uniform), empty where a block holds fewer survivors than
-  // waves.
-  // (The wave index through readfirstlane: the compiler does not know threadIdx.x >> 6 is
-  // uniform in a wave, and would keep the share bounds, the masks and the cursor in vector
-  // registers.)
-  const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint64_t bfirst = row.i_lo + (blk - row.block_start) * (256ull * R);
-  const uint64_t blen = row.i_hi - bfirst < 256ull * R ? row.i_hi - bfirst : 256ull * R;
-  const uint64_t share = (blen + 3) >> 2;
-  uint64_t next = bfirst + wv * share;
-  const uint64_t end = next + share < bfirst + blen ? next + share : bfirst + blen;
   const uint64_t qb_lo = row.qb_lo, qb_hi = row.qb_hi;
   const uint32_t slot = row.slot;
   Cand<L> c;
@@ -457,41 +501,43 @@ __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowD
   const uint32_t step_lim = lim.max_steps + sv.k;  // (the step count starts at k)
   uint32_t iters = 0;
   bool aborted = false;
+  // Starts the loaded records of the lanes in S (no running candidate, a loaded record); a
+  // look-ahead sieve first tests each, and a record it drops is simply not started.
+  auto start_loaded = [&](uint64_t S) {
+    bool keep = true;
+    if (in_mask(S)) {
+      const uint32_t f = rec_f(pf);
+      const uint64_t M = rec_M(pf);
+      if (sv.la) keep = la_keep(sv, f, (uint32_t)M + q3lo[f]);
+      const bool fits = cand_start<L>(c, M, q3[f], false, qb_lo, qb_hi, rec_r(pf), sv.k);
+      if constexpr (L != 3) {
+        if (!fits) c.v.w[L - 1] = 0xFFFFFFFFu;
+      }
+    }
+    const uint64_t kept = ballot(keep) & S;
+    const uint64_t ok = start_ok_mask<L, LOWCUT>(c.v) & kept;
+    const uint64_t bad = kept & ~ok;
+    if (__builtin_expect(bad != 0, 0)) {
+      if (in_mask(bad)) push_escalate(lists, c.n_lo, qb_hi, slot);
+    }
+    actm |= ok;
+    pfm &= ~S;
+  };
   for (;;) {
     if (__builtin_expect(++iters > lim.max_loop, 0)) {
       aborted = true;
       break;
     }
-    // (1) Lanes without a candidate start their loaded record (a look-ahead sieve first
-    //     tests it; a record it drops is simply not started).
+    // (1) Lanes without a candidate start their loaded record.
     {
       const uint64_t S = ~actm & pfm;
-      if (S) {
-        bool keep = true;
-        if (in_mask(S)) {
-          const uint32_t f = rec_f(pf);
-          const uint64_t M = rec_M(pf);
-          if (sv.la) keep = la_keep(sv, f, (uint32_t)M + q3lo[f]);
-          const bool fits = cand_start<L>(c, M, q3[f], false, qb_lo, qb_hi, rec_r(pf), sv.k);
-          if constexpr (L != 3) {
-            if (!fits) c.v.w[L - 1] = 0xFFFFFFFFu;
-          }
-        }
-        const uint64_t kept = ballot(keep) & S;
-        const uint64_t ok = start_ok_mask<L, LOWCUT>(c.v) & kept;
-        const uint64_t bad = kept & ~ok;
-        if (__builtin_expect(bad != 0, 0)) {
-          if (in_mask(bad)) push_escalate(lists, c.n_lo, qb_hi, slot);
-        }
-        actm |= ok;
-        pfm &= ~S;
-      }
+      if (S) start_loaded(S);
     }
     // (2) Lanes without a loaded record take the next survivors of the share.
     const uint64_t need = ~pfm;
     if (need && next < end) {
       const uint32_t want = (uint32_t)__popcll(need);
-      const uint32_t avail = (uint32_t)(end - next);  // (a share is at most 2^18 survivors)
+      const uint32_t avail = (uint32_t)(end - next);  // (a share is part of one row: < 2^32)
       const uint32_t take = avail < want ? avail : want;
       const uint32_t rank = lane_rank(need);
       const uint64_t tk = ballot(rank < take) & need;
@@ -501,30 +547,40 @@ __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowD
     }
     // (Nothing running and nothing loaded means the share is used up: otherwise (2) loaded.)
     if ((actm | pfm) == 0) break;
-    // A candidate past the step bound is abandoned.
+    // A candidate past the step bound is abandoned (tested once per refill: it may run one
+    // check beyond the bound first).
     const uint64_t stalled = ballot(c.steps >= step_lim) & actm;
     if (__builtin_expect(stalled != 0, 0)) {
       if (in_mask(stalled)) push_stall(lists, slot);
       actm &= ~stalled;
     }
-    // (3) One check on every lane.
-    sink.live = actm;
-    wave_chk += (uint32_t)__popcll(actm);
-    bool esc_lane, done_lane;  // (unused here: the masks below are taken instead)
-    one_check<L, LOWCUT>(c, qb_hi, sink, lim, lists, p3s, esc_lane, done_lane, in_mask(actm));
-    const uint64_t escm = esc_mask<L, LOWCUT>(c.v) & actm;
-    const uint64_t donem = lt_base_mask<L>(c.v, c.n_lo, qb_hi) & actm & ~escm;
-    if (__builtin_expect(escm != 0, 0)) {
-      if (in_mask(escm)) push_escalate(lists, c.n_lo, qb_hi, slot);
+    // (3) Two checks on every lane (a record loaded in (2) is then used two checks later, and
+    //     the refill's cost is shared); a lane whose candidate ends at the first starts its
+    //     loaded record before the second.
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      sink.live = actm;
+      wave_chk += (uint32_t)__popcll(actm);
+      bool esc_lane, done_lane;  // (unused here: the masks below are taken instead)
+      one_check<L, LOWCUT>(c, qb_hi, sink, lim, lists, p3s, esc_lane, done_lane,
+                           in_mask(actm));
+      const uint64_t escm = esc_mask<L, LOWCUT>(c.v) & actm;
+      const uint64_t donem = lt_base_mask<L>(c.v, c.n_lo, qb_hi) & actm & ~escm;
+      if (__builtin_expect(escm != 0, 0)) {
+        if (in_mask(escm)) push_escalate(lists, c.n_lo, qb_hi, slot);
+      }
+      if (donem) {
+        if (in_mask(donem)) finish<L>(c, acc, 0);  // (the step count includes k)
+      }
+      actm &= ~(escm | donem);
+      if (u == 0) {
+        const uint64_t S = ~actm & pfm;
+        if (S) start_loaded(S);
+      }
     }
-    if (donem) {
-      if (in_mask(donem)) finish<L>(c, acc, 0);  // (the step count includes k)
-    }
-    actm &= ~(escm | donem);
   }
   if (aborted && lane == 0) atomicAdd(lists.loop_aborts, 1ull);
-  // The wave's record (counts merged across its lanes, the max from its scalar best), then
-  // the block's.
+  // The share's record: counts merged across the lanes, the max from the scalar best.
   DevRec r = acc_to_rec<L>(acc);
   wave_merge(r);
   r.checks = wave_chk;
@@ -536,12 +592,7 @@ __global__ __launch_bounds__(256) void k_verify_plain(SieveArrays sv, const RowD
     r.max_start[0] = sink.bs_lo;
     r.max_start[1] = sink.bs_hi;
   }
-  if (lane == 0) lrec[wv] = r;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 1; i < 4; ++i) rec_merge(r, lrec[i]);
-    recs[blk] = r;
-  }
+  if (lane == 0) recs[gw] = r;
 }
 
 // ---- escalated candidates on three limbs, one thread each -----------------------------
@@ -596,7 +647,6 @@ __global__ __launch_bounds__(256) void k_escalated(SieveArrays sv, const EscEntr
 // for no lane; only the tile's last candidates run with some lanes idle.  Candidate data
 // is prefetched one check ahead so the loads overlap the arithmetic.  Row data (q << k,
 // the q * 3^f table) is the same for the whole tile and stays in SGPRs and LDS.
-constexpr int kWavesPerBlock = 4;
 
 // q * 3^f for f < 41 on L words.  Where the product does not fit L words the row is all
 // ones: adding any start value (>= 1) then carries out, which marks the candidate for the
@@ -629,7 +679,7 @@ struct LaStage {
 // UC: the checks per refill as a compile-time constant (1 to 4), or 0 to take them from
 // U.  With a constant the check loop unrolls, and with one check per refill the record set-up
 // for starts between checks (hoisted ahead of the loop) goes away.
-template <int L, bool LOWCUT, bool LA, bool BOUNDED = LA, int UC = 0>
+template <int L, bool LOWCUT, bool LA, bool BOUNDED = LA, int UC = 0, bool LOWKEY = false>
 __global__ __launch_bounds__(256) void k_verify_persistent(
     SieveArrays sv, const RowDesc* rows, const TileDesc* tiles, const uint32_t* order,
     Ctl* ctl, Limits lim, const uint64_t* pow3, DevRec* tile_recs, DeviceLists lists,
@@ -701,7 +751,7 @@ __global__ __launch_bounds__(256) void k_verify_persistent(
     // Per-lane counts and sums; the max is the wave's (SinkWave), cleared for each tile.
     Acc<L> acc;
     acc.init();
-    SinkWave<L> sink;
+    SinkWave<L, false, LOWKEY> sink;
     sink.clear();
     // Checks of the tile, counted per wave from the mask of lanes that run one (scalar
     // work; the lane accumulators' chk stays 0).
@@ -896,27 +946,31 @@ __global__ __launch_bounds__(256) void k_reduce_chunks(const DevRec* parts,
 }
 
 hipError_t launch_verify_plain(int limbs, const SieveArrays& sv, const RowDesc* rows,
-                               uint32_t nrows, uint64_t nblocks, uint32_t R, uint32_t mod9,
-                               const Limits& lim, const uint64_t* pow3, DevRec* block_recs,
+                               uint32_t nrows, uint64_t nshares, uint32_t mod9, bool lowkey,
+                               const Limits& lim, const uint64_t* pow3, DevRec* share_recs,
                                DeviceLists lists, hipStream_t s) {
-  if (nblocks == 0) return hipSuccess;
+  if (nshares == 0) return hipSuccess;
+  const uint64_t nblocks = (nshares + kWavesPerBlock - 1) / kWavesPerBlock;
   if (nblocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
   const dim3 grid((uint32_t)nblocks), block(256);
-  if (limbs == 1)
-    hipLaunchKernelGGL((k_verify_plain<3, false>), grid, block, 0, s, sv, rows, nrows, R, mod9,
-                       lim, pow3, block_recs, lists);
-  else if (limbs == 2 && lim.cutoff >= kWideLog2)
-    hipLaunchKernelGGL((k_verify_plain<4, false>), grid, block, 0, s, sv, rows, nrows, R, mod9,
-                       lim, pow3, block_recs, lists);
-  else if (limbs == 2)
-    hipLaunchKernelGGL((k_verify_plain<4, true>), grid, block, 0, s, sv, rows, nrows, R, mod9,
-                       lim, pow3, block_recs, lists);
-  else if (lim.cutoff >= 128)
-    hipLaunchKernelGGL((k_verify_plain<6, false>), grid, block, 0, s, sv, rows, nrows, R, mod9,
-                       lim, pow3, block_recs, lists);
-  else
-    hipLaunchKernelGGL((k_verify_plain<6, true>), grid, block, 0, s, sv, rows, nrows, R, mod9,
-                       lim, pow3, block_recs, lists);
+#define CZ_PLAIN(L, LC, LK)                                                                   \
+  hipLaunchKernelGGL((k_verify_plain<L, LC, LK>), grid, block, 0, s, sv, rows, nrows, nshares, \
+                     mod9, lim, pow3, share_recs, lists)
+  if (limbs == 1) {
+    if (lowkey) CZ_PLAIN(3, false, true);
+    else CZ_PLAIN(3, false, false);
+  } else if (limbs == 2 && lim.cutoff >= kWideLog2) {
+    if (lowkey) CZ_PLAIN(4, false, true);
+    else CZ_PLAIN(4, false, false);
+  } else if (limbs == 2) {
+    if (lowkey) CZ_PLAIN(4, true, true);
+    else CZ_PLAIN(4, true, false);
+  } else if (lim.cutoff >= 128) {
+    CZ_PLAIN(6, false, false);
+  } else {
+    CZ_PLAIN(6, true, false);
+  }
+#undef CZ_PLAIN
   return hipGetLastError();
 }
 
@@ -955,10 +1009,26 @@ uint32_t persistent_waves_fit(int limbs, uint32_t cutoff, bool la) {
 template <bool LA, bool BOUNDED>
 static void launch_persistent_lb(int limbs, const SieveArrays& sv, const RowDesc* rows,
                                  const TileDesc* tiles, const uint32_t* order, Ctl* ctl,
-                                 uint32_t nblocks, uint32_t U, const Limits& lim,
+                                 uint32_t nblocks, uint32_t U, bool lowkey, const Limits& lim,
                                  const uint64_t* pow3, DevRec* tile_recs, DeviceLists lists,
                                  hipStream_t s) {
   const dim3 grid(nblocks), block(256);
+  // Launches whose ranges end at or below 2^35 on 96 bits: the low-key test of the wave max
+  // (the forms without the look-ahead and the loop bound; elsewhere the top-word test, exact
+  // too, only slower on such ranges).
+  if constexpr (!LA && !BOUNDED) {
+    if (lowkey && limbs == 1) {
+#define CZ_LOWKEY(UC)                                                                        \
+  hipLaunchKernelGGL((k_verify_persistent<3, false, false, false, UC, true>), grid, block, 0, s, \
+                     sv, rows, tiles, order, ctl, lim, pow3, tile_recs, lists, U)
+      if (U == 1) CZ_LOWKEY(1);
+      else if (U == 2) CZ_LOWKEY(2);
+      else if (U == 3) CZ_LOWKEY(3);
+      else CZ_LOWKEY(0);
+#undef CZ_LOWKEY
+      return;
+    }
+  }
   // The hot forms (two-limb work below 2^68, with or without the look-ahead; without it only
   // where no loop bound is set): the tuner's checks per refill compiled in (its grid is 1..3,
   // and 1..4 with the look-ahead).
@@ -998,34 +1068,34 @@ static void launch_persistent_lb(int limbs, const SieveArrays& sv, const RowDesc
 template <bool LA>
 static void launch_persistent_la(int limbs, const SieveArrays& sv, const RowDesc* rows,
                                  const TileDesc* tiles, const uint32_t* order, Ctl* ctl,
-                                 uint32_t nblocks, uint32_t U, const Limits& lim,
+                                 uint32_t nblocks, uint32_t U, bool lowkey, const Limits& lim,
                                  const uint64_t* pow3, DevRec* tile_recs, DeviceLists lists,
                                  hipStream_t s) {
   // (The look-ahead form always carries the bound.)
   if constexpr (LA) {
-    launch_persistent_lb<true, true>(limbs, sv, rows, tiles, order, ctl, nblocks, U, lim,
-                                     pow3, tile_recs, lists, s);
+    launch_persistent_lb<true, true>(limbs, sv, rows, tiles, order, ctl, nblocks, U, lowkey,
+                                     lim, pow3, tile_recs, lists, s);
   } else {
     if (lim.max_loop < kMaxLoop)
-      launch_persistent_lb<false, true>(limbs, sv, rows, tiles, order, ctl, nblocks, U, lim,
-                                        pow3, tile_recs, lists, s);
+      launch_persistent_lb<false, true>(limbs, sv, rows, tiles, order, ctl, nblocks, U, lowkey,
+                                        lim, pow3, tile_recs, lists, s);
     else
-      launch_persistent_lb<false, false>(limbs, sv, rows, tiles, order, ctl, nblocks, U, lim,
-                                         pow3, tile_recs, lists, s);
+      launch_persistent_lb<false, false>(limbs, sv, rows, tiles, order, ctl, nblocks, U,
+                                         lowkey, lim, pow3, tile_recs, lists, s);
   }
 }
 
 hipError_t launch_verify_persistent(int limbs, const SieveArrays& sv, const RowDesc* rows,
                                     const TileDesc* tiles, const uint32_t* order, Ctl* ctl,
-                                    uint32_t nblocks, uint32_t U, const Limits& lim,
-                                    const uint64_t* pow3, DevRec* tile_recs, DeviceLists lists,
-                                    hipStream_t s) {
+                                    uint32_t nblocks, uint32_t U, bool lowkey,
+                                    const Limits& lim, const uint64_t* pow3, DevRec* tile_recs,
+                                    DeviceLists lists, hipStream_t s) {
   if (sv.la)
-    launch_persistent_la<true>(limbs, sv, rows, tiles, order, ctl, nblocks, U, lim, pow3,
-                               tile_recs, lists, s);
+    launch_persistent_la<true>(limbs, sv, rows, tiles, order, ctl, nblocks, U, lowkey, lim,
+                               pow3, tile_recs, lists, s);
   else
-    launch_persistent_la<false>(limbs, sv, rows, tiles, order, ctl, nblocks, U, lim, pow3,
-                                tile_recs, lists, s);
+    launch_persistent_la<false>(limbs, sv, rows, tiles, order, ctl, nblocks, U, lowkey, lim,
+                                pow3, tile_recs, lists, s);
   return hipGetLastError();
 }
 

@@ -146,11 +146,11 @@ int collatz_verify_range(const collatz_sieve* sieve, const uint64_t lo[2], const
  * collatz_verify_range (opts->form must be COLLATZ_FORM_PLAIN) with identical results, whose
  * buffers come from a fixed device array of about 20 MiB per device, held from library load
  * until the process ends; the call itself allocates no device memory.  Limits per call: at
- * most 4096 chunk records, 32768 (q, chunk, residue class) rows and 98304 kernel blocks (a
- * range past them fails with COLLATZ_EINVAL), and at most 4096 overflow starts listed
- * (overflow_written is exact; more starts than that return COLLATZ_ECAPACITY).  Calls on one
- * device share the array and are serialised by a lock; work enqueued on it by other means
- * would race. */
+ * most 4096 chunk records and 32768 (q, chunk, residue class) rows (a range past them fails
+ * with COLLATZ_EINVAL; the kernel's 98304 share records are sized to fit), and at most 4096
+ * overflow starts listed (overflow_written is exact; more starts than that return
+ * COLLATZ_ECAPACITY).  Calls on one device share the array and are serialised by a lock;
+ * work enqueued on it by other means would race. */
 int collatz_verify_range_noalloc(const collatz_sieve* sieve, const uint64_t lo[2],
                                  const uint64_t hi[2], const collatz_verify_opts* opts,
                                  void* stream, collatz_range_result* out);

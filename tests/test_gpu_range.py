@@ -125,6 +125,23 @@ def test_narrow_width_escalation(s20, o20, oracle):
     assert int(outs[0].total["max_exc"][1]) >> 27 != 0
 
 
+@pytest.mark.parametrize("limbs,cutoff", [(2, 187), (2, 100), (2, 64), (3, 187)])
+def test_low_ranges(s20, o20, oracle, limbs, cutoff):
+    """Ranges ending at or below 2^35 take the low-key test of the wave max (the plain
+    kernel on three words, or four under a cutoff below 2^91; the persistent kernel on three
+    words): records equal to the oracle's in both forms."""
+    check(s20, o20, oracle, 3, (1 << 24) + 12345, 20, limbs=limbs, cutoff=cutoff)
+
+
+def test_low_range_excursion_past_2p64(s20, o20, oracle):
+    """n = 23035537407 (< 2^35) climbs to 34419078320774113520 > 2^64 in the shifted map:
+    under the low-key test the wave's best then has a nonzero upper word, and every value
+    below 2^64 must stop reaching it."""
+    n = 23035537407
+    outs, total = check(s20, o20, oracle, n - (1 << 20), n + (1 << 20), 20)
+    assert int(total["max_exc"][1]) == 1 and int(total["max_start"][0]) == n
+
+
 @pytest.mark.parametrize("lo,hi", [
     ((1 << 68) - (1 << 22), 1 << 68),             # last range on 96 bits
     ((1 << 68) - (1 << 21), (1 << 68) + (1 << 21)),  # first above: two u64 limbs
