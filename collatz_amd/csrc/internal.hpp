@@ -290,7 +290,9 @@ hipError_t launch_la_build(uint32_t B, uint32_t k, uint32_t sem, uint32_t* table
 hipError_t launch_leaf_bins(const uint32_t* meta, uint64_t n, uint32_t tpo,
                             unsigned long long* bins, hipStream_t s);
 // Reference mode: Steps 2 and 3 (rust: Step 3 is main.rs's search3), and the wide rerun of
-// escalated candidates.  pow3w holds 3^f for f < 64 as (low, high) word pairs.
+// escalated candidates.  pow3w holds 3^f for f < 64 as (low, high) word pairs.  U checks run
+// between refills (kRefChecksPerRefill, compiled in on 96 bits, unless told otherwise).
+constexpr uint32_t kRefChecksPerRefill = 5;
 hipError_t launch_ref_fused(int limbs, bool rust, const RefTablesDev& T, const RefLeaves& lv,
                             uint64_t nleaves, uint32_t mbi, const Limits& lim,
                             const uint64_t* pow3w, uint32_t tile, uint32_t U, uint32_t nblocks,

@@ -413,8 +413,8 @@ extern "C" int collatz_reference_run(const collatz_ref_params* params, const col
         CZ_HIP(hipMemcpyAsync(&dtot.p->spot_rec, &f, sizeof(f), hipMemcpyHostToDevice, s));
       }
       CZ_HIP(launch_ref_fused(width, rust, T->dev, lv, ls.count, mbi, lim, T->pow3w.p, kTile,
-                              5, nblocks, dtot.p, ovf_cap ? dovf.p : nullptr, ovf_cap, desc.p,
-                              esc_cap, s));
+                              kRefChecksPerRefill, nblocks, dtot.p, ovf_cap ? dovf.p : nullptr,
+                              ovf_cap, desc.p, esc_cap, s));
       unsigned long long nesc = 0;
       CZ_HIP(hipMemcpyAsync(&nesc, &dtot.p->esc_count, 8, hipMemcpyDeviceToHost, s));
       CZ_HIP(hipStreamSynchronize(s));

@@ -399,7 +399,7 @@ constexpr uint32_t kRefQueue = 512;  // queue entries per wave: (leaf, a)
 // or leaves; the exit tests are ballots (uniform).
 // Eight waves per SIMD (the scalar registers this asks for are what the kernel had before
 // its deep-leaf, main.rs and spot-check paths; without the limit it took seven).
-template <int L, bool LOWCUT, bool RUST, bool WIDE, bool BOUNDED>
+template <int L, bool LOWCUT, bool RUST, bool WIDE, bool BOUNDED, int UC = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 3 && !RUST ? 8 : 1))) void k_ref_fused(RefTablesDev T, RefLeaves lv,
                                                    uint64_t nleaves, uint32_t mbi, Limits lim,
                                                    const uint64_t* pow3w, uint32_t tile,
@@ -563,7 +563,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 3 && !
       }
       // (3) Up to U checks; a lane whose candidate ends before the last of them starts its
       //     prefetched candidate at once.
-      for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t Un = UC > 0 ? (uint32_t)UC : U;  // (a constant when UC is set)
+#pragma unroll
+      for (uint32_t u = 0; u < Un; ++u) {
         if (act) {
           bool done, stop = false;
           if constexpr (RUST) {
@@ -620,7 +622,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 3 && !
             }
           }
         }
-        if (u + 1 < U && !act && pf) start_prefetched();
+        if (u + 1 < Un && !act && pf) start_prefetched();
       }
     }
     // A sampled tile (spot check) keeps its own results as well.  (t is uniform.)
@@ -751,7 +753,12 @@ static void launch_fused_sched(int limbs, const RefTablesDev& T, const RefLeaves
                                uint64_t ovf_cap, uint64_t* esc, uint64_t esc_cap,
                                hipStream_t s) {
   const dim3 grid(nblocks), block(256);
-  if (limbs == 1)
+  // On 96 bits the checks between refills are compiled in for the launches' usual five.
+  if (limbs == 1 && U == kRefChecksPerRefill)
+    hipLaunchKernelGGL((k_ref_fused<3, false, RUST, WIDE, BOUNDED, kRefChecksPerRefill>), grid,
+                       block, 0, s, T, lv, nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap,
+                       esc, esc_cap);
+  else if (limbs == 1)
     hipLaunchKernelGGL((k_ref_fused<3, false, RUST, WIDE, BOUNDED>), grid, block, 0, s, T, lv,
                        nleaves, mbi, lim, pow3w, tile, U, tot, ovf, ovf_cap, esc, esc_cap);
   else if (limbs == 2)
