@@ -508,9 +508,12 @@ def c0_leg(cz, steps, keep=None):
         ms.append(o.device_ms)
     d = (time.perf_counter() - t0) / steps
     dms = sum(ms) / len(ms)
+    ach = (int(o.checks) * ops_dr(3) + int(o.total["count"]) * ops_setup(3)) / \
+        (dms * 1e-3) / 1e12
     out = {"range": "[1, 2^32)", "k": 20, "chunk_log2": 24, "form": "plain",
            "device_ms": dms, "ms_per_call": d * 1e3, "n_per_s": ((1 << 32) - 1) / (dms * 1e-3),
            "host_inclusive_n_per_s": ((1 << 32) - 1) / d, "candidates": int(o.total["count"]),
+           "checks": int(o.checks), "roofline_frac": ach / VALU_PEAK_TOPS,
            "hash": "%08X" % int(o.total["hash"])}
     if keep is not None:
         keep["c0_chunks"] = o.chunks.copy()
@@ -560,6 +563,8 @@ def main():
     ap.add_argument("--only-range", default=None, metavar="FORMS",
                     help="run only the [2^40, 2^40+2^36) range leg in the given forms "
                          "(comma-separated: plain, persistent) and print it")
+    ap.add_argument("--only-c0", action="store_true",
+                    help="run only the [1, 2^32) leg at k = 20 (plain form) and print it")
     ap.add_argument("--only-large", action="store_true",
                     help="run only the k = 37 leg ([2^60, 2^60+2^44)) and print it")
     ap.add_argument("--large-steps", type=int, default=3)
@@ -598,6 +603,9 @@ def main():
 
     if args.only_large:
         print(json.dumps(large_leg(cz, args.large_steps)), flush=True)
+        return
+    if args.only_c0:
+        print(json.dumps(c0_leg(cz, args.range_steps)), flush=True)
         return
     if args.only_range:
         s32 = cz.Sieve(K)
