@@ -91,20 +91,28 @@ int verify_plain_with(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs
   // check at which every lane offers its value for the max, and an end at which lanes idle
   // while the share's last trajectories finish, together about the work of a few hundred
   // survivors; so S is at least 16384 (most rows of a 2^24-wide chunk at k = 32 are one
-  // share), but small enough that every resident wave has about four shares to take, and
-  // large enough that the shares fit the record buffer.  ([2^40, 2^40 + 2^36), k = 32: each
-  // row split over the four waves of a block ran 1.55 ms.)
+  // share), but small enough that every resident wave has about four shares to take.  The
+  // rows of the last quarter of the survivors, whose shares the launch's last blocks take
+  // (blocks start in index order), are cut at S / 2, so that the launch ends on shorter
+  // shares.  ([2^40, 2^40 + 2^36), k = 32: each row split over the four waves of a block
+  // ran 1.55 ms; one share per row 1.38-1.43 ms; with the shorter last quarter 1.34 ms, and
+  // the 4096 intervals' plain calls 5% less device time.)  The shares must also fit the
+  // record buffer: at most 1.25 slots / S + rows of them.
   uint64_t slots = 0;
   for (const RowDesc& r : plan.rows) slots += r.i_hi - r.i_lo;
   const uint64_t resident = (uint64_t)device_cu_count(sv->device) * 32;
   uint64_t S = std::max<uint64_t>(slots / (4 * resident),
                                   std::min<uint64_t>(16384, std::max<uint64_t>(1, slots / resident)));
   const uint64_t cap = B.max_shares() > plan.rows.size() ? B.max_shares() - plan.rows.size() : 1;
-  S = std::max<uint64_t>({S, slots ? (slots - 1) / cap + 1 : 1, 1});
-  uint64_t nshares = 0;
+  S = std::max<uint64_t>({S, slots ? (slots * 5 / 4) / cap + 2 : 1, 2});
+  const uint64_t tail_from = slots - slots / 4;
+  uint64_t nshares = 0, seen = 0;
   for (RowDesc& r : plan.rows) {
     r.block_start = nshares;
-    nshares += (r.i_hi - r.i_lo + S - 1) / S;
+    const uint64_t len = r.i_hi - r.i_lo;
+    const uint64_t piece = seen >= tail_from ? S / 2 : S;
+    nshares += (len + piece - 1) / piece;
+    seen += len;
   }
   std::vector<uint64_t> part_start(plan.nchunks + 1);
   for (uint64_t c = 0; c <= plan.nchunks; ++c) {
