@@ -439,15 +439,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 3 && !
     bool pf = false;
     uint32_t pf_leaf = 0, pf_a = 0, pf_meta = 0;
     uint64_t pf_lb = 0, pf_ln = 0, pf_lnh = 0;
+    // Converted candidate of the lane, ready to start: the prefetched one's start value, n,
+    // leaf and a (bit 31 of ps_a: the strict overflow compare; a < 2^12).  A refill converts
+    // the prefetched candidate of each lane that has none, one refill after its loads were
+    // issued, so a start within the window is a few register moves for the lanes that start
+    // (with per-lane flags, the code of a start runs whenever any lane starts one): the tree
+    // below 2^48 1.72 -> 1.59 s.  main.rs's blocks (RUST) start the prefetched candidate
+    // directly: there the converted one's registers cost more than the moves save (64-66
+    // VGPRs, 2% slower).
+    constexpr bool kConvert = !RUST;
+    bool ps = false;
+    Num<L> ps_v;
+    uint64_t ps_n = 0;
+    uint32_t ps_leaf = 0, ps_a = 0;
     // Running candidate of the lane.
     bool act = false, strict = false;
     Num<L> v;
     uint64_t n_lo = 0;
     uint32_t checks = 0, ch = 0, cur_a = 0, cur_leaf = 0;
-    auto escalate = [&]() {
+    auto push_esc = [&](uint32_t leaf, uint32_t a, bool st) {
       const unsigned long long i = atomicAdd(&tot->esc_count, 1ull);
-      if (i < esc_cap)
-        esc[i] = cur_leaf | ((uint64_t)cur_a << 32) | (strict ? (1ull << 48) : 0ull);
+      if (i < esc_cap) esc[i] = leaf | ((uint64_t)a << 32) | (st ? (1ull << 48) : 0ull);
+    };
+    auto escalate = [&]() {
+      push_esc(cur_leaf, cur_a, strict);
       cnt -= 1;
       act = false;
     };
@@ -458,8 +473,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 3 && !
                                        : (unsigned __int128)pf_a * p3w[2 * f] + pf_ln;
       set_words<L>(v, x);
       n_lo = pf_lb + ((uint64_t)pf_a << mbi);
-      // Leaves without a bit vector run on the CPU in the reference (search2/search3,
-      // collatz_cudav7.cu:517-519), whose second overflow test is '>' (:702).
       strict = (f - T.tpo) >= 16;
       cur_leaf = pf_leaf;
       cur_a = pf_a;
@@ -467,10 +480,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 3 && !
       ch = 0;
       act = true;
       cnt += 1;
-      // A start at or above the 96-bit check bound goes to the wide pass at once.
       if constexpr (L == 3) {
         if ((x >> kNarrowCheckLog2) != 0) escalate();
       }
+    };
+    auto convert_prefetched = [&]() {
+      pf = false;
+      const uint32_t f = meta_f(pf_meta);
+      const unsigned __int128 x = WIDE ? ref_start(pf_ln, pf_lnh, pf_a, p3w, f)
+                                       : (unsigned __int128)pf_a * p3w[2 * f] + pf_ln;
+      // Leaves without a bit vector run on the CPU in the reference (search2/search3,
+      // collatz_cudav7.cu:517-519), whose second overflow test is '>' (:702).
+      const bool st = (f - T.tpo) >= 16;
+      // A start at or above the 96-bit check bound goes to the wide pass at once (it is not
+      // counted here: the wide pass counts it).
+      if constexpr (L == 3) {
+        if ((x >> kNarrowCheckLog2) != 0) {
+          push_esc(pf_leaf, pf_a, st);
+          return;
+        }
+      }
+      set_words<L>(ps_v, x);
+      ps_n = pf_lb + ((uint64_t)pf_a << mbi);
+      ps_leaf = pf_leaf;
+      ps_a = pf_a | (st ? 0x80000000u : 0u);
+      ps = true;
+    };
+    auto start_converted = [&]() {
+      ps = false;
+      v = ps_v;
+      n_lo = ps_n;
+      cur_leaf = ps_leaf;
+      cur_a = ps_a & 0x7FFFFFFFu;
+      strict = (ps_a >> 31) != 0;
+      checks = 0;
+      ch = 0;
+      act = true;
+      cnt += 1;
     };
     // Iterations of the tile's loop (and of the take loop per refill), uniform: a wave
     // that reaches the bound abandons the tile; the abort is counted and the call fails.
@@ -481,8 +527,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 3 && !
         aborted = true;
         break;
       }
-      // (1) Lanes without a candidate start their prefetched one.
-      if (!act && pf) start_prefetched();
+      // (1) Lanes without a candidate start their converted one; lanes without a converted
+      //     one convert their prefetched one, and those still without a candidate (at the
+      //     start of a tile, or after two candidates ended within the last window) start it.
+      if constexpr (kConvert) {
+        if (!act && ps) start_converted();
+        if (!ps && pf) convert_prefetched();
+        if (!act && ps) start_converted();
+      } else {
+        if (!act && pf) start_prefetched();
+      }
       // (2) Lanes without a prefetch take the next queue entries; an empty queue is
       //     refilled from the next 64 mask words of the tile.
       const uint64_t need = __ballot(!pf);
@@ -553,7 +607,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 3 && !
           wcur += nfit;
         }
       }
-      if (__ballot(act || pf) == 0) break;
+      if (__ballot(act || ps || pf) == 0) break;
       // A candidate past the check bound is abandoned (tested once per refill: it may run
       // up to U checks beyond the bound first).
       if (act && __builtin_expect(checks >= lim.max_checks, 0)) {
@@ -622,7 +676,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 3 && !
             }
           }
         }
-        if (u + 1 < Un && !act && pf) start_prefetched();
+        if constexpr (kConvert) {
+          if (u + 1 < Un && !act && ps) start_converted();
+        } else {
+          if (u + 1 < Un && !act && pf) start_prefetched();
+        }
       }
     }
     // A sampled tile (spot check) keeps its own results as well.  (t is uniform.)
