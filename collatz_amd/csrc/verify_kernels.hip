@@ -81,8 +81,8 @@ __device__ __forceinline__ void words_apart(uint32_t* x) {
   for (int i = 0; i < L; ++i) asm volatile("" : "+v"(x[i]));
 }
 
-// Max of a whole wave, for the persistent kernel: the wave's best (value, start) is
-// uniform (scalar registers).  The offer runs in uniform control flow on every lane: a
+// Max of a whole wave, for the persistent and plain kernels: the wave's best (value, start)
+// is uniform (scalar registers).  The offer runs in uniform control flow on every lane: a
 // lane's value is compared with the top 64 bits of the wave's best, and only when some lane
 // of `live` (the lanes that hold a candidate, set by the caller before each check) reaches
 // them, rare once the best is high, are the reaching lanes merged one at a time by uniform
@@ -421,9 +421,9 @@ __device__ __forceinline__ bool run_candidate(Cand<L>& c, Acc<L>& acc, const Lim
 // one).  No state outlives the launch and no work moves between waves: this is the drop-in
 // kernel of collatz_verify_range.  A lane runs one candidate at a time, two checks per loop
 // iteration; a lane whose candidate has ended (descended, escalated, or abandoned at the
-// step bound) starts the survivor record it loaded earlier, and the lanes
-// without a loaded record take the next survivors of the wave's share by rank among them,
-// so no lane waits for its wave's slowest trajectory until the share is used up.  As in the
+// step bound) starts the survivor record it loaded earlier, and the lanes without a loaded
+// record take the next survivors of the wave's share by rank among them, so no lane waits
+// for its wave's slowest trajectory until the share is used up.  As in the
 // persistent kernel below, which lanes run a candidate and which hold a record are two
 // uniform masks (scalar registers), every lane runs the check (a lane without a candidate
 // on stale values, its effects dropped), the wave's max excursion is kept in scalar
