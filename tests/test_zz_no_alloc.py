@@ -83,3 +83,26 @@ def test_noalloc_limits():
     b = run(L.collatz_verify_range, sv, lo + 7, lo + (1 << 22), 20, 187, 16)
     assert a[0] == b[0] == cz.OK
     assert_chunks_equal(a[3], b[3], "after the limits")
+
+
+def test_noalloc_at_the_chunk_limit(oracle):
+    """4096 chunk records of 2^16 at k = 16 (24,576 rows, one per chunk and residue class):
+    the most rows a call of the limit holds, whose shares the plain form sizes to fit the
+    array's share records; equal to the plain form, and to the oracle on its first and last
+    chunks."""
+    import collatz_amd as cz
+    L = cz.load()
+    sv = cz.Sieve(16)
+    lo = (1 << 40) + 1
+    hi = (1 << 40) + (4096 << 16)
+    print(f"noalloc case [{lo:#x}, {hi:#x}) chunk 2^16, 4096 records", flush=True)
+    rc1, got, gtot, gchunks, _ = run(L.collatz_verify_range_noalloc, sv, lo, hi, 16, 187, 16)
+    rc0, want, wtot, wchunks, _ = run(L.collatz_verify_range, sv, lo, hi, 16, 187, 16)
+    assert rc0 == rc1 == cz.OK, L.collatz_last_error()
+    assert len(wchunks) == 4096
+    assert_chunks_equal(gchunks, wchunks, "noalloc at the chunk limit")
+    assert_total_equal(gtot, wtot, "noalloc total at the chunk limit")
+    o16 = oracle.sieve(16)
+    for a, b, idx in ((lo, (1 << 40) + (1 << 16), 0), ((1 << 40) + (4095 << 16), hi, 4095)):
+        ochunks, _ = oracle.verify_range(o16, a, b, 16)
+        assert_chunks_equal(gchunks[idx:idx + 1], ochunks, f"chunk {idx} against the oracle")
