@@ -134,12 +134,15 @@ static void server_loop(collatz_server* S) {
       // Continuous batching: every admitted request gets an equal share of the launch's work
       // budget, and at least one row, so a short request finishes within one launch however
       // long the requests ahead of it are; what the shares leave goes to the oldest.  The
-      // budget is slice_candidates, raised up to 16x while the admitted backlog is large
-      // (a quarter of it per launch), so a saturated server runs fewer, longer launches.
+      // budget is slice_candidates, raised up to 64x while the admitted backlog is large
+      // (a quarter of it per launch), so a saturated server runs fewer, longer launches
+      // (the whole request stream of seed 1 submitted at once: 5.8e13 n/s with at most 16x,
+      // 6.1e13 with 64x; at 500 requests/s, where the backlog stays small, p50 and p99 are
+      // the same).
       uint64_t backlog = 0;
       for (const Request* r : S->active) backlog += r->est_left;
       const uint64_t budget = std::min<uint64_t>(
-          std::max<uint64_t>(S->opts.slice_candidates, backlog / 4), 16 * S->opts.slice_candidates);
+          std::max<uint64_t>(S->opts.slice_candidates, backlog / 4), 64 * S->opts.slice_candidates);
       uint64_t taken = 0;
       launch_id = ++S->launches;
       auto take = [&](Request* r, uint64_t want, bool one_row_always) {

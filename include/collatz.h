@@ -254,7 +254,7 @@ typedef struct {
   uint32_t chunk_log2;       /* 0 = 24 */
   uint32_t cutoff_log2;      /* 0 = 187 */
   uint64_t max_device_bytes; /* admission budget for per-request buffers; 0 = 1 GiB */
-  uint64_t slice_candidates; /* candidates per launch (up to 16x more while the backlog is
+  uint64_t slice_candidates; /* candidates per launch (up to 64x more while the backlog is
                                 large); 0 = 2^26 */
   uint32_t max_inflight;     /* admitted requests at once; 0 = 1024 */
   uint32_t reserved;
