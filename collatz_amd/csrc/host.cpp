@@ -251,10 +251,10 @@ int build_leaves(const SieveRoot& root, hipStream_t s, LeafSet& out) {
       }
     }
     out.count = r.size();
-    CZ_HIP(out.res.alloc(std::max<size_t>(1, r.size())));
-    CZ_HIP(out.num.alloc(std::max<size_t>(1, r.size())));
-    CZ_HIP(out.num_hi.alloc(out.wide ? std::max<size_t>(1, r.size()) : 1));
-    CZ_HIP(out.meta.alloc(std::max<size_t>(1, r.size())));
+    CZ_HIP(out.res.reserve(std::max<size_t>(1, r.size())));
+    CZ_HIP(out.num.reserve(std::max<size_t>(1, r.size())));
+    CZ_HIP(out.num_hi.reserve(out.wide ? std::max<size_t>(1, r.size()) : 1));
+    CZ_HIP(out.meta.reserve(std::max<size_t>(1, r.size())));
     if (!r.empty()) {
       CZ_HIP(hipMemcpyAsync(out.res.p, r.data(), r.size() * 8, hipMemcpyHostToDevice, s));
       CZ_HIP(hipMemcpyAsync(out.num.p, n.data(), n.size() * 8, hipMemcpyHostToDevice, s));
@@ -268,14 +268,15 @@ int build_leaves(const SieveRoot& root, hipStream_t s, LeafSet& out) {
   }
   const uint32_t p = std::min<uint32_t>(e, 20);
   const uint64_t np = 1ull << p;
-  DevBuf<uint64_t> pnum, pnum_hi;
-  DevBuf<uint32_t> pmeta;
-  DevBuf<uint64_t> alive;
-  CZ_HIP(pnum.alloc(np));
-  CZ_HIP(pnum_hi.alloc(out.wide ? np : 1));
-  CZ_HIP(pmeta.alloc(np));
+  DevBuf<uint64_t>& pnum = out.pnum;
+  DevBuf<uint64_t>& pnum_hi = out.pnum_hi;
+  DevBuf<uint32_t>& pmeta = out.pmeta;
+  DevBuf<uint64_t>& alive = out.alive;
+  CZ_HIP(pnum.reserve(np));
+  CZ_HIP(pnum_hi.reserve(out.wide ? np : 1));
+  CZ_HIP(pmeta.reserve(np));
   const uint64_t alive_words64 = round_up(std::max<uint64_t>(1, np / 64), 512);
-  CZ_HIP(alive.alloc(alive_words64));
+  CZ_HIP(alive.reserve(alive_words64));
   CZ_HIP(hipMemsetAsync(alive.p, 0, alive_words64 * 8, s));
   CZ_HIP(launch_sieve_prefix(root, p, pnum.p, pnum_hi.p, pmeta.p, alive.p, s));
   const uint32_t* bits32;
@@ -283,33 +284,33 @@ int build_leaves(const SieveRoot& root, hipStream_t s, LeafSet& out) {
   if (e > p) {
     const uint64_t words = 1ull << (e - 5);
     nwords32 = round_up(words, 1024);
-    CZ_HIP(out.bits.alloc(nwords32));
+    CZ_HIP(out.bits.reserve(nwords32));
     if (nwords32 > words)
       CZ_HIP(hipMemsetAsync(out.bits.p + words, 0, (nwords32 - words) * 4, s));
     CZ_HIP(launch_sieve_expand(root, p, pnum.p, pnum_hi.p, pmeta.p, alive.p, out.bits.p, s));
     bits32 = out.bits.p;
   } else {
     nwords32 = alive_words64 * 2;
-    CZ_HIP(out.bits.alloc(nwords32));
+    CZ_HIP(out.bits.reserve(nwords32));
     CZ_HIP(hipMemcpyAsync(out.bits.p, alive.p, nwords32 * 4, hipMemcpyDeviceToDevice, s));
     bits32 = out.bits.p;
   }
   out.bits_bits = 1ull << e;
   const uint64_t nblocks = nwords32 / 1024;
-  DevBuf<uint32_t> counts;
-  DevBuf<uint64_t> offs;
-  CZ_HIP(counts.alloc(nblocks));
-  CZ_HIP(offs.alloc(nblocks + 1));
+  DevBuf<uint32_t>& counts = out.counts;
+  DevBuf<uint64_t>& offs = out.offs;
+  CZ_HIP(counts.reserve(nblocks));
+  CZ_HIP(offs.reserve(nblocks + 1));
   CZ_HIP(launch_compact_count(bits32, nwords32, counts.p, nblocks, s));
   CZ_HIP(launch_scan(counts.p, nblocks, offs.p, s));
   uint64_t total = 0;
   CZ_HIP(hipMemcpyAsync(&total, offs.p + nblocks, 8, hipMemcpyDeviceToHost, s));
   CZ_HIP(hipStreamSynchronize(s));
   out.count = total;
-  CZ_HIP(out.res.alloc(std::max<uint64_t>(1, total)));
-  CZ_HIP(out.num.alloc(std::max<uint64_t>(1, total)));
-  CZ_HIP(out.num_hi.alloc(out.wide ? std::max<uint64_t>(1, total) : 1));
-  CZ_HIP(out.meta.alloc(std::max<uint64_t>(1, total)));
+  CZ_HIP(out.res.reserve(std::max<uint64_t>(1, total)));
+  CZ_HIP(out.num.reserve(std::max<uint64_t>(1, total)));
+  CZ_HIP(out.num_hi.reserve(out.wide ? std::max<uint64_t>(1, total) : 1));
+  CZ_HIP(out.meta.reserve(std::max<uint64_t>(1, total)));
   CZ_HIP(launch_compact_write(bits32, nwords32, offs.p, out.res.p, s));
   CZ_HIP(launch_sieve_fill(root, p, pnum.p, pnum_hi.p, pmeta.p, out.res.p, out.num.p,
                            out.num_hi.p, out.meta.p, total, s));

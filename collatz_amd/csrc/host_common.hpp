@@ -212,6 +212,11 @@ struct LeafSet {
   DevBuf<uint32_t> bits;   // dense bit array over the extensions h (padded)
   uint64_t bits_bits = 0;
   uint64_t count = 0;
+  // Build scratch (prefix states, alive bits, per-block counts and offsets).  Every buffer
+  // of a LeafSet only grows, so a LeafSet reused for many builds allocates nothing once it
+  // has held the largest; sizes are the build's, not the buffers'.
+  DevBuf<uint64_t> pnum, pnum_hi, alive, offs;
+  DevBuf<uint32_t> pmeta, counts;
 };
 int build_leaves(const SieveRoot& root, hipStream_t s, LeafSet& out);
 

@@ -55,6 +55,42 @@ def test_case_lines_cuda_semantics_match_oracle(oracle, bv_cuda_n45):
             o.complete_hash, o.launches, o.candidates, o.leaves), t
 
 
+def test_calls_from_threads_and_reused_buffers():
+    """Case lines of different sizes one after another (the device's kept buffers grow and
+    are reused), then the same lines from eight threads at once (one call holds the kept
+    buffers, the others take their own): every result equals the first pass's (which
+    test_case_lines_cuda_semantics_match_oracle and the deep-line tests pin to the oracle)."""
+    import threading
+    import collatz_amd as cz
+    lines = read_cases("cases_big_sample.txt")[:8]
+    # a small line after the largest, a deep (128-bit leaf) line between two 64-bit ones
+    runs = [(t, 45) for t in lines[:4]] + [(lines[4], 52), (lines[5], 45), (lines[6], 52),
+                                           (lines[7], 45)]
+    key = lambda r: (r.complete_hash, r.candidates, r.leaves, r.checks, r.launches,
+                     r.overflow_count)
+    want = [key(cz.reference_run(t, n, 10, semantics=cz.SEM_CUDA)) for t, n in runs]
+    again = [key(cz.reference_run(t, n, 10, semantics=cz.SEM_CUDA)) for t, n in reversed(runs)]
+    assert again[::-1] == want
+    got = [None] * len(runs)
+    errs = []
+
+    def run(i):
+        try:
+            t, n = runs[i]
+            got[i] = key(cz.reference_run(t, n, 10, semantics=cz.SEM_CUDA))
+        except Exception as e:  # reported below, on the test's thread
+            errs.append((i, e))
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(len(runs))]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=100)
+    assert not any(th.is_alive() for th in threads)
+    assert not errs, errs
+    assert got == want
+
+
 @pytest.mark.parametrize("N,expected", [(35, 31109345), (36, 56414851), (37, 106551583),
                                         (38, 205942265), (39, 390055737), (40, 756583624),
                                         (41, 1408862059), (42, 2658801769), (43, 5143671444),
