@@ -123,6 +123,18 @@ struct RowPlan {
   std::vector<uint64_t> n_checked;  // per chunk
   uint64_t first_chunk = 0;
   uint64_t nchunks = 0;
+  // Empties the plan and keeps its vectors' capacity (a plan kept between calls reallocates
+  // nothing once it has held the largest).
+  void clear() {
+    rows.clear();
+    keys.clear();
+    key_cls.clear();
+    key_lo.clear();
+    key_hi.clear();
+    chunk_first_row.clear();
+    n_checked.clear();
+    first_chunk = nchunks = 0;
+  }
 };
 
 int plan_rows(uint32_t k, u128 lo, u128 hi, uint32_t C, uint32_t slot_base, bool mod9,
@@ -139,6 +151,7 @@ struct PlainScratch {
   DevBuf<unsigned long long> cnt;
   DevBuf<uint32_t> stall;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  RowPlan plan;  // host-side row plan, kept for its capacity
   PlainScratch() = default;
   PlainScratch(const PlainScratch&) = delete;
   PlainScratch& operator=(const PlainScratch&) = delete;
@@ -188,6 +201,11 @@ class PlainBuffers {
   virtual int launch(uint64_t nrows, uint64_t nshares, uint64_t nchunks, uint64_t ovf_cap,
                      bool esc, PlainView& v) = 0;
   virtual int esc_recs(uint64_t n, PlainView& v) = 0;
+  // A row plan to fill (cleared by the caller); the default is the provider's own.
+  virtual RowPlan& plan() { return plan_; }
+
+ private:
+  RowPlan plan_;
 };
 
 // The plain form over [lo, hi) with the given buffers (the caller serialises calls that

@@ -1,5 +1,6 @@
 // C ABI: collatz_verify_range, the plain single-range form.
 #include <algorithm>
+#include <optional>
 #include <vector>
 
 #include "engine.hpp"
@@ -80,10 +81,15 @@ int run_escalated(const collatz_sieve* sv, const DevBuf<EscEntry>& esc, uint64_t
 int verify_plain_with(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs& a,
                       hipStream_t s, collatz_range_result* out, PlainBuffers& B) {
   PlainView v;
-  RowPlan plan;
+  RowPlan& plan = B.plan();
+  plan.clear();
+  // Host steps under COLLATZ_TRACE=1 (each span ends where the next begins).
+  std::optional<Span> step;
+  step.emplace("plain: plan rows");
   int rc = plan_rows(sv->k, lo, hi, a.C, 0, sv->mod9 != 0, plan, B.max_rows());
   if (rc) return rc;
   if (plan.nchunks > B.max_chunks()) return fail(COLLATZ_EINVAL, "range has too many chunks");
+  step.emplace("plain: resolve rows");
   rc = resolve_rows(sv, plan, B, v, s);
   if (rc) return rc;
   // One wave per share: each row is cut into equal pieces of at most S survivors.  A share
@@ -98,6 +104,7 @@ int verify_plain_with(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs
   // ran 1.55 ms; one share per row 1.38-1.43 ms; with the shorter last quarter 1.34 ms, and
   // the 4096 intervals' plain calls 5% less device time.)  The shares must also fit the
   // record buffer: at most 1.25 slots / S + rows of them.
+  step.emplace("plain: shares, launch");
   uint64_t slots = 0;
   for (const RowDesc& r : plan.rows) slots += r.i_hi - r.i_lo;
   const uint64_t resident = (uint64_t)device_cu_count(sv->device) * 32;
@@ -146,12 +153,14 @@ int verify_plain_with(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs
                              nshares, sv->mod9, lowkey, lim, sv->pow3.p, v.block, lists, s));
   CZ_HIP(launch_reduce_chunks(v.block, v.part, (uint32_t)plan.nchunks, v.chunk, s));
   CZ_HIP(hipEventRecord(v.ev[1], s));
+  step.emplace("plain: wait, records back");
   std::vector<DevRec> recs(plan.nchunks);
   unsigned long long cnt[4];
   CZ_HIP(hipMemcpyAsync(recs.data(), v.chunk, recs.size() * sizeof(DevRec),
                         hipMemcpyDeviceToHost, s));
   CZ_HIP(hipMemcpyAsync(cnt, v.cnt, 32, hipMemcpyDeviceToHost, s));
   CZ_HIP(hipStreamSynchronize(s));
+  step.emplace("plain: chunk records");
   float ms = 0;
   (void)hipEventElapsedTime(&ms, v.ev[0], v.ev[1]);
   if (cnt[3])
@@ -225,6 +234,7 @@ int verify_plain_with(const collatz_sieve* sv, u128 lo, u128 hi, const RangeArgs
 class ScratchBuffers final : public PlainBuffers {
  public:
   explicit ScratchBuffers(PlainScratch& s) : scr_(s) {}
+  RowPlan& plan() override { return scr_.plan; }
   uint64_t max_rows() const override { return 1ull << 26; }
   uint64_t max_chunks() const override { return 0x7FFFFFFFull; }
   uint64_t max_shares() const override { return ~0ull; }
