@@ -186,7 +186,10 @@ int collatz_verify_batch(const collatz_sieve* sieve, uint64_t n,
  * The reference's own full program for one case line "base sigfig threepower number
  * doubleeven numevens" (collatz_cudav7.cu:95-183): search1 to depth N - A, the BV_i
  * look-ahead and mod 9 filter of the top A bits (Step 2, :219-235), the schedule, the
- * complete hash and the launch count it would print. */
+ * complete hash and the launch count it would print.  The library keeps a run's device
+ * buffers (leaf arrays, totals, lists) per device between calls and grows them as needed;
+ * calls from several threads are safe, and one that finds its device's buffers in use
+ * allocates its own for that call. */
 typedef struct {
   uint64_t base[2];
   uint64_t number[2];
